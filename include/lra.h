@@ -1,0 +1,210 @@
+/*
+ * lra.h -- C ABI of liblra_hip.so, the MI355X (gfx950) implementation of LowRankApprox.jl's
+ * sketch-then-partial-factorize path.
+ *
+ * Conventions (shared by every entry point):
+ *   - column-major storage, int64_t sizes and leading dimensions (Julia's BlasInt on ILP64);
+ *   - every matrix/vector pointer is DEVICE memory unless the parameter says "host";
+ *   - pivot / index vectors are 1-based, exactly as the reference hands them around;
+ *   - calls are asynchronous on the context's stream unless documented otherwise; lra_sync
+ *     waits; a context is used by one host thread at a time (contexts are independent);
+ *   - status: 0 = ok, < 0 = -(index of the first invalid argument, 1-based) like LAPACK info,
+ *     > 0 = runtime error (LRA_E_*); lra_last_error() returns a message for the calling thread.
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository, version 0.5.5).
+ */
+#ifndef LRA_H
+#define LRA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LRA_OK = 0,
+  LRA_E_HIP = 1,       /* HIP runtime error */
+  LRA_E_ALLOC = 2,     /* device allocation failed / HBM budget exceeded */
+  LRA_E_UNSUPPORTED = 3, /* shape outside what the kernels were built for */
+  LRA_E_TIMEOUT = 4,   /* a cooperative kernel's bounded wait expired */
+  LRA_E_STATE = 5      /* invalid handle / state */
+};
+
+/* sketch types (LRAOptions.sketch, src/LowRankApprox.jl:86,106) */
+enum { LRA_SKETCH_NONE = 0, LRA_SKETCH_RANDN = 1, LRA_SKETCH_SRFT = 2, LRA_SKETCH_SUB = 3,
+       LRA_SKETCH_SPRN = 4 };
+/* arithmetic of the sketch GEMM (MI355X extension; F64 is the reference behaviour) */
+enum { LRA_PREC_F64 = 0, LRA_PREC_F32 = 1, LRA_PREC_BF16 = 2, LRA_PREC_BF16X3 = 3 };
+/* pqrfact_retval bits ("q", "r", "t"; src/pqr.jl:299-301) */
+enum { LRA_RET_Q = 1, LRA_RET_R = 2, LRA_RET_T = 4 };
+/* trans / side symbols (:n / :c, :left / :right) */
+enum { LRA_TRANS_N = 0, LRA_TRANS_C = 1 };
+enum { LRA_SIDE_LEFT = 0, LRA_SIDE_RIGHT = 1 };
+
+/*
+ * Mirror of LRAOptions (src/LowRankApprox.jl:77-94), same field names and, through
+ * lra_opts_default, the same Float64 defaults (:96-119).  The sketchfact_*_samp functions
+ * are affine maps n -> mul*n + add (defaults n+8, n+8, 4n+8).  The fields after `verb` are
+ * MI355X extensions.
+ */
+typedef struct lra_opts {
+  double atol;
+  int64_t maxdet_niter;
+  double maxdet_tol;
+  int64_t nb;
+  double pheig_orthtol;
+  int32_t pqrfact_retval; /* LRA_RET_* bits */
+  int64_t rank;
+  double rtol;
+  int32_t sketch; /* LRA_SKETCH_* */
+  int64_t sketch_randn_niter;
+  int32_t sketchfact_adap;
+  int64_t randn_samp_mul, randn_samp_add;
+  int64_t srft_samp_mul, srft_samp_add;
+  int64_t sub_samp_mul, sub_samp_add;
+  int64_t snorm_niter;
+  int32_t verb;
+  /* --- extensions --- */
+  int32_t precision; /* LRA_PREC_*: arithmetic of the sketch GEMM */
+  uint64_t seed;     /* key of the counter-based generator (Philox4x32-10) */
+} lra_opts;
+
+typedef struct lra_ctx lra_ctx;
+
+/* ID result (IDPackedV, src/id.jl:15-19).  sk and rd are caller device buffers of N entries
+ * each (N = columns of op(A)); the first k of sk and N-k of rd are written.  T: when the
+ * caller passes T = NULL the library stores T (k x (N-k)) in a buffer it owns, valid until the
+ * next call on the same context, and sets T and ldt = max(k,1); otherwise T is a caller buffer
+ * with ldt >= k and room for ldt*(N-k) doubles.  k and the adaptive round count are written
+ * on the host. */
+typedef struct lra_id_out {
+  int64_t* sk;
+  int64_t* rd;
+  double* T;
+  int64_t ldt;
+  int64_t k;      /* host: rank */
+  int64_t rounds; /* host: adaptive rounds used */
+} lra_id_out;
+
+/* Matrix-free operator (LinearOperator mul!/mulc!, src/linop.jl:9-40,105-113):
+ * Y (rows x nrhs, ldy) = op(L) * X (cols x nrhs, ldx) with op = L (trans 0) or L' (trans 1),
+ * device buffers, work enqueued on `stream` (a hipStream_t). Return 0 on success. */
+typedef int (*lra_apply_fn)(void* user, int trans, int64_t nrhs, const double* X, int64_t ldx,
+                            double* Y, int64_t ldy, void* stream);
+
+/* ---- context ------------------------------------------------------------------------ */
+void lra_opts_default(lra_opts* opts);
+int lra_ctx_create(int device, size_t hbm_budget, lra_ctx** ctx);
+int lra_ctx_destroy(lra_ctx* ctx);
+/* Enqueue all work of ctx on `stream` (a hipStream_t; NULL = the null stream).  A new
+ * context starts on a non-blocking stream of its own. */
+int lra_ctx_set_stream(lra_ctx* ctx, void* stream);
+int lra_sync(lra_ctx* ctx);
+const char* lra_last_error(void);
+/* Device properties the host side needs (CUs, LDS per CU). */
+int lra_device_info(lra_ctx* ctx, int64_t* num_cu, int64_t* lds_bytes);
+
+/* ---- primitives --------------------------------------------------------------------- */
+
+/* Standard-normal fill of a rows x cols column-major matrix from the counter-based
+ * generator: element (i,j) is normal #(j*rows+i) of stream `stream_id` under `seed`.
+ * Replaces crandn (src/util.jl:4) behind RandomGaussian (src/sketch.jl:91-110). */
+int lra_gaussian_fill(lra_ctx* ctx, uint64_t seed, uint64_t stream_id, int64_t rows,
+                      int64_t cols, double* out, int64_t ld);
+
+/* Gaussian sketch B = Omega*op(A) (side left) or op(A)*Omega (side right) with op = A or
+ * A' (trans), Omega either injected (omega != NULL: ell x K left, K x ell right,
+ * column-major, ld = rows) or generated from (seed, stream_id) as lra_gaussian_fill does.
+ * prec selects the MFMA arithmetic (F32/BF16/BF16X3 accumulate in FP32).
+ * Replaces mul!(C, RandomGaussian, A) -> crandn + dgemm (src/sketch.jl:91-110, 129-220). */
+int lra_sketch_gaussian(lra_ctx* ctx, int side, int trans, int64_t m, int64_t n,
+                        const double* A, int64_t lda, int64_t ell, const double* omega,
+                        uint64_t seed, uint64_t stream_id, int prec, double* B, int64_t ldb);
+
+/* SRFT sketch on the left, B (ell x n) = SRFT(ell) * op(A) (real case, reference quirks
+ * kept).  d: +-1 signs (K entries), idx: 1-based samples (ell entries), both device; when
+ * d == NULL both are generated from (seed, stream_id).
+ * Replaces mul!(C, SRFT, A) (src/sketch.jl:474-484, 499-510) incl. FFTW r2r. */
+int lra_sketch_srft(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A,
+                    int64_t lda, int64_t ell, const double* d, const int64_t* idx,
+                    uint64_t seed, uint64_t stream_id, double* B, int64_t ldb);
+
+/* Random row subset on the left, B[i,:] = op(A)[r[i],:] (r 1-based, device, or generated
+ * when NULL).  Replaces mul!(C, RandomSubset, A) (src/sketch.jl:248-257, 271-281). */
+int lra_sketch_subset(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A,
+                      int64_t lda, int64_t ell, const int64_t* r, uint64_t seed,
+                      uint64_t stream_id, double* B, int64_t ldb);
+
+/* Sparse random Gaussian on the left (perm r 1-based and coefficients s, K entries each,
+ * device, or generated when NULL).  Replaces mul!(C, SparseRandomGaussian, A)
+ * (src/sketch.jl:571-589, 614-633). */
+int lra_sketch_sprn(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A,
+                    int64_t lda, int64_t ell, const int64_t* r, const double* s,
+                    uint64_t seed, uint64_t stream_id, double* B, int64_t ldb);
+
+/* Exact ?laqps drop-in for the ccall at src/lapack.jl:129-136 (argument list of dlaqps by
+ * value instead of by reference): one block of at most nb pivoted Householder steps on the
+ * m x n panel A (rows offset+1..m active), jpvt/vn1/vn2 in/out, tau out, *kb (host) out.
+ * auxv and F are accepted for signature parity and not used (they are dlaqps workspace). */
+int lra_laqps_f64(lra_ctx* ctx, int64_t m, int64_t n, int64_t offset, int64_t nb,
+                  int64_t* kb, double* A, int64_t lda, int64_t* jpvt, double* tau,
+                  double* vn1, double* vn2, double* auxv, double* F, int64_t ldf);
+
+/* Rank-adaptive GEQP3 (geqp3_adap! + geqp3_adap_main!, src/pqr.jl:348-418) in one launch:
+ * B (m x n) overwritten by the LAPACK-layout factorization, jpvt (n, 1-based) and tau
+ * (min(m,n)) out, *k (host) = rank.  rank_cap < 0 means min(m,n). */
+int lra_geqp3_adap_f64(lra_ctx* ctx, int64_t m, int64_t n, double* B, int64_t ldb,
+                       int64_t rank_cap, double atol, double rtol, int64_t nb, int64_t* jpvt,
+                       double* tau, int64_t* k);
+
+/* T = R11 \ R12 for R = triu(R(1:k,:)) (maxdet_t, src/pqr.jl:438-442; the dtrsm behind
+ * ldiv!(UpperTriangular, .)).  R is k x n (ldr), T k x (n-k) (ldt). */
+int lra_id_T_f64(lra_ctx* ctx, int64_t k, int64_t n, const double* R, int64_t ldr, double* T,
+                 int64_t ldt);
+
+/* Strong-RRQR determinant-maximising swaps on (p, T) (maxdet_swapcols!/maxdet_update!,
+ * src/pqr.jl:445-501); p device 1-based (n), T k x (n-k).  *niter (host) out. */
+int lra_maxdet_swapcols_f64(lra_ctx* ctx, int64_t k, int64_t n, int64_t* p, double* T,
+                            int64_t ldt, double tol, int64_t niter_max, int64_t* niter);
+
+/* Spectral norm by power iteration (snorm, src/snorm.jl:14-45) of a dense A (isherm != 0:
+ * iterate on A itself).  x0 (n, device) is the start vector or NULL (generated). */
+int lra_snorm(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda, int isherm,
+              const lra_opts* opts, const double* x0, double* result);
+
+/* snormdiff(A, ID) = snorm(A - A[:,sk]*[I T]P') (src/snorm.jl:50-53 with the ID as a
+ * LinearOperator): the difference is applied matrix-free.  sk/rd/T as in lra_id_out. */
+int lra_snormdiff_id(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                     int64_t k, const int64_t* sk, const int64_t* rd, const double* T,
+                     int64_t ldt, const lra_opts* opts, const double* x0, double* result);
+
+/* ---- whole-pipeline entry points (mirror the reference API) -------------------------- */
+
+/* idfact(trans, A, opts) (src/id.jl:434-447): sketchfact(:left, trans, A) (adaptive loop,
+ * src/sketch.jl:223-240 and siblings) -> QRCP -> T.  With opts->precision != F64 the sketch
+ * GEMM runs on low-precision MFMA and only selects the skeleton; T is recomputed in FP64
+ * from A's skeleton columns with one step of iterative refinement. */
+int lra_idfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
+               const lra_opts* opts, lra_id_out* out);
+
+/* idfact on a matrix-free operator (idfact(L::LinearOperator), src/id.jl:434-447 with
+ * src/linop.jl:105-113): the sketch is one application of L' to ell vectors. */
+int lra_idfact_op(lra_ctx* ctx, int trans, int64_t m, int64_t n, lra_apply_fn apply,
+                  void* user, const lra_opts* opts, lra_id_out* out);
+
+/* Batched ID of many independent blocks (MI355X extension; each block behaves as idfact
+ * with a fresh Gaussian sketch of stream `b` under opts->seed): one persistent fused launch
+ * (sketch GEMM + QRCP + T per block).  A[b], sk[b], rd[b], T[b] are device pointers given
+ * in DEVICE arrays; k_out (device, nblk) receives the ranks.  Blocks whose rank reaches the
+ * first sketch order are re-run through lra_idfact (status reports how many). */
+int lra_idfact_batched(lra_ctx* ctx, int64_t nblk, const int64_t* m, const int64_t* n,
+                       const double* const* A, const int64_t* lda, const lra_opts* opts,
+                       int64_t* const* sk, int64_t* const* rd, double* const* T,
+                       const int64_t* ldt, int64_t* k_out, int64_t* n_fallback);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LRA_H */

@@ -1,0 +1,357 @@
+"""MI355X-native sketch-then-partial-factorize path of LowRankApprox.jl.
+
+Host-side mirror of the reference API (src/LowRankApprox.jl exports) over the C ABI of
+liblra_hip.so.  Matrices are torch CUDA tensors in column-major layout (a numpy Fortran array
+or any (m, n) tensor with strides (1, ld)); indices are 1-based like the reference.
+
+  LRAOptions            src/LowRankApprox.jl:77-148
+  idfact / id           src/id.jl:429-458           -> IDPackedV(sk, rd, T)
+  sketch                src/sketch.jl:35-50
+  geqp3_adap / laqps    src/pqr.jl:348-418, src/lapack.jl:117-139
+  snorm / snormdiff     src/snorm.jl:14-53
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import LraError, check
+
+__all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "geqp3_adap", "laqps",
+           "id_T", "gaussian", "to_device", "LraError", "context"]
+
+_EPS = float(np.finfo(np.float64).eps)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclasses.dataclass
+class LRAOptions:
+    """LRAOptions with the reference's Float64 defaults (src/LowRankApprox.jl:96-119).
+    `sketchfact_*_samp` are affine maps n -> a*n + b given as (a, b).  `precision` (sketch
+    GEMM arithmetic: "f64", "f32", "bf16", "bf16x3") and `seed` (Philox key) are MI355X
+    extensions."""
+    atol: float = 0.0
+    maxdet_niter: int = -1
+    maxdet_tol: float = -1.0
+    nb: int = 32
+    pheig_orthtol: float = math.sqrt(_EPS)
+    pqrfact_retval: str = "qr"
+    rank: int = -1
+    rtol: float = 5 * _EPS
+    sketch: str = "randn"
+    sketch_randn_niter: int = 0
+    sketchfact_adap: bool = True
+    sketchfact_randn_samp: tuple = (1, 8)
+    sketchfact_srft_samp: tuple = (1, 8)
+    sketchfact_sub_samp: tuple = (4, 8)
+    snorm_niter: int = 32
+    verb: bool = True
+    precision: str = "f64"
+    seed: int = 0
+
+    def copy(self, **kw) -> "LRAOptions":
+        o = dataclasses.replace(self)
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise AttributeError(k)
+            setattr(o, k, v)
+        return o
+
+    def check(self) -> None:
+        """chkopts! (src/LowRankApprox.jl:133-141)."""
+        if not self.atol >= 0: raise ValueError("atol")
+        if not self.nb > 0: raise ValueError("nb")
+        if not self.pheig_orthtol >= 0: raise ValueError("pheig_orthtol")
+        if not self.rtol >= 0: raise ValueError("rtol")
+        if self.sketch not in _lib.SKETCH: raise ValueError("sketch")
+        if self.precision not in _lib.PREC: raise ValueError("precision")
+        self.pqrfact_retval = self.pqrfact_retval.lower()
+
+    def to_c(self) -> _lib.lra_opts:
+        self.check()
+        o = _lib.default_opts()
+        o.atol = self.atol
+        o.maxdet_niter = self.maxdet_niter
+        o.maxdet_tol = self.maxdet_tol
+        o.nb = self.nb
+        o.pheig_orthtol = self.pheig_orthtol
+        rv = 0
+        for ch, bit in (("q", _lib.RET_Q), ("r", _lib.RET_R), ("t", _lib.RET_T)):
+            if ch in self.pqrfact_retval:
+                rv |= bit
+        o.pqrfact_retval = rv
+        o.rank = self.rank
+        o.rtol = self.rtol
+        o.sketch = _lib.SKETCH[self.sketch]
+        o.sketch_randn_niter = self.sketch_randn_niter
+        o.sketchfact_adap = int(bool(self.sketchfact_adap))
+        o.randn_samp_mul, o.randn_samp_add = self.sketchfact_randn_samp
+        o.srft_samp_mul, o.srft_samp_add = self.sketchfact_srft_samp
+        o.sub_samp_mul, o.sub_samp_add = self.sketchfact_sub_samp
+        o.snorm_niter = self.snorm_niter
+        o.verb = int(bool(self.verb))
+        o.precision = _lib.PREC[self.precision]
+        o.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        return o
+
+
+# ----------------------------------------------------------------------------------------
+# device plumbing
+
+def context(device: Optional[int] = None) -> _lib.Context:
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise LraError("no GPU visible: the MI355X path has no CPU fallback")
+    dev = torch.cuda.current_device() if device is None else device
+    ctx = _lib.Context.get(dev)
+    ctx.bind_stream(torch.cuda.current_stream(dev).cuda_stream)
+    return ctx
+
+
+def to_device(A, dtype=None):
+    """Column-major float64 CUDA tensor for A (numpy array or tensor)."""
+    torch = _torch()
+    if isinstance(A, np.ndarray):
+        A = torch.from_numpy(np.asfortranarray(A, dtype=np.float64 if dtype is None else dtype))
+    if A.dim() == 1:
+        return A.to("cuda", dtype=torch.float64 if dtype is None else dtype).contiguous()
+    if A.dtype != torch.float64:
+        A = A.to(torch.float64)
+    if not A.is_cuda:
+        A = A.cuda()
+    if not (A.stride(0) == 1 and A.stride(1) >= max(1, A.shape[0])):
+        A = A.t().contiguous().t()
+    return A
+
+
+def _ld(A) -> int:
+    return max(1, A.stride(1)) if A.dim() == 2 else max(1, A.shape[0])
+
+
+def empty_colmajor(m: int, n: int, dtype=None):
+    torch = _torch()
+    return torch.empty((n, m), dtype=torch.float64 if dtype is None else dtype,
+                       device="cuda").t()
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+# ----------------------------------------------------------------------------------------
+# primitives
+
+def gaussian(rows: int, cols: int, seed: int = 0, stream: int = 0):
+    """The rows x cols standard-normal matrix of (seed, stream) (Philox4x32-10)."""
+    ctx = context()
+    out = empty_colmajor(rows, cols)
+    check(_lib.load().lra_gaussian_fill(ctx.h, seed, stream, rows, cols, _ptr(out),
+                                        max(1, rows)), "lra_gaussian_fill")
+    return out
+
+
+def sketch(side: str, trans: str, A, order: int, opts: LRAOptions = None, omega=None,
+           stream: int = 0, **kw):
+    """sketch (src/sketch.jl:35-50), Gaussian case: B = Omega*op(A) or op(A)*Omega."""
+    opts = (opts or LRAOptions()).copy(**kw)
+    if order < 0:
+        raise ValueError("order")
+    if side not in _lib.SIDE: raise ValueError("side")
+    if trans not in _lib.TRANS: raise ValueError("trans")
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    if opts.sketch != "randn":
+        return _sketch_structured(side, trans, A, order, opts, stream, ctx)
+    if side == "left":
+        B = empty_colmajor(order, n if trans == "n" else m)
+    else:
+        B = empty_colmajor(m if trans == "n" else n, order)
+    om = to_device(omega) if omega is not None else None
+    check(_lib.load().lra_sketch_gaussian(
+        ctx.h, _lib.SIDE[side], _lib.TRANS[trans], m, n, _ptr(A), _ld(A), order, _ptr(om),
+        opts.seed, stream, _lib.PREC[opts.precision], _ptr(B), _ld(B)), "lra_sketch_gaussian")
+    return B
+
+
+def _sketch_structured(side, trans, A, order, opts, stream, ctx):
+    torch = _torch()
+    if side == "right":
+        # A*S = (S'*A')' with the same structure along the other dimension
+        Bt = _sketch_structured("left", "c" if trans == "n" else "n", A, order, opts, stream, ctx)
+        return Bt.t()
+    m, n = A.shape
+    B = empty_colmajor(order, n if trans == "n" else m)
+    L = _lib.load()
+    t = _lib.TRANS[trans]
+    if opts.sketch == "srft":
+        check(L.lra_sketch_srft(ctx.h, t, m, n, _ptr(A), _ld(A), order, None, None, opts.seed,
+                                stream, _ptr(B), _ld(B)), "lra_sketch_srft")
+    elif opts.sketch == "sub":
+        check(L.lra_sketch_subset(ctx.h, t, m, n, _ptr(A), _ld(A), order, None, opts.seed,
+                                  stream, _ptr(B), _ld(B)), "lra_sketch_subset")
+    elif opts.sketch == "sprn":
+        check(L.lra_sketch_sprn(ctx.h, t, m, n, _ptr(A), _ld(A), order, None, None, opts.seed,
+                                stream, _ptr(B), _ld(B)), "lra_sketch_sprn")
+    else:
+        raise ValueError(opts.sketch)
+    return B
+
+
+def geqp3_adap(B, rank: int = -1, atol: float = 0.0, rtol: float = 5 * _EPS, nb: int = 32):
+    """Rank-adaptive pivoted QR in place (geqp3_adap!, src/pqr.jl:348-418).
+    Returns (jpvt 1-based, tau, k)."""
+    torch = _torch()
+    ctx = context()
+    if not (B.is_cuda and B.dtype == torch.float64 and B.stride(0) == 1):
+        raise ValueError("B must be a column-major float64 CUDA tensor (updated in place)")
+    m, n = B.shape
+    jpvt = torch.empty(max(n, 1), dtype=torch.int64, device="cuda")
+    tau = torch.empty(max(min(m, n), 1), dtype=torch.float64, device="cuda")
+    k = _lib.i64(0)
+    check(_lib.load().lra_geqp3_adap_f64(ctx.h, m, n, _ptr(B), _ld(B), rank, atol, rtol, nb,
+                                         _ptr(jpvt), _ptr(tau), C.byref(k)), "lra_geqp3_adap_f64")
+    return jpvt[:n], tau[:min(m, n)], int(k.value)
+
+
+def laqps(A, offset: int, nb: int, jpvt, vn1, vn2):
+    """One ?laqps block step in place (src/lapack.jl:117-139).  Returns (tau, kb)."""
+    torch = _torch()
+    ctx = context()
+    m, n = A.shape
+    tau = torch.zeros(max(nb, 1), dtype=torch.float64, device="cuda")
+    kb = _lib.i64(0)
+    check(_lib.load().lra_laqps_f64(ctx.h, m, n, offset, nb, C.byref(kb), _ptr(A), _ld(A),
+                                    _ptr(jpvt), _ptr(tau), _ptr(vn1), _ptr(vn2), None, None,
+                                    max(1, n)), "lra_laqps_f64")
+    return tau[: int(kb.value)], int(kb.value)
+
+
+def id_T(R, k: int):
+    """T = R[:, :k] \\ R[:, k:] with R upper trapezoidal (maxdet_t, src/pqr.jl:438-442)."""
+    ctx = context()
+    n = R.shape[1]
+    T = empty_colmajor(max(k, 1), max(n - k, 0))
+    check(_lib.load().lra_id_T_f64(ctx.h, k, n, _ptr(R), _ld(R), _ptr(T), max(1, k)),
+          "lra_id_T_f64")
+    return T[:k, :]
+
+
+# ----------------------------------------------------------------------------------------
+# IDPackedV / idfact
+
+@dataclasses.dataclass
+class IDPackedV:
+    """IDPackedV (src/id.jl:15-19): A[:, rd] ~ A[:, sk] * T (1-based indices, device)."""
+    sk: object
+    rd: object
+    T: object
+    rounds: int = 1
+
+    @property
+    def k(self) -> int:
+        return int(self.sk.shape[0])
+
+    def __getitem__(self, key):
+        # getindex (src/id.jl:72-81)
+        torch = _torch()
+        if key == "sk": return self.sk
+        if key == "rd": return self.rd
+        if key == "T": return self.T
+        if key == "k": return self.k
+        if key == "p": return torch.cat([self.sk, self.rd])
+        raise KeyError(key)
+
+
+def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
+    """idfact(trans, A, opts; kw...) (src/id.jl:434-447); idfact(A) means trans = :n."""
+    torch = _torch()
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    if isinstance(A, LinearOperator):
+        return A.idfact(trans, opts, **kw)
+    opts = (opts or LRAOptions()).copy(**kw)
+    opts.pqrfact_retval = "t"
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    N = n if trans == "n" else m
+    sk = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+    rd = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+    out = _lib.lra_id_out()
+    out.sk, out.rd = _ptr(sk), _ptr(rd)
+    out.T, out.ldt = None, 0  # library-owned T, copied out below
+    copts = opts.to_c()
+    check(_lib.load().lra_idfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
+                                 C.byref(copts), C.byref(out)), "lra_idfact")
+    k = int(out.k)
+    T = _borrow_T(out, k, N - k)
+    return IDPackedV(sk[:k], rd[: N - k], T, int(out.rounds))
+
+
+def _borrow_T(out, k: int, nr: int):
+    """Copy the library-owned k x nr T (ldt = out.ldt) into a new column-major tensor."""
+    torch = _torch()
+    T = empty_colmajor(k, nr)
+    if k > 0 and nr > 0:
+        ldt = int(out.ldt)
+        _copy_device(T.data_ptr(), int(out.T), k, nr, ldt, k)
+    return T
+
+
+def _copy_device(dst: int, src: int, rows: int, cols: int, lds: int, ldd: int) -> None:
+    """2-D device copy of doubles on the current stream (hipMemcpy2DAsync via torch)."""
+    torch = _torch()
+    import ctypes
+    hip = _hip()
+    st = torch.cuda.current_stream().cuda_stream
+    rc = hip.hipMemcpy2DAsync(ctypes.c_void_p(dst), ctypes.c_size_t(8 * ldd), ctypes.c_void_p(src),
+                              ctypes.c_size_t(8 * lds), ctypes.c_size_t(8 * rows),
+                              ctypes.c_size_t(cols), ctypes.c_int(3), ctypes.c_void_p(st))
+    if rc != 0:
+        raise LraError(f"hipMemcpy2DAsync failed ({rc})")
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        _HIP = ctypes.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy2DAsync.restype = ctypes.c_int
+    return _HIP
+
+
+def id(trans, A=None, opts: LRAOptions = None, **kw):
+    """id (src/id.jl:452-455): (sk, rd, T)."""
+    V = idfact(trans, A, opts, **kw)
+    return V.sk, V.rd, V.T
+
+
+class LinearOperator:
+    """Matrix-free operator (src/linop.jl:9-40): `mul(X) -> L*X`, `mulc(X) -> L'*X` on
+    column-major CUDA tensors."""
+
+    def __init__(self, m: int, n: int, mul, mulc=None, hermitian: bool = False):
+        self.m, self.n = m, n
+        self.mul = mul
+        self.mulc = mulc if mulc is not None else mul
+        self.hermitian = hermitian
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    def idfact(self, trans, opts, **kw):
+        raise LraError("matrix-free idfact: not built yet")

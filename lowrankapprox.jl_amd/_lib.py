@@ -1,0 +1,160 @@
+"""ctypes binding of liblra_hip.so (the C ABI declared in include/lra.h).
+
+The shared library is built in-tree by build.py.  Loading fails loudly when it is missing:
+there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblra_hip.so")
+
+i64 = C.c_int64
+P_D = C.c_void_p  # device pointers are passed as raw addresses
+P_I = C.c_void_p
+
+LRA_OK = 0
+SKETCH = {"none": 0, "randn": 1, "srft": 2, "sub": 3, "sprn": 4}
+PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16x3": 3}
+RET_Q, RET_R, RET_T = 1, 2, 4
+TRANS = {"n": 0, "c": 1}
+SIDE = {"left": 0, "right": 1}
+
+
+class lra_opts(C.Structure):
+    """Mirror of `lra_opts` (include/lra.h); field order matters."""
+    _fields_ = [
+        ("atol", C.c_double),
+        ("maxdet_niter", i64),
+        ("maxdet_tol", C.c_double),
+        ("nb", i64),
+        ("pheig_orthtol", C.c_double),
+        ("pqrfact_retval", C.c_int32),
+        ("rank", i64),
+        ("rtol", C.c_double),
+        ("sketch", C.c_int32),
+        ("sketch_randn_niter", i64),
+        ("sketchfact_adap", C.c_int32),
+        ("randn_samp_mul", i64), ("randn_samp_add", i64),
+        ("srft_samp_mul", i64), ("srft_samp_add", i64),
+        ("sub_samp_mul", i64), ("sub_samp_add", i64),
+        ("snorm_niter", i64),
+        ("verb", C.c_int32),
+        ("precision", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class lra_id_out(C.Structure):
+    _fields_ = [("sk", P_I), ("rd", P_I), ("T", P_D), ("ldt", i64), ("k", i64),
+                ("rounds", i64)]
+
+
+APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, i64, C.c_void_p, i64, C.c_void_p, i64,
+                       C.c_void_p)
+
+# (name, restype, argtypes) of every exported symbol declared in include/lra.h
+SIGNATURES = [
+    ("lra_opts_default", None, [C.POINTER(lra_opts)]),
+    ("lra_ctx_create", C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("lra_ctx_destroy", C.c_int, [C.c_void_p]),
+    ("lra_ctx_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("lra_sync", C.c_int, [C.c_void_p]),
+    ("lra_last_error", C.c_char_p, []),
+    ("lra_device_info", C.c_int, [C.c_void_p, C.POINTER(i64), C.POINTER(i64)]),
+    ("lra_gaussian_fill", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, i64, i64, P_D, i64]),
+    ("lra_sketch_gaussian", C.c_int, [C.c_void_p, C.c_int, C.c_int, i64, i64, P_D, i64, i64,
+                                      P_D, C.c_uint64, C.c_uint64, C.c_int, P_D, i64]),
+    ("lra_sketch_srft", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, i64, P_D, P_I,
+                                  C.c_uint64, C.c_uint64, P_D, i64]),
+    ("lra_sketch_subset", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, i64, P_I,
+                                    C.c_uint64, C.c_uint64, P_D, i64]),
+    ("lra_sketch_sprn", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, i64, P_I, P_D,
+                                  C.c_uint64, C.c_uint64, P_D, i64]),
+    ("lra_laqps_f64", C.c_int, [C.c_void_p, i64, i64, i64, i64, C.POINTER(i64), P_D, i64,
+                                P_I, P_D, P_D, P_D, P_D, P_D, i64]),
+    ("lra_geqp3_adap_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, C.c_double,
+                                     C.c_double, i64, P_I, P_D, C.POINTER(i64)]),
+    ("lra_id_T_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64]),
+    ("lra_maxdet_swapcols_f64", C.c_int, [C.c_void_p, i64, i64, P_I, P_D, i64, C.c_double,
+                                          i64, C.POINTER(i64)]),
+    ("lra_snorm", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.c_int, C.POINTER(lra_opts),
+                            P_D, C.POINTER(C.c_double)]),
+    ("lra_snormdiff_id", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, P_I, P_I, P_D, i64,
+                                   C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
+    ("lra_idfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                             C.POINTER(lra_id_out)]),
+    ("lra_idfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p,
+                                C.POINTER(lra_opts), C.POINTER(lra_id_out)]),
+    ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
+                                     C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     P_I, P_I, C.POINTER(i64)]),
+]
+
+
+class LraError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load liblra_hip.so and declare every signature.  Raises if the library is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise LraError(
+                f"{path} is missing: build it with `python lowrankapprox.jl_amd/build.py` "
+                "(the HIP path has no CPU fallback)")
+        L = C.CDLL(path)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return L
+
+
+def check(status: int, what: str = "") -> None:
+    if status != LRA_OK:
+        msg = load().lra_last_error()
+        msg = msg.decode() if msg else ""
+        raise LraError(f"{what} failed with status {status}: {msg}")
+
+
+def default_opts() -> lra_opts:
+    o = lra_opts()
+    load().lra_opts_default(C.byref(o))
+    return o
+
+
+class Context:
+    """One lra_ctx per (process, device); work is enqueued on torch's current stream."""
+
+    _ctx: dict = {}
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        check(load().lra_ctx_create(device, hbm_budget, C.byref(h)), "lra_ctx_create")
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def get(cls, device: int = 0) -> "Context":
+        c = cls._ctx.get(device)
+        if c is None:
+            c = cls._ctx[device] = Context(device)
+        return c
+
+    def bind_stream(self, stream_ptr: int) -> None:
+        check(load().lra_ctx_set_stream(self.h, C.c_void_p(stream_ptr)), "lra_ctx_set_stream")
+
+    def sync(self) -> None:
+        check(load().lra_sync(self.h), "lra_sync")

@@ -1,0 +1,573 @@
+// api.cpp -- C ABI of liblra_hip.so: contexts, workspace, and the host drivers that mirror
+// the reference's control flow (adaptive sketch loop, postprocessing) on top of the kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "lra_internal.h"
+
+static thread_local char g_err[1024] = "";
+
+void lra_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* lra_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------------------
+// options
+
+extern "C" void lra_opts_default(lra_opts* o) {
+  // LRAOptions(Float64) defaults, src/LowRankApprox.jl:96-119
+  std::memset(o, 0, sizeof(*o));
+  o->atol = 0.0;
+  o->maxdet_niter = -1;
+  o->maxdet_tol = -1.0;
+  o->nb = 32;
+  o->pheig_orthtol = std::sqrt(2.220446049250313e-16);
+  o->pqrfact_retval = LRA_RET_Q | LRA_RET_R;
+  o->rank = -1;
+  o->rtol = 5 * 2.220446049250313e-16;
+  o->sketch = LRA_SKETCH_RANDN;
+  o->sketch_randn_niter = 0;
+  o->sketchfact_adap = 1;
+  o->randn_samp_mul = 1; o->randn_samp_add = 8;
+  o->srft_samp_mul = 1; o->srft_samp_add = 8;
+  o->sub_samp_mul = 4; o->sub_samp_add = 8;
+  o->snorm_niter = 32;
+  o->verb = 1;
+  o->precision = LRA_PREC_F64;
+  o->seed = 0;
+}
+
+// chkopts! (src/LowRankApprox.jl:133-141)
+static int chkopts(const lra_opts* o) {
+  if (!(o->atol >= 0)) { lra_set_error("atol"); return -1; }
+  if (!(o->nb > 0)) { lra_set_error("nb"); return -1; }
+  if (!(o->pheig_orthtol >= 0)) { lra_set_error("pheig_orthtol"); return -1; }
+  if (!(o->rtol >= 0)) { lra_set_error("rtol"); return -1; }
+  if (o->sketch < LRA_SKETCH_NONE || o->sketch > LRA_SKETCH_SPRN) { lra_set_error("sketch"); return -1; }
+  if (o->precision < LRA_PREC_F64 || o->precision > LRA_PREC_BF16X3) { lra_set_error("precision"); return -1; }
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// context
+
+extern "C" int lra_ctx_create(int device, size_t hbm_budget, lra_ctx** out) {
+  LRA_ARG(out != nullptr, 3);
+  int ndev = 0;
+  LRA_HIP(hipGetDeviceCount(&ndev));
+  LRA_ARG(device >= 0 && device < ndev, 1);
+  LRA_HIP(hipSetDevice(device));
+  lra_ctx* c = new lra_ctx();
+  c->device = device;
+  c->hbm_budget = hbm_budget;
+  hipDeviceProp_t prop;
+  LRA_HIP(hipGetDeviceProperties(&prop, device));
+  c->num_cu = prop.multiProcessorCount;
+  c->lds_bytes = (int)prop.sharedMemPerMultiprocessor;
+  LRA_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
+  LRA_HIP(hipHostMalloc(&c->host_pinned, 4096, hipHostMallocDefault));
+  LRA_HIP(hipMalloc(&c->d_err, 64));
+  LRA_HIP(hipMemset(c->d_err, 0, 64));
+  *out = c;
+  return LRA_OK;
+}
+
+extern "C" int lra_ctx_destroy(lra_ctx* c) {
+  if (!c) return LRA_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (int i = 0; i < lra_ctx::kSlots; ++i)
+    if (c->ws[i]) hipFree(c->ws[i]);
+  if (c->coop) hipFree(c->coop);
+  if (c->d_err) hipFree(c->d_err);
+  if (c->host_pinned) hipHostFree(c->host_pinned);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+  return LRA_OK;
+}
+
+extern "C" int lra_ctx_set_stream(lra_ctx* c, void* stream) {
+  LRA_ARG(c != nullptr, 1);
+  c->stream = (hipStream_t)stream;  // NULL is the device's null (legacy default) stream
+  return LRA_OK;
+}
+
+extern "C" int lra_sync(lra_ctx* c) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_HIP(hipStreamSynchronize(c->stream));
+  return lra_check_device_error(c);
+}
+
+extern "C" int lra_device_info(lra_ctx* c, int64_t* num_cu, int64_t* lds) {
+  LRA_ARG(c != nullptr, 1);
+  if (num_cu) *num_cu = c->num_cu;
+  if (lds) *lds = c->lds_bytes;
+  return LRA_OK;
+}
+
+static size_t total_ws(lra_ctx* c) {
+  size_t s = c->coop_size;
+  for (int i = 0; i < lra_ctx::kSlots; ++i) s += c->ws_size[i];
+  return s;
+}
+
+int lra_ws(lra_ctx* c, int slot, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (c->ws_size[slot] < bytes) {
+    const size_t want = std::max(bytes, c->ws_size[slot] + c->ws_size[slot] / 4);
+    if (c->hbm_budget && total_ws(c) - c->ws_size[slot] + want > c->hbm_budget) {
+      lra_set_error("workspace: %zu bytes exceed the HBM budget %zu", want, c->hbm_budget);
+      return LRA_E_ALLOC;
+    }
+    if (c->ws[slot]) {
+      LRA_HIP(hipStreamSynchronize(c->stream));
+      LRA_HIP(hipFree(c->ws[slot]));
+      c->ws[slot] = nullptr;
+      c->ws_size[slot] = 0;
+    }
+    hipError_t e = hipMalloc(&c->ws[slot], want);
+    if (e != hipSuccess) {
+      lra_set_error("workspace hipMalloc(%zu): %s", want, hipGetErrorString(e));
+      return LRA_E_ALLOC;
+    }
+    c->ws_size[slot] = want;
+  }
+  *out = c->ws[slot];
+  return LRA_OK;
+}
+
+int lra_coop_area(lra_ctx* c, size_t bytes, void** out) {
+  if (c->coop_size < bytes) {
+    if (c->coop) {
+      LRA_HIP(hipStreamSynchronize(c->stream));
+      LRA_HIP(hipFree(c->coop));
+      c->coop = nullptr;
+    }
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&c->coop, want);
+    if (e != hipSuccess) {
+      lra_set_error("coop hipMalloc(%zu): %s", want, hipGetErrorString(e));
+      return LRA_E_ALLOC;
+    }
+    c->coop_size = want;
+  }
+  *out = c->coop;
+  return LRA_OK;
+}
+
+int lra_check_device_error(lra_ctx* c) {
+  int h = 0;
+  LRA_HIP(hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (h != 0) {
+    hipMemset(c->d_err, 0, sizeof(int));
+    lra_set_error("device error %d (a bounded wait expired in a cooperative kernel)", h);
+    return h;
+  }
+  return LRA_OK;
+}
+
+// synchronous read of device scalars through the pinned staging area
+template <class T>
+static int read_scalars(lra_ctx* c, const T* dev, T* host, int count) {
+  LRA_HIP(hipMemcpyAsync(c->host_pinned, dev, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+  LRA_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(host, c->host_pinned, sizeof(T) * count);
+  return lra_check_device_error(c);
+}
+
+// ---------------------------------------------------------------------------------------
+// primitives
+
+extern "C" int lra_gaussian_fill(lra_ctx* c, uint64_t seed, uint64_t stream_id, int64_t rows,
+                                 int64_t cols, double* out, int64_t ld) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(rows >= 0, 4);
+  LRA_ARG(cols >= 0, 5);
+  LRA_ARG(out != nullptr || rows * cols == 0, 6);
+  LRA_ARG(ld >= std::max<int64_t>(1, rows), 7);
+  return lra_launch_gaussian_fill(c->stream, seed, stream_id, rows, cols, out, ld);
+}
+
+// Gaussian sketch into B; Omega materialised in WS_OMEGA when not injected.
+static int sketch_gaussian_impl(lra_ctx* c, int side, int trans, i64 m, i64 n, const double* A,
+                                i64 lda, i64 ell, const double* omega, uint64_t seed,
+                                uint64_t stream_id, int prec, double* B, i64 ldb) {
+  // op(A) is p x q
+  const i64 p = trans == LRA_TRANS_N ? m : n;
+  const i64 q = trans == LRA_TRANS_N ? n : m;
+  GemmArgs g;
+  g.alpha = 1.0; g.beta = 0.0; g.C = B; g.ldc = ldb; g.bgather = nullptr; g.prec = prec;
+  if (side == LRA_SIDE_LEFT) {
+    // B (ell x q) = Omega (ell x p) * op(A)
+    const double* om = omega;
+    if (!om) {
+      double* w = nullptr;
+      LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)ell * p, &w));
+      LRA_CHECK(lra_launch_gaussian_fill(c->stream, seed, stream_id, ell, p, w, ell));
+      om = w;
+    }
+    g.M = ell; g.N = q; g.K = p;
+    g.A = om; g.lda = ell; g.transA = 0;
+    g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 0 : 1;
+  } else {
+    // B (p x ell) = op(A) (p x q) * Omega (q x ell)
+    const double* om = omega;
+    if (!om) {
+      double* w = nullptr;
+      LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)q * ell, &w));
+      LRA_CHECK(lra_launch_gaussian_fill(c->stream, seed, stream_id, q, ell, w, q));
+      om = w;
+    }
+    g.M = p; g.N = ell; g.K = q;
+    g.A = A; g.lda = lda; g.transA = trans == LRA_TRANS_N ? 0 : 1;
+    g.B = om; g.ldb = q; g.transB = 0;
+  }
+  return lra_gemm(c, g);
+}
+
+extern "C" int lra_sketch_gaussian(lra_ctx* c, int side, int trans, int64_t m, int64_t n,
+                                   const double* A, int64_t lda, int64_t ell,
+                                   const double* omega, uint64_t seed, uint64_t stream_id,
+                                   int prec, double* B, int64_t ldb) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(side == LRA_SIDE_LEFT || side == LRA_SIDE_RIGHT, 2);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 3);
+  LRA_ARG(m >= 0, 4);
+  LRA_ARG(n >= 0, 5);
+  LRA_ARG(A != nullptr || m * n == 0, 6);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 7);
+  LRA_ARG(ell >= 0, 8);
+  LRA_ARG(prec >= LRA_PREC_F64 && prec <= LRA_PREC_BF16X3, 12);
+  const i64 rows = side == LRA_SIDE_LEFT ? ell : (trans == LRA_TRANS_N ? m : n);
+  LRA_ARG(ldb >= std::max<int64_t>(1, rows), 14);
+  return sketch_gaussian_impl(c, side, trans, m, n, A, lda, ell, omega, seed, stream_id, prec,
+                              B, ldb);
+}
+
+extern "C" int lra_geqp3_adap_f64(lra_ctx* c, int64_t m, int64_t n, double* B, int64_t ldb,
+                                  int64_t rank_cap, double atol, double rtol, int64_t nb,
+                                  int64_t* jpvt, double* tau, int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0 && m < (1 << 20), 2);
+  LRA_ARG(n >= 0 && n < (1 << 30), 3);
+  LRA_ARG(B != nullptr || m * n == 0, 4);
+  LRA_ARG(ldb >= std::max<int64_t>(1, m), 5);
+  LRA_ARG(atol >= 0, 7);
+  LRA_ARG(rtol >= 0, 8);
+  LRA_ARG(nb > 0, 9);
+  LRA_ARG(k != nullptr, 12);
+  const i64 l = std::min(m, n);
+  const i64 kmax = (rank_cap < 0 || rank_cap > l) ? l : rank_cap;
+  i64* dk = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_SCAL, 16, &dk));
+  QrcpCall q{};
+  q.A = B; q.lda = ldb; q.m = (int)m; q.n = (int)n; q.offset = 0; q.kmax = (int)kmax;
+  q.nb = (int)std::min<i64>(nb, std::max<i64>(kmax, 1));
+  q.mode = 0; q.atol = atol; q.rtol = rtol; q.jpvt = jpvt; q.tau = tau; q.vn1 = nullptr;
+  q.vn2 = nullptr; q.stop_after = 0; q.k_out = dk; q.early_out = nullptr;
+  LRA_CHECK(lra_qrcp(c, q));
+  i64 hk = 0;
+  LRA_CHECK(read_scalars(c, dk, &hk, 1));
+  *k = hk;
+  return LRA_OK;
+}
+
+extern "C" int lra_laqps_f64(lra_ctx* c, int64_t m, int64_t n, int64_t offset, int64_t nb,
+                             int64_t* kb, double* A, int64_t lda, int64_t* jpvt, double* tau,
+                             double* vn1, double* vn2, double* auxv, double* F, int64_t ldf) {
+  (void)auxv; (void)F; (void)ldf;
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0 && m < (1 << 20), 2);
+  LRA_ARG(n >= 0 && n < (1 << 30), 3);
+  LRA_ARG(offset >= 0 && offset <= m, 4);
+  LRA_ARG(nb >= 0, 5);
+  LRA_ARG(kb != nullptr, 6);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 8);
+  const i64 kmax = std::min<i64>(nb, std::min<i64>(n, m - offset));
+  i64* dk = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_SCAL, 16, &dk));
+  QrcpCall q{};
+  q.A = A; q.lda = lda; q.m = (int)m; q.n = (int)n; q.offset = (int)offset;
+  q.kmax = (int)kmax; q.nb = (int)std::max<i64>(kmax, 1); q.mode = 1; q.atol = 0; q.rtol = 0;
+  q.jpvt = jpvt; q.tau = tau; q.vn1 = vn1; q.vn2 = vn2; q.stop_after = 0; q.k_out = dk;
+  q.early_out = nullptr;
+  if (kmax <= 0) { *kb = 0; return LRA_OK; }
+  LRA_CHECK(lra_qrcp(c, q));
+  i64 hk = 0;
+  LRA_CHECK(read_scalars(c, dk, &hk, 1));
+  *kb = hk;
+  return LRA_OK;
+}
+
+extern "C" int lra_id_T_f64(lra_ctx* c, int64_t k, int64_t n, const double* R, int64_t ldr,
+                            double* T, int64_t ldt) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(k >= 0, 2);
+  LRA_ARG(n >= k, 3);
+  LRA_ARG(ldr >= std::max<int64_t>(1, k), 5);
+  LRA_ARG(ldt >= std::max<int64_t>(1, k), 7);
+  return lra_launch_id_T(c->stream, k, n, R, ldr, T, ldt);
+}
+
+// ---------------------------------------------------------------------------------------
+// sketchfact: the adaptive loop (src/sketch.jl:223-240 and the srft/sub/sprn siblings)
+
+struct SketchFactOut {
+  double* B;  // factored sketch (ell x N, ld = ell)
+  double* Bsave;  // the sketch before factorization (mixed form only, else nullptr)
+  i64 ell, N;
+  i64* jpvt;
+  double* tau;
+  i64 k;
+  i64 rounds;
+  int prec;      // arithmetic of the accepted round
+  i64 stream_id; // Philox stream of the accepted round's sketch
+};
+
+int lra_sketch_any(lra_ctx* c, int sketch, int trans, i64 m, i64 n, const double* A, i64 lda,
+                   i64 ell, uint64_t seed, uint64_t stream_id, int prec, double* B, i64 ldb);
+
+static i64 samp_of(const lra_opts* o, i64 x) {
+  switch (o->sketch) {
+    case LRA_SKETCH_RANDN: return o->randn_samp_mul * x + o->randn_samp_add;
+    case LRA_SKETCH_SRFT: return o->srft_samp_mul * x + o->srft_samp_add;
+    case LRA_SKETCH_SUB: return o->sub_samp_mul * x + o->sub_samp_add;
+    default: return x;  // sprn: order = n (src/sketch.jl:680)
+  }
+}
+
+// Relative noise floor of a sketch computed with `prec` (FP32 accumulation): diagonals of
+// its pivoted QR below floor * max column norm carry no information.
+static double prec_floor(int prec) {
+  switch (prec) {
+    case LRA_PREC_F32: return 1e-6;
+    case LRA_PREC_BF16X3: return 1e-6;
+    case LRA_PREC_BF16: return 5e-2;
+    default: return 0.0;
+  }
+}
+
+// sketchfact(:left, trans, A) (src/sketch.jl:223-240 and the srft/sub/sprn drivers).
+// Mixed precision (o->precision != F64, Gaussian sketch): a round on low-precision MFMA is
+// trusted when its answer does not depend on diagonals below the precision's noise floor:
+//   - rtol >= floor: the low-precision round is final (selection + rank); T is refined in FP64
+//     afterwards (lra_mixed_refine_T);
+//   - rtol <  floor: a round that certifies rank >= target above the floor doubles as usual;
+//     the first round that cannot is recomputed in FP64 with the same Omega, and every later
+//     round stays FP64, so the accepted factorization is exactly the FP64 drop-in's.
+static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                           const lra_opts* o, SketchFactOut* out) {
+  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(A)
+  const bool gauss = o->sketch == LRA_SKETCH_RANDN;
+  int prec = gauss ? o->precision : LRA_PREC_F64;
+  const bool adap = o->sketchfact_adap || o->rank < 0;
+  i64 target = adap ? o->nb : o->rank;
+  i64* dk = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_SCAL, 16, &dk));
+  int* dearly = (int*)(dk + 4);
+  for (i64 rnd = 0;;) {
+    const i64 ell = samp_of(o, target);
+    if (ell <= 0) { lra_set_error("sketch order %lld", (long long)ell); return LRA_E_UNSUPPORTED; }
+    double* B = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)ell * std::max<i64>(N, 1), &B));
+    LRA_CHECK(lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, (uint64_t)rnd,
+                             prec, B, ell));
+    const double floor = prec_floor(prec);
+    const bool lowp = prec != LRA_PREC_F64;
+    double* Bsave = nullptr;
+    if (lowp && o->rtol >= floor) {
+      LRA_CHECK(lra_wsT(c, WS_SKETCH2, (size_t)ell * std::max<i64>(N, 1), &Bsave));
+      LRA_CHECK(lra_launch_copy2d(c->stream, ell, N, B, ell, Bsave, ell));
+    }
+    i64* jpvt = nullptr;
+    double* tau = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_JPVT, (size_t)std::max<i64>(N, 1), &jpvt));
+    LRA_CHECK(lra_wsT(c, WS_TAU, (size_t)std::max<i64>(std::min(ell, N), 1), &tau));
+    const i64 l = std::min(ell, N);
+    const i64 kmax = (o->rank < 0 || o->rank > l) ? l : o->rank;
+    QrcpCall q{};
+    q.A = B; q.lda = ell; q.m = (int)ell; q.n = (int)N; q.offset = 0; q.kmax = (int)kmax;
+    q.nb = (int)std::min<i64>(o->nb, std::max<i64>(kmax, 1)); q.mode = 0;
+    q.atol = o->atol; q.rtol = lowp ? std::max(o->rtol, floor) : o->rtol;
+    q.jpvt = jpvt; q.tau = tau; q.vn1 = nullptr;
+    q.vn2 = nullptr; q.stop_after = adap ? (int)std::min<i64>(target, 1 << 30) : 0;
+    q.k_out = dk; q.early_out = dearly;
+    LRA_CHECK(lra_qrcp(c, q));
+    i64 hk = 0;
+    LRA_CHECK(read_scalars(c, dk, &hk, 1));
+    const bool accept = !adap || hk < target;
+    if (accept && lowp && o->rtol < floor) {
+      // the low-precision sketch cannot resolve this round: redo it in FP64 (same Omega)
+      prec = LRA_PREC_F64;
+      continue;
+    }
+    if (accept) {
+      out->B = B; out->Bsave = Bsave; out->ell = ell; out->N = N; out->jpvt = jpvt;
+      out->tau = tau; out->k = hk; out->rounds = rnd + 1; out->prec = prec;
+      out->stream_id = rnd;
+      return LRA_OK;
+    }
+    target *= 2;
+    ++rnd;
+  }
+}
+
+// Low-precision sketch selected the skeleton: recompute T in FP64 from A's skeleton columns
+// with one step of iterative refinement.  T0 = R11 \ R12 of the low-precision sketch; the
+// residual E = op(A)[:, rd] - op(A)[:, sk] * T0 is formed in FP64 (the one full-size FP64
+// GEMM), sketched with the same Omega in the low precision (E is small, so its rounding is
+// second order), and the correction solves the sketched least-squares problem through the
+// semi-normal equations with the sketch's R11:  R11' R11 dT = (S*op(A)[:, sk])' (S*E).
+static int mixed_refine_T(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                          const SketchFactOut& f, const i64* sk, const i64* rd, double* T,
+                          i64 ldt, const lra_opts* o) {
+  const i64 k = f.k, N = f.N, nr = N - k, ell = f.ell;
+  const i64 P = trans == LRA_TRANS_N ? m : n;  // rows of op(A)
+  double *C = nullptr, *E = nullptr, *SE = nullptr, *Bsk = nullptr, *X = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_TMP0, (size_t)P * k, &C));
+  LRA_CHECK(lra_wsT(c, WS_TMP1, (size_t)P * nr, &E));
+  LRA_CHECK(lra_wsT(c, WS_TMP2, (size_t)ell * nr, &SE));
+  LRA_CHECK(lra_wsT(c, WS_TMP3, (size_t)ell * k, &Bsk));
+  LRA_CHECK(lra_wsT(c, WS_R, (size_t)k * nr, &X));
+  if (trans == LRA_TRANS_N) {
+    LRA_CHECK(lra_launch_gather_cols(c->stream, m, k, A, lda, sk, C, P));
+    LRA_CHECK(lra_launch_gather_cols(c->stream, m, nr, A, lda, rd, E, P));
+  } else {
+    LRA_CHECK(lra_launch_gather_rows_t(c->stream, n, k, A, lda, sk, C, P));
+    LRA_CHECK(lra_launch_gather_rows_t(c->stream, n, nr, A, lda, rd, E, P));
+  }
+  // E -= C * T0   (FP64 MFMA)
+  GemmArgs g;
+  g.M = P; g.N = nr; g.K = k; g.alpha = -1.0; g.beta = 1.0;
+  g.A = C; g.lda = P; g.transA = 0; g.B = T; g.ldb = ldt; g.transB = 0;
+  g.C = E; g.ldc = P; g.bgather = nullptr; g.prec = LRA_PREC_F64;
+  LRA_CHECK(lra_gemm(c, g));
+  // S*E with the accepted round's Omega, low precision
+  LRA_CHECK(lra_sketch_any(c, LRA_SKETCH_RANDN, LRA_TRANS_N, P, nr, E, P, ell, o->seed,
+                           (uint64_t)f.stream_id, f.prec, SE, ell));
+  // X = (S*op(A)[:, sk])' * (S*E), S*op(A)[:, sk] = saved sketch columns sk
+  LRA_CHECK(lra_launch_gather_cols(c->stream, ell, k, f.Bsave, ell, sk, Bsk, ell));
+  g.M = k; g.N = nr; g.K = ell; g.alpha = 1.0; g.beta = 0.0;
+  g.A = Bsk; g.lda = ell; g.transA = 1; g.B = SE; g.ldb = ell; g.transB = 0;
+  g.C = X; g.ldc = k; g.prec = LRA_PREC_F64;
+  LRA_CHECK(lra_gemm(c, g));
+  // dT = R11^{-1} R11^{-T} X, then T += dT
+  LRA_CHECK(lra_launch_trsm_upper(c->stream, k, nr, f.B, ell, X, k, 1));
+  LRA_CHECK(lra_launch_trsm_upper(c->stream, k, nr, f.B, ell, X, k, 0));
+  LRA_CHECK(lra_launch_axpy2d(c->stream, k, nr, 1.0, X, k, T, ldt));
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// idfact (src/id.jl:434-447)
+
+extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                          int64_t lda, const lra_opts* opts, lra_id_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(A != nullptr || m * n == 0, 5);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(out != nullptr, 8);
+  if (chkopts(opts) != LRA_OK) return -7;
+  lra_opts o = *opts;
+  o.pqrfact_retval = LRA_RET_T;
+  const i64 N = trans == LRA_TRANS_N ? n : m;
+  SketchFactOut f{};
+  if (o.sketch == LRA_SKETCH_NONE) {
+    // pqrfact_none: rank-adaptive QRCP of op(A) itself (src/pqr.jl:323-327,343-346)
+    const i64 P = trans == LRA_TRANS_N ? m : n;
+    double* B = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)std::max<i64>(P * N, 1), &B));
+    if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, A, lda, B, m));
+    else LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, B, n));
+    i64* jpvt = nullptr;
+    double* tau = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_JPVT, (size_t)std::max<i64>(N, 1), &jpvt));
+    LRA_CHECK(lra_wsT(c, WS_TAU, (size_t)std::max<i64>(std::min(P, N), 1), &tau));
+    i64 k = 0;
+    LRA_CHECK(lra_geqp3_adap_f64(c, P, N, B, std::max<i64>(P, 1), o.rank, o.atol, o.rtol, o.nb,
+                                 jpvt, tau, &k));
+    f.B = B; f.Bsave = nullptr; f.ell = P; f.N = N; f.jpvt = jpvt; f.tau = tau; f.k = k;
+    f.rounds = 1; f.prec = LRA_PREC_F64; f.stream_id = 0;
+  } else {
+    LRA_CHECK(sketchfact_left(c, trans, m, n, A, lda, &o, &f));
+  }
+  const i64 k = f.k;
+  LRA_CHECK(lra_launch_split_ids(c->stream, N, k, f.jpvt, out->sk, out->rd));
+  if (!out->T) {
+    // library-owned T (valid until the next call on this context), compact ldt = k
+    double* t = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_T, (size_t)std::max<i64>(k, 1) * std::max<i64>(N - k, 1), &t));
+    out->T = t;
+    out->ldt = std::max<i64>(k, 1);
+  }
+  if (k > 0 && N > k) {
+    LRA_ARG(out->ldt >= k, 8);
+    LRA_CHECK(lra_launch_id_T(c->stream, k, N, f.B, f.ell, out->T, out->ldt));
+    if (f.prec != LRA_PREC_F64 && f.Bsave)
+      LRA_CHECK(mixed_refine_T(c, trans, m, n, A, lda, f, out->sk, out->rd, out->T, out->ldt, &o));
+    if (o.maxdet_tol >= 0) {
+      // strong-RRQR swaps (pqrback_postproc with maxdet, src/pqr.jl:426-433)
+      i64* p = nullptr;
+      LRA_CHECK(lra_wsT(c, WS_TMP3, (size_t)N, &p));
+      LRA_HIP(hipMemcpyAsync(p, out->sk, sizeof(i64) * k, hipMemcpyDeviceToDevice, c->stream));
+      LRA_HIP(hipMemcpyAsync(p + k, out->rd, sizeof(i64) * (N - k), hipMemcpyDeviceToDevice, c->stream));
+      i64 it = 0;
+      LRA_CHECK(lra_maxdet_swapcols_f64(c, k, N, p, out->T, out->ldt, o.maxdet_tol, o.maxdet_niter, &it));
+      LRA_HIP(hipMemcpyAsync(out->sk, p, sizeof(i64) * k, hipMemcpyDeviceToDevice, c->stream));
+      LRA_HIP(hipMemcpyAsync(out->rd, p + k, sizeof(i64) * (N - k), hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  out->k = k;
+  out->rounds = f.rounds;
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// sketch dispatch (sketch(), src/sketch.jl:35-50) -- left side, op(A) = A or A'
+
+int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                         i64 ell, const double* d, const i64* idx, uint64_t seed,
+                         uint64_t stream_id, double* B, i64 ldb);
+int lra_sketch_subset_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                           i64 ell, const i64* r, uint64_t seed, uint64_t stream_id, double* B,
+                           i64 ldb);
+int lra_sketch_sprn_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                         i64 ell, const i64* r, const double* s, uint64_t seed,
+                         uint64_t stream_id, double* B, i64 ldb);
+
+int lra_sketch_any(lra_ctx* c, int sketch, int trans, i64 m, i64 n, const double* A, i64 lda,
+                   i64 ell, uint64_t seed, uint64_t stream_id, int prec, double* B, i64 ldb) {
+  switch (sketch) {
+    case LRA_SKETCH_RANDN:
+      return sketch_gaussian_impl(c, LRA_SIDE_LEFT, trans, m, n, A, lda, ell, nullptr, seed,
+                                  stream_id, prec, B, ldb);
+    case LRA_SKETCH_SRFT:
+      return lra_sketch_srft_impl(c, trans, m, n, A, lda, ell, nullptr, nullptr, seed,
+                                  stream_id, B, ldb);
+    case LRA_SKETCH_SUB:
+      return lra_sketch_subset_impl(c, trans, m, n, A, lda, ell, nullptr, seed, stream_id, B,
+                                    ldb);
+    case LRA_SKETCH_SPRN:
+      return lra_sketch_sprn_impl(c, trans, m, n, A, lda, ell, nullptr, nullptr, seed,
+                                  stream_id, B, ldb);
+  }
+  lra_set_error("unknown sketch %d", sketch);
+  return LRA_E_UNSUPPORTED;
+}

@@ -1,0 +1,461 @@
+// gemm.hip -- MFMA GEMM for the sketches and the factor products (gfx950).
+//
+// C (M x N) = alpha * op(A) (M x K) * op(B) (K x N) + beta * C, FP64 in and out.  Replaces the
+// dgemm behind mul!(C, S, A) in the Gaussian sketch (src/sketch.jl:91-110) and the products of
+// the factorizations.  Arithmetic (prec):
+//   F64    v_mfma_f64_16x16x4f64, FP64 accumulate (the drop-in);
+//   F32    operands rounded to FP32, v_mfma_f32_16x16x4f32, FP32 accumulate;
+//   BF16   operands rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulate;
+//   BF16X3 each operand split into three BF16 terms (hi + mid + lo ~ 24 bits), the six
+//          products of order <= 2 accumulated in FP32 (~FP32 accuracy at BF16 MFMA rate).
+// Tiles are staged through LDS with k contiguous ([m][k] and [n][k]), so the sketch case
+// (op(A) = Omega stored K x ell, op(B) = A stored K x N) is a straight copy; the next k-tile
+// is prefetched into registers while the current one is multiplied.  Blocks that share a
+// B tile (the M tiles of one N tile) are mapped to the same XCD so A is fetched once into
+// that XCD's L2.  Split-K writes FP64 partials that a second kernel sums in fixed order.
+#include "lra_internal.h"
+
+#include <algorithm>
+#include <map>
+#include <type_traits>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <int PREC> struct PT;
+template <> struct PT<LRA_PREC_F64> { typedef double E; static const int P = 1; static const int BK = 16; static const int PAD = 1; };
+template <> struct PT<LRA_PREC_F32> { typedef float E; static const int P = 1; static const int BK = 32; static const int PAD = 1; };
+template <> struct PT<LRA_PREC_BF16> { typedef __bf16 E; static const int P = 1; static const int BK = 32; static const int PAD = 8; };
+template <> struct PT<LRA_PREC_BF16X3> { typedef __bf16 E; static const int P = 3; static const int BK = 32; static const int PAD = 8; };
+
+struct GemmDev {
+  int M, N, K;
+  double alpha, beta;
+  const double* A; i64 lda; int transA;
+  const double* B; i64 ldb; int transB;
+  double* C; i64 ldc;
+  const i64* bgather;
+  double* partial;  // split-K partial sums [split][N][M]... laid out as split*(M*N) col-major M x N
+  int splits, kchunk;
+  int MT, NT, NG;  // M tiles, N tiles, groups (= NT * splits)
+};
+
+__device__ __forceinline__ void split3(double x, __bf16* h, __bf16* m, __bf16* l) {
+  const __bf16 hi = (__bf16)(float)x;
+  const double r1 = x - (double)(float)hi;
+  const __bf16 mi = (__bf16)(float)r1;
+  const double r2 = r1 - (double)(float)mi;
+  *h = hi; *m = mi; *l = (__bf16)(float)r2;
+}
+
+template <int PREC, int BM, int BN, int WAVES_M, int WAVES_N>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
+  typedef typename PT<PREC>::E E;
+  constexpr int P = PT<PREC>::P;
+  constexpr int BK = PT<PREC>::BK;
+  constexpr int LDK = BK + PT<PREC>::PAD;  // leading dimension (elements) of an LDS row
+  constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int LA = BM * BK / 256;  // elements of A tile per thread
+  constexpr int LB = BN * BK / 256;
+  static_assert(WAVES_M * WAVES_N == 4, "4 waves");
+  static_assert(LA * 256 == BM * BK && LB * 256 == BN * BK, "tile/threads");
+
+  __shared__ E sA[P][BM * LDK];
+  __shared__ E sB[P][BN * LDK];
+
+  // ---- XCD-aware block -> (group, mt) mapping
+  const int b = blockIdx.x;
+  const int xcd = b & 7, idx = b >> 3;
+  const int grp = (idx / p.MT) * 8 + xcd;
+  const int mt = idx % p.MT;
+  if (grp >= p.NG) return;
+  const int nt = grp % p.NT, split = grp / p.NT;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int kbeg = split * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WAVES_N, wn = wave % WAVES_N;
+
+  // ---- staging map: thread loads LA elements of the A tile and LB of the B tile.
+  // k-contiguous source (trans = 1 for A, 0 for B): element e -> (row r, k c) with k fastest.
+  // m/n-contiguous source: element e -> row fastest.
+  double ra[LA], rb[LB];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      int r, c;
+      if (p.transA) { r = e / BK; c = e % BK; } else { r = e % BM; c = e / BM; }
+      const int gm = m0 + r, gk = k0 + c;
+      double v = 0.0;
+      if (gm < p.M && gk < kend)
+        v = p.transA ? p.A[gk + (i64)gm * p.lda] : p.A[gm + (i64)gk * p.lda];
+      ra[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      int r, c;
+      if (!p.transB) { r = e / BK; c = e % BK; } else { r = e % BN; c = e / BN; }
+      const int gn = n0 + r, gk = k0 + c;
+      double v = 0.0;
+      if (gn < p.N && gk < kend) {
+        const i64 kk = p.bgather ? (p.bgather[gk] - 1) : gk;
+        v = p.transB ? p.B[gn + kk * p.ldb] : p.B[kk + (i64)gn * p.ldb];
+      }
+      rb[t] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      int r, c;
+      if (p.transA) { r = e / BK; c = e % BK; } else { r = e % BM; c = e / BM; }
+      const int o = r * LDK + c;
+      if constexpr (PREC == LRA_PREC_BF16X3) {
+        __bf16 h, mi, l;
+        split3(ra[t], &h, &mi, &l);
+        sA[0][o] = h; sA[1][o] = mi; sA[2][o] = l;
+      } else if constexpr (PREC == LRA_PREC_BF16) {
+        sA[0][o] = (__bf16)(float)ra[t];
+      } else {
+        sA[0][o] = (E)ra[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      int r, c;
+      if (!p.transB) { r = e / BK; c = e % BK; } else { r = e % BN; c = e / BN; }
+      const int o = r * LDK + c;
+      if constexpr (PREC == LRA_PREC_BF16X3) {
+        __bf16 h, mi, l;
+        split3(rb[t], &h, &mi, &l);
+        sB[0][o] = h; sB[1][o] = mi; sB[2][o] = l;
+      } else if constexpr (PREC == LRA_PREC_BF16) {
+        sB[0][o] = (__bf16)(float)rb[t];
+      } else {
+        sB[0][o] = (E)rb[t];
+      }
+    }
+  };
+
+  typedef typename std::conditional<PREC == LRA_PREC_F64, d4, f4>::type AccT;
+  AccT acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
+
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int am0 = wm * WTM, bn0 = wn * WTN;
+
+  auto compute = [&]() {
+    if constexpr (PREC == LRA_PREC_F64 || PREC == LRA_PREC_F32) {
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        E af[TM], bfv[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) af[i] = sA[0][(am0 + i * 16 + l16) * LDK + kk + lq];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bfv[j] = sB[0][(bn0 + j * 16 + l16) * LDK + kk + lq];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (PREC == LRA_PREC_F64)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bfv[j], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bfv[j], acc[i][j], 0, 0, 0);
+          }
+      }
+    } else {
+      // one 16x16x32 step per BK = 32
+      bf8 af[P][TM], bfv[P][TN];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          af[q][i] = *reinterpret_cast<const bf8*>(&sA[q][(am0 + i * 16 + l16) * LDK + lq * 8]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          bfv[q][j] = *reinterpret_cast<const bf8*>(&sB[q][(bn0 + j * 16 + l16) * LDK + lq * 8]);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          if constexpr (PREC == LRA_PREC_BF16) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfv[0][j], acc[i][j], 0, 0, 0);
+          } else {
+            // smallest terms first: lo*hi, hi*lo, mid*mid, mid*hi, hi*mid, hi*hi
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][i], bfv[0][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfv[2][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfv[1][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfv[0][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfv[1][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfv[0][j], acc[i][j], 0, 0, 0);
+          }
+        }
+    }
+  };
+
+  if (kbeg < kend) {
+    load_tiles(kbeg);
+    store_tiles();
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) load_tiles(k0 + BK);
+      compute();
+      __syncthreads();
+      if (more) {
+        store_tiles();
+        __syncthreads();
+      }
+    }
+  }
+
+  // ---- epilogue.  Output lane layout of a 16x16 tile (column l16 in every case):
+  //   f32/bf16 MFMA: register r holds row 4*lq + r;  f64 MFMA: register r holds row lq + 4*r.
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + bn0 + j * 16 + l16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + am0 + i * 16 + (PREC == LRA_PREC_F64 ? lq + 4 * r : lq * 4 + r);
+        if (row < p.M && col < p.N) {
+          const double v = (double)acc[i][j][r];
+          if (p.splits > 1) {
+            p.partial[(i64)split * p.M * p.N + row + (i64)col * p.M] = v;
+          } else {
+            double* c = p.C + row + (i64)col * p.ldc;
+            *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+          }
+        }
+      }
+    }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, int splits, const double* part, double alpha,
+                                     double beta, double* C, i64 ldc) {
+  const i64 total = (i64)M * N;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
+       e += (i64)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < splits; ++q) s += part[(i64)q * total + e];
+    const i64 row = e % M, col = e / M;
+    double* c = C + row + col * ldc;
+    *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+  }
+}
+
+// ---- configurations -------------------------------------------------------------------
+struct Cfg { int id, BM, BN; };
+// id -> (BM, BN, WAVES_M, WAVES_N)
+static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 256}, {4, 128, 128}};
+
+template <int PREC>
+hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
+  switch (id) {
+    case 0: hipLaunchKernelGGL((gemm_kernel<PREC, 16, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 1: hipLaunchKernelGGL((gemm_kernel<PREC, 32, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 2: hipLaunchKernelGGL((gemm_kernel<PREC, 48, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 3: hipLaunchKernelGGL((gemm_kernel<PREC, 64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 4: hipLaunchKernelGGL((gemm_kernel<PREC, 128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_prec(int prec, int id, const GemmDev& d, int blocks, hipStream_t st) {
+  switch (prec) {
+    case LRA_PREC_F64: return launch_cfg<LRA_PREC_F64>(id, d, blocks, st);
+    case LRA_PREC_F32: return launch_cfg<LRA_PREC_F32>(id, d, blocks, st);
+    case LRA_PREC_BF16: return launch_cfg<LRA_PREC_BF16>(id, d, blocks, st);
+    case LRA_PREC_BF16X3: return launch_cfg<LRA_PREC_BF16X3>(id, d, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+int bk_of(int prec) { return prec == LRA_PREC_F64 ? 16 : 32; }
+
+// Autotuner: the first call for a shape class times every candidate (config, splits) on the
+// live operands and caches the fastest; later calls reuse the choice.
+struct TuneKey {
+  int prec, transA, transB, mb, nb, kb;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, transA, transB, mb, nb, kb) <
+           std::tie(o.prec, o.transA, o.transB, o.mb, o.nb, o.kb);
+  }
+};
+struct TuneVal { int cfg, splits; };
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+int g_autotune = -1;
+
+int bucket(i64 x) {  // power-of-two bucket
+  int b = 0;
+  while ((1ll << b) < x) ++b;
+  return b;
+}
+
+int prepare(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits, GemmDev* d, int* blocks) {
+  const Cfg& c = kCfgs[cfg];
+  d->M = (int)g.M; d->N = (int)g.N; d->K = (int)g.K;
+  d->alpha = g.alpha; d->beta = g.beta;
+  d->A = g.A; d->lda = g.lda; d->transA = g.transA;
+  d->B = g.B; d->ldb = g.ldb; d->transB = g.transB;
+  d->C = g.C; d->ldc = g.ldc; d->bgather = g.bgather;
+  const int bk = bk_of(g.prec);
+  i64 kchunk = 0;
+  if (g.K <= 0) {
+    splits = 1;
+  } else {
+    kchunk = (g.K + splits - 1) / splits;
+    kchunk = (kchunk + bk - 1) / bk * bk;
+    splits = (int)((g.K + kchunk - 1) / kchunk);
+  }
+  d->splits = splits; d->kchunk = (int)kchunk;
+  d->MT = (int)((g.M + c.BM - 1) / c.BM);
+  d->NT = (int)((g.N + c.BN - 1) / c.BN);
+  d->NG = d->NT * splits;
+  d->partial = nullptr;
+  if (splits > 1) {
+    double* part = nullptr;
+    LRA_CHECK(lra_wsT(ctx, WS_SPLITK, (size_t)splits * g.M * g.N, &part));
+    d->partial = part;
+  }
+  const i64 groups8 = ((i64)d->NG + 7) / 8;
+  const i64 nblk = groups8 * 8 * d->MT;
+  if (nblk > 2147483647ll) { lra_set_error("gemm: grid too large"); return LRA_E_UNSUPPORTED; }
+  *blocks = (int)nblk;
+  return LRA_OK;
+}
+
+int run(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits) {
+  GemmDev d;
+  int blocks = 0;
+  LRA_CHECK(prepare(ctx, g, cfg, splits, &d, &blocks));
+  hipError_t e = launch_prec(g.prec, cfg, d, blocks, ctx->stream);
+  if (e != hipSuccess) {
+    lra_set_error("gemm launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  if (d.splits > 1) {
+    const i64 total = g.M * g.N;
+    int rb = (int)std::min<i64>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb), dim3(256), 0, ctx->stream, (int)g.M,
+                       (int)g.N, d.splits, d.partial, g.alpha, g.beta, g.C, g.ldc);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+      lra_set_error("splitk reduce launch: %s", hipGetErrorString(e));
+      return LRA_E_HIP;
+    }
+  }
+  return LRA_OK;
+}
+
+void candidates(const GemmArgs& g, int num_cu, std::vector<TuneVal>* out) {
+  for (int cfg = 0; cfg < 5; ++cfg) {
+    const Cfg& c = kCfgs[cfg];
+    // skip configs that waste most of the M tile
+    if (c.BM > 16 && g.M <= c.BM / 2) continue;
+    if (cfg == 4 && g.M < 96) continue;
+    const i64 tiles = ((g.M + c.BM - 1) / c.BM) * ((g.N + c.BN - 1) / c.BN);
+    for (int s = 1; s <= 32; s *= 2) {
+      if (s > 1 && tiles * s > 8ll * num_cu) break;
+      if (s > 1 && g.K / s < 256) break;
+      out->push_back({cfg, s});
+    }
+  }
+  if (out->empty()) out->push_back({4, 1});
+}
+
+TuneVal heuristic(const GemmArgs& g, int num_cu) {
+  int cfg = 4;
+  if (g.M <= 16) cfg = 0;
+  else if (g.M <= 32) cfg = 1;
+  else if (g.M <= 48) cfg = 2;
+  else if (g.M <= 64) cfg = 3;
+  const Cfg& c = kCfgs[cfg];
+  const i64 tiles = ((g.M + c.BM - 1) / c.BM) * ((g.N + c.BN - 1) / c.BN);
+  int s = 1;
+  while (tiles * s < 2ll * num_cu && g.K / (2 * s) >= 512 && s < 32) s *= 2;
+  return {cfg, s};
+}
+
+}  // namespace
+
+int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
+  if (g.M <= 0 || g.N <= 0) return LRA_OK;
+  if (g.M > 2147483647ll || g.N > 2147483647ll || g.K > 2147483647ll) {
+    lra_set_error("gemm: dimension exceeds int32");
+    return LRA_E_UNSUPPORTED;
+  }
+  if (g_autotune < 0) {
+    const char* env = getenv("LRA_AUTOTUNE");
+    g_autotune = (env && env[0] == '0') ? 0 : 1;
+  }
+  TuneKey key{g.prec, g.transA, g.transB, bucket(g.M), bucket(g.N), bucket(g.K)};
+  TuneVal choice;
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) { choice = it->second; have = true; }
+  }
+  if (!have) {
+    if (!g_autotune || g.beta != 0.0 || (g.M * g.N * (g.K > 0 ? g.K : 1)) < (1ll << 24)) {
+      choice = heuristic(g, ctx->num_cu);
+    } else {
+      std::vector<TuneVal> cands;
+      candidates(g, ctx->num_cu, &cands);
+      hipEvent_t e0, e1;
+      LRA_HIP(hipEventCreate(&e0));
+      LRA_HIP(hipEventCreate(&e1));
+      float best = 1e30f;
+      choice = cands[0];
+      for (const TuneVal& c : cands) {
+        LRA_CHECK(run(ctx, g, c.cfg, c.splits));  // warm
+        LRA_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 2; ++r) LRA_CHECK(run(ctx, g, c.cfg, c.splits));
+        LRA_HIP(hipEventRecord(e1, ctx->stream));
+        LRA_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LRA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; choice = c; }
+      }
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = choice;
+  }
+  return run(ctx, g, choice.cfg, choice.splits);
+}
+
+// Report the configuration chosen for a shape (for the bench's roofline bookkeeping).
+extern "C" int lra_gemm_choice(int prec, int transA, int transB, int64_t M, int64_t N,
+                               int64_t K, int* cfg, int* splits, int* bm, int* bn) {
+  TuneKey key{prec, transA, transB, bucket(M), bucket(N), bucket(K)};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return LRA_E_STATE;
+  *cfg = it->second.cfg;
+  *splits = it->second.splits;
+  *bm = kCfgs[it->second.cfg].BM;
+  *bn = kCfgs[it->second.cfg].BN;
+  return LRA_OK;
+}

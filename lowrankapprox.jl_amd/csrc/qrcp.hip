@@ -1,0 +1,511 @@
+// qrcp.hip -- early-terminating Householder QR with column pivoting, register-resident.
+//
+// Replaces the geqp3_adap_main! loop over ?laqps blocks (src/pqr.jl:361-418 calling
+// src/lapack.jl:129-136) and, in mode 1, a single ?laqps block.
+//
+// Design (gfx950): the ell x n sketch is spread over a co-resident grid of G workgroups;
+// every column lives in the registers of `tpc` lanes (lane lic holds rows lic*RPT ..
+// lic*RPT+RPT-1, so every per-row LDS offset is an immediate), so a pivot
+// step never touches HBM.  Each step: every workgroup publishes its best remaining column
+// (norm, position, data), one grid barrier, every workgroup picks the same global winner
+// (largest norm, lowest position -- idamax's first-index rule), computes the same reflector
+// redundantly in LDS, and applies it to its own columns at once (right-looking, unblocked).
+// Exact arithmetic is identical to dlaqps' deferred-update scheme; the block structure of
+// geqp3_adap_main! is emulated so the rank-termination test runs at the same points:
+// blocks of min(nb, k-j+1) columns that end early at the first step whose norm downdate
+// needed a recompute (dlaqps' lsticc), and after each block the |R(jn-1,jn-1)| <= ptol test
+// returns the first small diagonal of the block.  Columns are never moved during the
+// factorization; each carries its LAPACK position (jpvt order) and is written to it at the
+// end.
+#include "lra_internal.h"
+
+#include <algorithm>
+#include <climits>
+
+namespace {
+
+struct Slot {
+  double val;
+  int pos;
+  int owner;
+  int lc;
+  int flag;
+  int pad[2];
+};
+
+struct QrcpDev {
+  double* A;
+  i64 lda;
+  int m, n, offset, kmax, nb, mode, stop_after, lastrk;
+  double atol, rtol;
+  i64* jpvt;
+  double* tau;
+  double* vn1g;
+  double* vn2g;
+  i64* k_out;
+  int* early_out;
+  int G, cpw, tpc, cand_stride;
+  unsigned* bar;
+  Slot* slots;
+  double* cand;
+  int* err;
+  int* abort_flag;
+};
+
+constexpr int NB_MAX = 1024;
+
+LRA_DEV double lapy2(double x, double y) {
+  const double xa = fabs(x), ya = fabs(y);
+  const double w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == 0.0) return w;
+  const double q = z / w;
+  return w * sqrt(1.0 + q * q);
+}
+
+template <int RPT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
+  constexpr int NW = BLOCK / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tpc = p.tpc;
+  const int gpw = 64 / tpc;
+  const int lic = lane & (tpc - 1);
+  const int lc = wave * gpw + lane / tpc;
+  const int g = blockIdx.x;
+  const int col = g * p.cpw + lc;
+  const bool have = (lc < p.cpw) && (col < p.n);
+  const int m = p.m;
+  const int rbase = lic * RPT;  // first row held by this lane
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(2^-53) = sqrt(dlamch('E'))
+
+  __shared__ double s_x[64 * RPT];
+  __shared__ double s_diag[NB_MAX];
+  __shared__ double s_wval[NW];
+  __shared__ int s_wpos[NW], s_wlc[NW], s_wflag[NW];
+  __shared__ double s_tau, s_beta, s_ptol, s_winval;
+  __shared__ int s_winowner, s_winlc, s_winpos, s_flagprev, s_blc;
+
+  // s_x holds the current reflector by ABSOLUTE row: zeros above row rk and in the padding,
+  // so the dot/update loops below need no per-row predicates.
+  for (int t = tid; t < 64 * RPT; t += BLOCK) s_x[t] = 0.0;
+  double a[RPT];
+  {
+    const double* src = p.A + (i64)col * p.lda + rbase;
+    const int hi = have ? m - rbase : 0;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) a[i] = (i < hi) ? src[i] : 0.0;
+  }
+  int pos = col;
+  bool active = have;
+  i64 jp = 0;
+  double vn1 = 0.0, vn2 = 0.0;
+  if (p.mode == 0) {
+    double ss = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) ss += a[i] * a[i];
+    ss = lra_group_sum(ss, tpc);
+    vn1 = vn2 = sqrt(ss);
+    jp = col + 1;
+  } else if (have) {
+    vn1 = p.vn1g[col];
+    vn2 = p.vn2g[col];
+    jp = p.jpvt[col];
+  }
+  int myflag = 0;
+  int j0 = 0;
+  int jb = min(p.nb, p.kmax);
+  int k_result = p.kmax;
+  int early = 0;
+  bool aborted = false;
+
+  int s = 0;
+  for (;; ++s) {
+    // ---------------- local candidate (largest norm, then lowest position) ------------
+    double bv = -1.0;
+    int bp = INT_MAX, blc = -1;
+    if (active) { bv = vn1; bp = pos; blc = lc; }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double ov = __shfl_xor(bv, d, 64);
+      const int op = __shfl_xor(bp, d, 64);
+      const int ol = __shfl_xor(blc, d, 64);
+      if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; blc = ol; }
+    }
+    const int wflag = __any(myflag) ? 1 : 0;
+    if (lane == 0) { s_wval[wave] = bv; s_wpos[wave] = bp; s_wlc[wave] = blc; s_wflag[wave] = wflag; }
+    __syncthreads();
+    // every thread reduces the NW wave candidates the same way
+    double cv = s_wval[0];
+    int cp = s_wpos[0], clc = s_wlc[0], cflag = s_wflag[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      const double ov = s_wval[w];
+      const int op = s_wpos[w];
+      if (ov > cv || (ov == cv && op < cp)) { cv = ov; cp = op; clc = s_wlc[w]; }
+      cflag |= s_wflag[w];
+    }
+    const bool done_steps = (s == p.kmax);
+
+    if (p.G > 1) {
+      // publish the block candidate and its column rows rk..m-1
+      const int rk = p.offset + s;
+      if (!done_steps) {
+        double* dst = p.cand + ((size_t)(s & 1) * p.G + g) * p.cand_stride;
+        if (have && active && lc == clc) {
+          double* d = dst + (rbase - rk);
+          const int lo = rk - rbase, hi = m - rbase;
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (i >= lo && i < hi) d[i] = a[i];
+        }
+      }
+      if (tid == 0) {
+        Slot sl;
+        sl.val = cv; sl.pos = cp; sl.owner = g; sl.lc = clc; sl.flag = cflag;
+        p.slots[(size_t)(s & 1) * p.G + g] = sl;
+      }
+      if (!lra_grid_barrier(p.bar, (unsigned)(s + 1) * (unsigned)p.G, p.err, p.abort_flag)) {
+        aborted = true;
+        break;
+      }
+      if (wave == 0) {
+        double v = -1.0;
+        int vp = INT_MAX, vo = -1, vl = -1, fl = 0;
+        for (int t = lane; t < p.G; t += 64) {
+          const Slot sl = p.slots[(size_t)(s & 1) * p.G + t];
+          fl |= sl.flag;
+          if (sl.val > v || (sl.val == v && sl.pos < vp)) { v = sl.val; vp = sl.pos; vo = sl.owner; vl = sl.lc; }
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const double ov = __shfl_xor(v, d, 64);
+          const int op = __shfl_xor(vp, d, 64);
+          const int oo = __shfl_xor(vo, d, 64);
+          const int ol = __shfl_xor(vl, d, 64);
+          if (ov > v || (ov == v && op < vp)) { v = ov; vp = op; vo = oo; vl = ol; }
+          fl |= __shfl_xor(fl, d, 64);
+        }
+        if (lane == 0) { s_winval = v; s_winpos = vp; s_winowner = vo; s_winlc = vl; s_flagprev = fl; }
+      }
+      __syncthreads();
+    } else {
+      if (tid == 0) { s_winval = cv; s_winpos = cp; s_winowner = 0; s_winlc = clc; s_flagprev = cflag; }
+      __syncthreads();
+    }
+    myflag = 0;
+
+    // ---------------- block bookkeeping for the step just finished (s-1) --------------
+    if (s == 0) {
+      if (tid == 0) s_ptol = fmax(p.atol, p.rtol * s_winval);
+      __syncthreads();
+    } else {
+      const int last = s - 1;
+      const bool size_end = (last - j0 + 1 == jb);
+      const bool blk_end = size_end || (s_flagprev != 0) || done_steps;
+      if (blk_end) {
+        if (p.mode == 1) {
+          k_result = s;
+          break;
+        }
+        const double ptol = s_ptol;
+        if (fabs(s_diag[last - j0]) <= ptol) {
+          int i = j0;
+          for (; i <= last; ++i)
+            if (fabs(s_diag[i - j0]) <= ptol) break;
+          if (i <= last) { k_result = i; break; }
+        }
+        if (done_steps) { k_result = p.kmax; break; }
+        if (p.stop_after > 0 && s >= p.stop_after) { k_result = s; early = 1; break; }
+        j0 = s;
+        jb = min(p.nb, p.kmax - s);
+      }
+    }
+    if (done_steps) { k_result = s; break; }
+
+    // ---------------- winner column -> LDS ---------------------------------------------
+    const int rk = p.offset + s;
+    const int len = m - rk;
+    const int win_owner = s_winowner, win_lc = s_winlc, win_pos = s_winpos;
+    if (tid == 0 && rk > 0) s_x[rk - 1] = 0.0;  // previous step's v(1) = 1
+    if (p.G > 1) {
+      const double* src = p.cand + ((size_t)(s & 1) * p.G + win_owner) * p.cand_stride;
+      for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = src[t];
+    } else {
+      if (have && active && lc == win_lc) {
+        double* d = s_x + rbase;
+        const int lo = rk - rbase, hi = m - rbase;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (i >= lo && i < hi) d[i] = a[i];
+      }
+    }
+    __syncthreads();
+
+    // ---------------- Householder reflector (dlarfg) in wave 0 --------------------------
+    if (wave == 0) {
+      double* x = s_x + rk;
+      double alpha = x[0];
+      double ss = 0.0;
+      for (int t = 1 + lane; t < len; t += 64) ss += x[t] * x[t];
+      ss = lra_wave_sum(ss);
+      double xnorm = sqrt(ss);
+      double tau = 0.0, beta = alpha, scal = 1.0;
+      if (len > 1 && xnorm != 0.0) {
+        beta = -copysign(lapy2(alpha, xnorm), alpha);
+        const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+        int knt = 0;
+        if (fabs(beta) < safmin) {
+          const double rsafmn = 1.0 / safmin;
+          do {
+            ++knt;
+            for (int t = 1 + lane; t < len; t += 64) x[t] *= rsafmn;
+            beta *= rsafmn;
+            alpha *= rsafmn;
+          } while (fabs(beta) < safmin && knt < 20);
+          double s2 = 0.0;
+          for (int t = 1 + lane; t < len; t += 64) s2 += x[t] * x[t];
+          s2 = lra_wave_sum(s2);
+          xnorm = sqrt(s2);
+          beta = -copysign(lapy2(alpha, xnorm), alpha);
+        }
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+        for (int j = 0; j < knt; ++j) beta *= safmin;
+      }
+      if (lane == 0) { s_tau = tau; s_beta = beta; }
+      // scale x into v (v(1) = 1 implicit, stored at s_x[0] below)
+      for (int t = 1 + lane; t < len; t += 64) x[t] *= scal;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_x[rk] = 1.0;
+      if (g == 0) p.tau[s] = s_tau;
+      s_diag[s - j0] = s_beta;
+    }
+    __syncthreads();
+    const double tau = s_tau, beta = s_beta;
+
+    // ---------------- apply H to the block's columns, position swap, norm downdate ------
+    const bool is_win = have && active && (g == win_owner) && (lc == win_lc);
+    if (is_win) {
+      const double* vx = s_x + rbase;
+      const int lo = rk - rbase, hi = m - rbase;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        if (i == lo) a[i] = beta;
+        else if (i > lo && i < hi) a[i] = vx[i];
+      }
+      active = false;
+      pos = s;
+    } else {
+      if (have && pos == s) pos = win_pos;  // LAPACK swaps columns k and pvt
+      if (active) {
+        const double* vx = s_x + rbase;
+        double w = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) w += vx[i] * a[i];
+        w = lra_group_sum(w, tpc);
+        const double tw = tau * w;
+        // re-read v from LDS below instead of keeping RPT values live across the two loops
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) a[i] -= tw * vx[i];
+        if (rk + 1 < p.lastrk) {
+          // new value of row rk lives in lane rk / RPT of the group, register rk % RPT
+          const int irk = rk % RPT;
+          double xr = 0.0;
+#pragma unroll
+          for (int i = 0; i < RPT; ++i) xr = (i == irk) ? a[i] : xr;
+          xr = __shfl(xr, (lane & ~(tpc - 1)) | (rk / RPT), 64);
+          if (vn1 != 0.0) {
+            double temp = fabs(xr) / vn1;
+            temp = (1.0 + temp) * (1.0 - temp);
+            temp = temp < 0.0 ? 0.0 : temp;
+            const double r = vn1 / vn2;
+            const double temp2 = temp * r * r;
+            if (temp2 <= tol3z) {
+              double ss = 0.0;
+              const int lo = rk - rbase;
+#pragma unroll
+              for (int i = 0; i < RPT; ++i) ss += (i > lo) ? a[i] * a[i] : 0.0;
+              ss = lra_group_sum(ss, tpc);
+              vn1 = vn2 = sqrt(ss);
+              myflag = 1;
+            } else {
+              vn1 *= sqrt(temp);
+            }
+          }
+        }
+      }
+    }
+    // s_x is rewritten next step only after the next __syncthreads
+  }
+
+  if (aborted) return;
+  // ---------------- write-out to LAPACK positions --------------------------------------
+  // (with G > 1 every block loaded its columns before step 0's barrier, so the in-place
+  // permuted write cannot overwrite a column somebody still has to read)
+  if (have) {
+    double* dst = p.A + (i64)pos * p.lda + rbase;
+    const int hi = m - rbase;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+      if (i < hi) dst[i] = a[i];
+    if (lic == 0) {
+      p.jpvt[pos] = jp;
+      if (p.mode == 1) { p.vn1g[pos] = vn1; p.vn2g[pos] = vn2; }
+    }
+  }
+  if (g == 0 && tid == 0) {
+    *p.k_out = k_result;
+    if (p.early_out) *p.early_out = early;
+  }
+}
+
+template <int RPT, int BLOCK>
+hipError_t launch_rpt(const QrcpDev& d, hipStream_t st, bool coop) {
+  dim3 grid(d.G), block(BLOCK);
+  if (coop) {
+    void* args[] = {(void*)&d};
+    return hipLaunchCooperativeKernel((const void*)qrcp_kernel<RPT, BLOCK>, grid, block, args,
+                                      0, st);
+  }
+  hipLaunchKernelGGL((qrcp_kernel<RPT, BLOCK>), grid, block, 0, st, d);
+  return hipGetLastError();
+}
+
+#define LRA_RPT_256(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48) \
+  X(52) X(56) X(60) X(64) X(68) X(72) X(76) X(80)
+#define LRA_RPT_512(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48)
+#define LRA_RPT_1024(X) X(4) X(8) X(12) X(16) X(20)
+
+hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st, bool coop) {
+  if (block == 1024) {
+    switch (rpt) {
+#define CASE(R) case R: return launch_rpt<R, 1024>(d, st, coop);
+      LRA_RPT_1024(CASE)
+#undef CASE
+    }
+  } else if (block == 512) {
+    switch (rpt) {
+#define CASE(R) case R: return launch_rpt<R, 512>(d, st, coop);
+      LRA_RPT_512(CASE)
+#undef CASE
+    }
+  } else {
+    switch (rpt) {
+#define CASE(R) case R: return launch_rpt<R, 256>(d, st, coop);
+      LRA_RPT_256(CASE)
+#undef CASE
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+int occupancy_any(int rpt, int block, int* nb) {
+  hipError_t e = hipErrorInvalidValue;
+  if (block == 1024) {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 1024>, 1024, 0); break;
+      LRA_RPT_1024(CASE)
+#undef CASE
+    }
+  } else if (block == 512) {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 512>, 512, 0); break;
+      LRA_RPT_512(CASE)
+#undef CASE
+    }
+  } else {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 256>, 256, 0); break;
+      LRA_RPT_256(CASE)
+#undef CASE
+    }
+  }
+  return e == hipSuccess ? 0 : 1;
+}
+
+}  // namespace
+
+// Shape selection: single 1024-thread block when the panel fits one CU's registers, else a
+// cooperative grid of 256-thread blocks with `tpc` lanes per column.
+struct QrcpShape {
+  int tpc, rpt, block, G, cpw;
+};
+
+static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
+  for (int tpc = 1; tpc <= 64; tpc <<= 1) {
+    int rpt = (m + tpc - 1) / tpc;
+    rpt = (rpt + 3) / 4 * 4;
+    if (rpt < 4) rpt = 4;
+    if (rpt <= 20 && n <= 1024 / tpc) {
+      *out = {tpc, rpt, 1024, 1, 1024 / tpc};
+      return true;
+    }
+    if (rpt <= 48 && n <= 512 / tpc) {
+      *out = {tpc, rpt, 512, 1, 512 / tpc};
+      return true;
+    }
+  }
+  for (int tpc = 1; tpc <= 64; tpc <<= 1) {
+    int rpt = (m + tpc - 1) / tpc;
+    rpt = (rpt + 3) / 4 * 4;
+    if (rpt < 4) rpt = 4;
+    if (rpt > 80) continue;
+    const int cpw = 256 / tpc;
+    const int G = (n + cpw - 1) / cpw;
+    int per_cu = 0;
+    if (occupancy_any(rpt, 256, &per_cu) != 0 || per_cu < 1) continue;
+    if (G == 1 || G <= per_cu * ctx->num_cu) {
+      *out = {tpc, rpt, 256, G, cpw};
+      return true;
+    }
+  }
+  return false;
+}
+
+int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
+  if (q.kmax <= 0) {
+    // nothing to factor: k = 0 (and jpvt = identity in mode 0)
+    if (q.mode == 0 && q.n > 0) LRA_CHECK(lra_launch_fill_i64(ctx->stream, q.jpvt, q.n, 1));
+    LRA_HIP(hipMemsetAsync(q.k_out, 0, sizeof(i64), ctx->stream));
+    if (q.early_out) LRA_HIP(hipMemsetAsync(q.early_out, 0, sizeof(int), ctx->stream));
+    return LRA_OK;
+  }
+  if (q.nb > NB_MAX) {
+    lra_set_error("qrcp: nb=%d exceeds %d", q.nb, NB_MAX);
+    return LRA_E_UNSUPPORTED;
+  }
+  QrcpShape sh;
+  if (!choose_shape(ctx, q.m, q.n, &sh)) {
+    lra_set_error("qrcp: panel %d x %d does not fit the register-resident engine", q.m, q.n);
+    return LRA_E_UNSUPPORTED;
+  }
+  QrcpDev d;
+  d.A = q.A; d.lda = q.lda; d.m = q.m; d.n = q.n; d.offset = q.offset; d.kmax = q.kmax;
+  d.nb = q.nb; d.mode = q.mode; d.stop_after = q.stop_after;
+  d.lastrk = std::min(q.m, q.n + q.offset);
+  d.atol = q.atol; d.rtol = q.rtol; d.jpvt = q.jpvt; d.tau = q.tau; d.vn1g = q.vn1;
+  d.vn2g = q.vn2; d.k_out = q.k_out; d.early_out = q.early_out;
+  d.G = sh.G; d.cpw = sh.cpw; d.tpc = sh.tpc;
+  d.cand_stride = ((q.m + 7) / 8) * 8;
+  d.err = ctx->d_err;
+  // cooperative area: [bar u32 x 16][abort int x 16][slots 2*G][cand 2*G*stride]
+  const size_t slot_bytes = sizeof(Slot) * 2 * (size_t)sh.G;
+  const size_t cand_bytes = sizeof(double) * 2 * (size_t)sh.G * d.cand_stride;
+  void* area = nullptr;
+  LRA_CHECK(lra_coop_area(ctx, 256 + slot_bytes + cand_bytes, &area));
+  char* base = (char*)area;
+  d.bar = (unsigned*)base;
+  d.abort_flag = (int*)(base + 64);
+  d.slots = (Slot*)(base + 256);
+  d.cand = (double*)(base + 256 + slot_bytes);
+  LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, 64));
+  hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream, sh.G > 1);
+  if (e != hipSuccess) {
+    lra_set_error("qrcp launch (m=%d n=%d tpc=%d rpt=%d G=%d): %s", q.m, q.n, sh.tpc, sh.rpt,
+                  sh.G, hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}

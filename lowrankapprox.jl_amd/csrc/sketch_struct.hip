@@ -1,0 +1,343 @@
+// sketch_struct.hip -- SRFT, random-subset and sparse-Gaussian sketches (left side).
+//
+// SRFT (src/sketch.jl:334-562, real case): with l = largest divisor of m <= ell and mm = m/l,
+// output slots i = 0, 2, 4, ... of a column hold Re and Im of DFT_m(d .* x) at frequency
+// f = idx[i]-1 (the last slot of an odd ell holds Re only; DC/Nyquist still emit their zero
+// imaginary part, see the always-false `in == 0` test at :425; no 1/sqrt(m) scaling).  The
+// reference evaluates it as an R2HC FFT of length l on each of the mm columns of the l x mm
+// reshape X[p,q] = (d.*x)[p*mm+q] followed by sum_q w^q Xhat[f mod l, q] with w = e^{-2 pi i f/m}.
+// Here: one workgroup per column of A streams X through LDS in slabs of 2*QC columns packed
+// as QC complex sequences, transforms them with an in-place radix-2 FFT (power-of-two l) or a
+// direct DFT (other l), unpacks the two real spectra and accumulates every output frequency in
+// registers.  A is read exactly once (HBM-bound).
+//
+// Random subset (src/sketch.jl:248-257): B[i,:] = op(A)[r[i],:].
+// Sparse Gaussian (src/sketch.jl:571-589): output row i sums p_i = fld(m-i,ell)+1 rows of op(A)
+// taken from consecutive positions of the permutation r with coefficients s, in the
+// reference's order (one thread per output row, deterministic).
+#include "lra_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+constexpr int SRFT_THREADS = 256;
+
+// grid: one block per column of A (trans n).  LDS: Z (l x qc complex) [+ W for the DFT path]
+// + twiddles (l/2 complex).
+__global__ __launch_bounds__(SRFT_THREADS) void srft_kernel(
+    i64 m, const double* __restrict__ A, i64 lda, int l, i64 mm, int qc, int pow2, int logl,
+    const double* __restrict__ d, const i64* __restrict__ idx, int ell, double* __restrict__ B,
+    i64 ldb) {
+  extern __shared__ double lds[];
+  cplx* Z = reinterpret_cast<cplx*>(lds);               // [p][c] row-major, l x qc
+  cplx* W = Z + (size_t)l * qc;                         // DFT path output (l x qc)
+  cplx* tw = pow2 ? W : W + (size_t)l * qc;             // l/2 twiddles e^{-2 pi i j / l}
+  const int tid = threadIdx.x;
+  const i64 col = blockIdx.x;
+  const double* x = A + col * lda;
+  for (int j = tid; j < l / 2 + 1; j += SRFT_THREADS) {
+    double s, c;
+    sincospi(-2.0 * (double)j / (double)l, &s, &c);
+    tw[j] = {c, s};
+  }
+  const int nslots = (ell + 1) / 2;
+  // per-thread output slots: s = tid, tid + 256, ... (<= 4 slots per thread for ell <= 2048)
+  constexpr int MAXS = 4;
+  cplx acc[MAXS], wstep[MAXS];
+  int fcol[MAXS], fconj[MAXS];
+  i64 ff[MAXS];
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) {
+    const int s = tid + u * SRFT_THREADS;
+    acc[u] = {0.0, 0.0};
+    if (s < nslots) {
+      const i64 f = idx[2 * s] - 1;
+      ff[u] = f;
+      fcol[u] = (int)(f % l);
+      fconj[u] = (int)((l - f % l) % l);
+      double sn, cs;
+      sincospi(-2.0 * (double)f / (double)m, &sn, &cs);
+      wstep[u] = {cs, sn};
+    }
+  }
+  const i64 nq = mm;  // real columns of X
+  for (i64 q0 = 0; q0 < nq; q0 += 2 * qc) {
+    __syncthreads();
+    // load slab: Z[p][c] = X[p, q0+2c] + i X[p, q0+2c+1], with signs; bit-reversed p for FFT
+    for (i64 e = tid; e < (i64)l * qc; e += SRFT_THREADS) {
+      const int c = (int)(e % qc), p = (int)(e / qc);
+      const i64 q = q0 + 2 * c;
+      const i64 t0 = (i64)p * mm + q;
+      const double a0 = q < nq ? d[t0] * x[t0] : 0.0;
+      const double a1 = q + 1 < nq ? d[t0 + 1] * x[t0 + 1] : 0.0;
+      int pp = p;
+      if (pow2 && logl > 0) pp = (int)(__brev((unsigned)p) >> (32 - logl));
+      Z[(size_t)pp * qc + c] = {a0, a1};
+    }
+    __syncthreads();
+    cplx* S = Z;
+    if (pow2) {
+      for (int h = 1; h < l; h <<= 1) {
+        const int tstep = l / (2 * h);
+        for (i64 e = tid; e < (i64)(l / 2) * qc; e += SRFT_THREADS) {
+          const int c = (int)(e % qc), bidx = (int)(e / qc);
+          const int grp = bidx / h, k = bidx % h;
+          const int i0 = grp * 2 * h + k, i1 = i0 + h;
+          const cplx w = tw[k * tstep];
+          const cplx a = Z[(size_t)i0 * qc + c];
+          const cplx b = cmul(w, Z[(size_t)i1 * qc + c]);
+          Z[(size_t)i0 * qc + c] = {a.re + b.re, a.im + b.im};
+          Z[(size_t)i1 * qc + c] = {a.re - b.re, a.im - b.im};
+        }
+        __syncthreads();
+      }
+    } else {
+      // direct DFT of length l: W[f][c] = sum_p Z[p][c] e^{-2 pi i f p / l}
+      for (i64 e = tid; e < (i64)l * qc; e += SRFT_THREADS) {
+        const int c = (int)(e % qc), f = (int)(e / qc);
+        double sr = 0.0, si = 0.0;
+        for (int p = 0; p < l; ++p) {
+          const int j = (int)(((i64)f * p) % l);
+          double sn, cs;
+          if (j <= l / 2) { cs = tw[j].re; sn = tw[j].im; }
+          else { cs = tw[l - j].re; sn = -tw[l - j].im; }
+          const cplx z = Z[(size_t)p * qc + c];
+          sr += z.re * cs - z.im * sn;
+          si += z.re * sn + z.im * cs;
+        }
+        W[(size_t)f * qc + c] = {sr, si};
+      }
+      __syncthreads();
+      S = W;
+    }
+    // unpack the two real spectra and accumulate every output frequency
+#pragma unroll
+    for (int u = 0; u < MAXS; ++u) {
+      const int s = tid + u * SRFT_THREADS;
+      if (s >= nslots) continue;
+      // w^{q0} with the exponent reduced mod m (exact), then stepped by w
+      const i64 e0 = (ff[u] * (q0 % m)) % m;
+      double sn, cs;
+      sincospi(-2.0 * (double)e0 / (double)m, &sn, &cs);
+      cplx wq = {cs, sn};
+      cplx a = acc[u];
+      for (int c = 0; c < qc; ++c) {
+        const i64 q = q0 + 2 * c;
+        if (q >= nq) break;
+        const cplx zc = S[(size_t)fcol[u] * qc + c];
+        const cplx zn = S[(size_t)fconj[u] * qc + c];
+        // X1 = (zc + conj(zn)) / 2,  X2 = (zc - conj(zn)) / (2i)
+        const cplx x1 = {0.5 * (zc.re + zn.re), 0.5 * (zc.im - zn.im)};
+        const cplx x2 = {0.5 * (zc.im + zn.im), -0.5 * (zc.re - zn.re)};
+        cplx t = cmul(wq, x1);
+        a.re += t.re; a.im += t.im;
+        wq = cmul(wq, wstep[u]);
+        if (q + 1 < nq) {
+          t = cmul(wq, x2);
+          a.re += t.re; a.im += t.im;
+        }
+        wq = cmul(wq, wstep[u]);
+      }
+      acc[u] = a;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) {
+    const int s = tid + u * SRFT_THREADS;
+    if (s >= nslots) continue;
+    double* y = B + col * ldb;
+    y[2 * s] = acc[u].re;
+    if (2 * s + 1 < ell) y[2 * s + 1] = acc[u].im;
+  }
+}
+
+// B[i, j] = A[r[i]-1, j]
+__global__ void subset_kernel(i64 n, const double* A, i64 lda, int ell, const i64* r, double* B,
+                              i64 ldb) {
+  const i64 total = (i64)ell * n;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
+       e += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(e % ell);
+    const i64 j = e / ell;
+    B[i + j * ldb] = A[(r[i] - 1) + j * lda];
+  }
+}
+
+// one block per column; thread i sums output row i over its group of the permutation
+__global__ void sprn_kernel(i64 m, const double* __restrict__ A, i64 lda, int ell,
+                            const i64* __restrict__ r, const double* __restrict__ s,
+                            double* __restrict__ B, i64 ldb) {
+  const i64 col = blockIdx.x;
+  const double* x = A + col * lda;
+  for (int i = threadIdx.x; i < ell; i += blockDim.x) {
+    // group sizes p_t = fld(m-t, ell) + 1 (t = 1..ell): qq+1 for t <= rr, qq after,
+    // with m = qq*ell + rr (also right for m < ell: qq = 0, empty groups beyond m)
+    const i64 qq = m / ell, rr = m % ell;
+    const i64 t = i;  // groups before this one (1-based row i+1)
+    const i64 start = t <= rr ? t * (qq + 1) : rr * (qq + 1) + (t - rr) * qq;
+    const i64 p = (i + 1) <= rr ? qq + 1 : qq;
+    double acc = 0.0;
+    for (i64 l = 0; l < p; ++l) acc += s[start + l] * x[r[start + l] - 1];
+    B[i + col * ldb] = acc;
+  }
+}
+
+}  // namespace
+
+#define LRA_LAUNCH_CHECK(what)                                                          \
+  do {                                                                                  \
+    hipError_t _e = hipGetLastError();                                                  \
+    if (_e != hipSuccess) {                                                             \
+      lra_set_error("%s launch: %s", what, hipGetErrorString(_e));                      \
+      return LRA_E_HIP;                                                                 \
+    }                                                                                   \
+  } while (0)
+
+// op(A) as a column-major P x Q matrix: trans n uses A directly, trans c a transposed copy.
+static int op_matrix(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                     const double** X, i64* ldx, i64* P, i64* Q) {
+  if (trans == LRA_TRANS_N) {
+    *X = A; *ldx = lda; *P = m; *Q = n;
+    return LRA_OK;
+  }
+  double* t = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_TMP0, (size_t)m * n, &t));
+  LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, t, std::max<i64>(n, 1)));
+  *X = t; *ldx = std::max<i64>(n, 1); *P = n; *Q = m;
+  return LRA_OK;
+}
+
+int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                         i64 ell, const double* d, const i64* idx, uint64_t seed,
+                         uint64_t stream_id, double* B, i64 ldb) {
+  const double* X;
+  i64 ldx, P, Q;
+  LRA_CHECK(op_matrix(c, trans, m, n, A, lda, &X, &ldx, &P, &Q));
+  if (ell <= 0 || Q <= 0) return LRA_OK;
+  if (P <= 0) { lra_set_error("srft: empty operand"); return LRA_E_UNSUPPORTED; }
+  if (ell > 2048) { lra_set_error("srft: ell=%lld > 2048", (long long)ell); return LRA_E_UNSUPPORTED; }
+  if (Q > 2147483647ll) { lra_set_error("srft: too many columns"); return LRA_E_UNSUPPORTED; }
+  if (!d) {
+    double* dd = nullptr;
+    i64* ii = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_LP0, (size_t)P, &dd));
+    LRA_CHECK(lra_wsT(c, WS_LP1, (size_t)ell, &ii));
+    LRA_CHECK(lra_launch_srft_rand(c->stream, seed, stream_id, P, ell, dd, ii));
+    d = dd;
+    idx = ii;
+  }
+  i64 l = std::min<i64>(ell, P);
+  while (P % l) --l;
+  const i64 mm = P / l;
+  const bool pow2 = (l & (l - 1)) == 0;
+  int logl = 0;
+  while ((1ll << logl) < l) ++logl;
+  // packed complex columns per slab: LDS = l*qc*16 (+ l*qc*16 for the DFT path) + twiddles
+  const size_t lds_cap = 120 * 1024;
+  int qc = 16;
+  auto need = [&](int q) {
+    return (size_t)l * q * 16 * (pow2 ? 1 : 2) + (size_t)(l / 2 + 1) * 16;
+  };
+  while (qc > 1 && need(qc) > lds_cap) qc >>= 1;
+  if (need(qc) > lds_cap) { lra_set_error("srft: l=%lld too large", (long long)l); return LRA_E_UNSUPPORTED; }
+  if (need(qc) > 64 * 1024)
+    LRA_HIP(hipFuncSetAttribute((const void*)srft_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)need(qc)));
+  hipLaunchKernelGGL(srft_kernel, dim3((unsigned)Q), dim3(SRFT_THREADS), need(qc), c->stream, P,
+                     X, ldx, (int)l, mm, qc, pow2 ? 1 : 0, logl, d, idx, (int)ell, B, ldb);
+  LRA_LAUNCH_CHECK("srft");
+  return LRA_OK;
+}
+
+int lra_sketch_subset_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                           i64 ell, const i64* r, uint64_t seed, uint64_t stream_id, double* B,
+                           i64 ldb) {
+  const double* X;
+  i64 ldx, P, Q;
+  LRA_CHECK(op_matrix(c, trans, m, n, A, lda, &X, &ldx, &P, &Q));
+  if (ell <= 0 || Q <= 0) return LRA_OK;
+  if (!r) {
+    i64* rr = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_LP1, (size_t)ell, &rr));
+    LRA_CHECK(lra_launch_subset_rand(c->stream, seed, stream_id, P, ell, rr));
+    r = rr;
+  }
+  const i64 total = ell * Q;
+  const int blocks = (int)std::min<i64>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(subset_kernel, dim3(blocks), dim3(256), 0, c->stream, Q, X, ldx, (int)ell,
+                     r, B, ldb);
+  LRA_LAUNCH_CHECK("subset");
+  return LRA_OK;
+}
+
+int lra_sketch_sprn_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                         i64 ell, const i64* r, const double* s, uint64_t seed,
+                         uint64_t stream_id, double* B, i64 ldb) {
+  const double* X;
+  i64 ldx, P, Q;
+  LRA_CHECK(op_matrix(c, trans, m, n, A, lda, &X, &ldx, &P, &Q));
+  if (ell <= 0 || Q <= 0) return LRA_OK;
+  if (!r) {
+    double* ss = nullptr;
+    i64* rr = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_LP0, (size_t)P, &ss));
+    LRA_CHECK(lra_wsT(c, WS_LP1, (size_t)P, &rr));
+    LRA_CHECK(lra_launch_sprn_rand(c->stream, seed, stream_id, P, ss, rr));
+    r = rr;
+    s = ss;
+  }
+  if (Q > 2147483647ll) { lra_set_error("sprn: too many columns"); return LRA_E_UNSUPPORTED; }
+  hipLaunchKernelGGL(sprn_kernel, dim3((unsigned)Q), dim3(256), 0, c->stream, P, X, ldx,
+                     (int)ell, r, s, B, ldb);
+  LRA_LAUNCH_CHECK("sprn");
+  return LRA_OK;
+}
+
+extern "C" int lra_sketch_srft(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                               int64_t lda, int64_t ell, const double* d, const int64_t* idx,
+                               uint64_t seed, uint64_t stream_id, double* B, int64_t ldb) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(ell >= 0, 7);
+  LRA_ARG((d == nullptr) == (idx == nullptr), 9);
+  LRA_ARG(ldb >= std::max<int64_t>(1, ell), 13);
+  return lra_sketch_srft_impl(c, trans, m, n, A, lda, ell, d, idx, seed, stream_id, B, ldb);
+}
+
+extern "C" int lra_sketch_subset(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                                 int64_t lda, int64_t ell, const int64_t* r, uint64_t seed,
+                                 uint64_t stream_id, double* B, int64_t ldb) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(ell >= 0, 7);
+  LRA_ARG(ldb >= std::max<int64_t>(1, ell), 12);
+  return lra_sketch_subset_impl(c, trans, m, n, A, lda, ell, r, seed, stream_id, B, ldb);
+}
+
+extern "C" int lra_sketch_sprn(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                               int64_t lda, int64_t ell, const int64_t* r, const double* s,
+                               uint64_t seed, uint64_t stream_id, double* B, int64_t ldb) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(ell >= 0, 7);
+  LRA_ARG((r == nullptr) == (s == nullptr), 9);
+  LRA_ARG(ldb >= std::max<int64_t>(1, ell), 13);
+  return lra_sketch_sprn_impl(c, trans, m, n, A, lda, ell, r, s, seed, stream_id, B, ldb);
+}

@@ -1,0 +1,149 @@
+// trsm.hip -- T = R11 \ R12 (maxdet_t, src/pqr.jl:438-442; the dtrsm behind
+// ldiv!(UpperTriangular(R[:,1:k]), R[:,k+1:n])).
+//
+// One workgroup per slab of `nc` right-hand sides; the k x nc slab stays in LDS for the whole
+// back substitution, the diagonal block column R(0:i, i) is read from L2 once per row.
+#include "lra_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+__global__ __launch_bounds__(256) void id_T_kernel(i64 k, i64 nr, const double* __restrict__ R,
+                                                   i64 ldr, double* __restrict__ T, i64 ldt,
+                                                   int nc) {
+  extern __shared__ double slab[];  // k x nc, column-major with leading dimension k
+  const i64 c0 = (i64)blockIdx.x * nc;
+  const int ncols = (int)std::min<i64>(nc, nr - c0);
+  const int tid = threadIdx.x;
+  // thread -> (column c, row group g); nc divides 256
+  const int c = tid % nc, g = tid / nc, ng = 256 / nc;
+  const bool live = c < ncols;
+  double* x = slab + (i64)c * k;
+  if (live)
+    for (i64 r = g; r < k; r += ng) x[r] = R[r + (k + c0 + c) * ldr];
+  __syncthreads();
+  for (i64 i = k - 1; i >= 0; --i) {
+    // x(i) = b(i) / r(i,i); b(0:i-1) -= R(0:i-1, i) * x(i)
+    const double xi = live ? x[i] / R[i + i * ldr] : 0.0;
+    if (live)
+      for (i64 r = g; r < i; r += ng) x[r] -= R[r + i * ldr] * xi;
+    __syncthreads();
+    if (live && g == 0) x[i] = xi;
+    __syncthreads();
+  }
+  if (live)
+    for (i64 r = g; r < k; r += ng) T[r + (c0 + c) * ldt] = x[r];
+}
+
+}  // namespace
+
+int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt) {
+  const i64 nr = n - k;
+  if (k <= 0 || nr <= 0) return LRA_OK;
+  const size_t lds_cap = 64 * 1024;  // dynamic LDS budget per block
+  int nc = 32;  // power of two <= 32 (divides the 256 threads)
+  while (nc > 1 && (size_t)k * nc * sizeof(double) > lds_cap) nc >>= 1;
+  if ((size_t)k * nc * sizeof(double) > lds_cap) {
+    lra_set_error("id_T: k=%lld too large for the LDS slab", (long long)k);
+    return LRA_E_UNSUPPORTED;
+  }
+  const i64 blocks = (nr + nc - 1) / nc;
+  hipLaunchKernelGGL(id_T_kernel, dim3((unsigned)blocks), dim3(256),
+                     (size_t)k * nc * sizeof(double), st, k, nr, R, ldr, T, ldt, nc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    lra_set_error("id_T launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// In-place solve op(U) X = B for the k x k upper triangle U of R (ldr), B k x nrhs (ldb);
+// op = U (trans 0, back substitution) or U' (trans 1, forward substitution).  Same slab
+// scheme as id_T.
+namespace {
+__global__ __launch_bounds__(256) void trsm_upper_kernel(i64 k, i64 nrhs,
+                                                         const double* __restrict__ R, i64 ldr,
+                                                         double* __restrict__ B, i64 ldb,
+                                                         int trans, int nc) {
+  extern __shared__ double slab[];
+  const i64 c0 = (i64)blockIdx.x * nc;
+  const int ncols = (int)std::min<i64>(nc, nrhs - c0);
+  const int tid = threadIdx.x;
+  const int c = tid % nc, g = tid / nc, ng = 256 / nc;
+  const bool live = c < ncols;
+  double* x = slab + (i64)c * k;
+  if (live)
+    for (i64 r = g; r < k; r += ng) x[r] = B[r + (c0 + c) * ldb];
+  __syncthreads();
+  if (!trans) {
+    for (i64 i = k - 1; i >= 0; --i) {
+      const double xi = live ? x[i] / R[i + i * ldr] : 0.0;
+      if (live)
+        for (i64 r = g; r < i; r += ng) x[r] -= R[r + i * ldr] * xi;
+      __syncthreads();
+      if (live && g == 0) x[i] = xi;
+      __syncthreads();
+    }
+  } else {
+    // U' is lower triangular: (U')(r, i) = U(i, r)
+    for (i64 i = 0; i < k; ++i) {
+      const double xi = live ? x[i] / R[i + i * ldr] : 0.0;
+      if (live)
+        for (i64 r = i + 1 + g; r < k; r += ng) x[r] -= R[i + r * ldr] * xi;
+      __syncthreads();
+      if (live && g == 0) x[i] = xi;
+      __syncthreads();
+    }
+  }
+  if (live)
+    for (i64 r = g; r < k; r += ng) B[r + (c0 + c) * ldb] = x[r];
+}
+
+__global__ void axpy2d_kernel(i64 m, i64 n, double alpha, const double* X, i64 ldx, double* Y,
+                              i64 ldy) {
+  const i64 total = m * n;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % m, j = e / m;
+    Y[i + j * ldy] += alpha * X[i + j * ldx];
+  }
+}
+}  // namespace
+
+int lra_launch_trsm_upper(hipStream_t st, i64 k, i64 nrhs, const double* R, i64 ldr, double* B,
+                          i64 ldb, int trans) {
+  if (k <= 0 || nrhs <= 0) return LRA_OK;
+  const size_t lds_cap = 64 * 1024;
+  int nc = 32;
+  while (nc > 1 && (size_t)k * nc * sizeof(double) > lds_cap) nc >>= 1;
+  if ((size_t)k * nc * sizeof(double) > lds_cap) {
+    lra_set_error("trsm: k=%lld too large for the LDS slab", (long long)k);
+    return LRA_E_UNSUPPORTED;
+  }
+  const i64 blocks = (nrhs + nc - 1) / nc;
+  hipLaunchKernelGGL(trsm_upper_kernel, dim3((unsigned)blocks), dim3(256),
+                     (size_t)k * nc * sizeof(double), st, k, nrhs, R, ldr, B, ldb, trans, nc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    lra_set_error("trsm launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
+int lra_launch_axpy2d(hipStream_t st, i64 m, i64 n, double alpha, const double* X, i64 ldx,
+                      double* Y, i64 ldy) {
+  if (m <= 0 || n <= 0) return LRA_OK;
+  const i64 total = m * n;
+  const int blocks = (int)std::min<i64>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(axpy2d_kernel, dim3(blocks), dim3(256), 0, st, m, n, alpha, X, ldx, Y, ldy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    lra_set_error("axpy launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
