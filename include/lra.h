@@ -174,6 +174,12 @@ int lra_maxdet_swapcols_f64(lra_ctx* ctx, int64_t k, int64_t n, int64_t* p, doub
 int lra_snorm(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda, int isherm,
               const lra_opts* opts, const double* x0, double* result);
 
+/* snormdiff(A, B) = snorm(A - B) for two dense m x n matrices (src/snorm.jl:50-53; the
+ * difference is applied as A x - B x, src/linop.jl:155-187). */
+int lra_snormdiff(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                  const double* B, int64_t ldb, int isherm, const lra_opts* opts,
+                  const double* x0, double* result);
+
 /* snormdiff(A, ID) = snorm(A - A[:,sk]*[I T]P') (src/snorm.jl:50-53 with the ID as a
  * LinearOperator): the difference is applied matrix-free.  sk/rd/T as in lra_id_out. */
 int lra_snormdiff_id(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,

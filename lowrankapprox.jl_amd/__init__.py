@@ -355,3 +355,50 @@ class LinearOperator:
 
     def idfact(self, trans, opts, **kw):
         raise LraError("matrix-free idfact: not built yet")
+
+
+# ----------------------------------------------------------------------------------------
+# snorm / snormdiff (src/snorm.jl:14-53)
+
+def _ishermitian_dev(A) -> bool:
+    torch = _torch()
+    return A.shape[0] == A.shape[1] and bool(torch.equal(A, A.t()))
+
+
+def snorm(A, opts: LRAOptions = None, x0=None, **kw) -> float:
+    """Spectral norm by power iteration on A'A (A itself when Hermitian)."""
+    if isinstance(A, LinearOperator):
+        return A.snorm(opts, x0, **kw)
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    x = to_device(np.asarray(x0, dtype=np.float64)) if x0 is not None else None
+    res = C.c_double(0.0)
+    copts = opts.to_c()
+    check(_lib.load().lra_snorm(ctx.h, m, n, _ptr(A), _ld(A), int(_ishermitian_dev(A)),
+                                C.byref(copts), _ptr(x), C.byref(res)), "lra_snorm")
+    return float(res.value)
+
+
+def snormdiff(A, B, opts: LRAOptions = None, x0=None, **kw) -> float:
+    """snorm(A - B); B may be a dense matrix or an IDPackedV of A (applied matrix-free)."""
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    x = to_device(np.asarray(x0, dtype=np.float64)) if x0 is not None else None
+    res = C.c_double(0.0)
+    copts = opts.to_c()
+    L = _lib.load()
+    if isinstance(B, IDPackedV):
+        T = to_device(B.T) if B.T.numel() else empty_colmajor(max(B.k, 1), 1)
+        check(L.lra_snormdiff_id(ctx.h, m, n, _ptr(A), _ld(A), B.k, _ptr(B.sk), _ptr(B.rd),
+                                 _ptr(T), max(1, _ld(T)), C.byref(copts), _ptr(x),
+                                 C.byref(res)), "lra_snormdiff_id")
+    else:
+        Bd = to_device(B)
+        herm = int(_ishermitian_dev(A) and _ishermitian_dev(Bd))
+        check(L.lra_snormdiff(ctx.h, m, n, _ptr(A), _ld(A), _ptr(Bd), _ld(Bd), herm,
+                              C.byref(copts), _ptr(x), C.byref(res)), "lra_snormdiff")
+    return float(res.value)

@@ -83,6 +83,8 @@ SIGNATURES = [
                                           i64, C.POINTER(i64)]),
     ("lra_snorm", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.c_int, C.POINTER(lra_opts),
                             P_D, C.POINTER(C.c_double)]),
+    ("lra_snormdiff", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64, C.c_int,
+                                C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_snormdiff_id", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, P_I, P_I, P_D, i64,
                                    C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_idfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),

@@ -91,7 +91,7 @@ extern "C" int lra_ctx_destroy(lra_ctx* c) {
   hipStreamSynchronize(c->stream);
   for (int i = 0; i < lra_ctx::kSlots; ++i)
     if (c->ws[i]) hipFree(c->ws[i]);
-  if (c->coop) hipFree(c->coop);
+  if (c->coop) hipFree(c->coop);  // the uncached area (coop_uc) stays allocated
   if (c->d_err) hipFree(c->d_err);
   if (c->host_pinned) hipHostFree(c->host_pinned);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -149,7 +149,27 @@ int lra_ws(lra_ctx* c, int slot, size_t bytes, void** out) {
   return LRA_OK;
 }
 
-int lra_coop_area(lra_ctx* c, size_t bytes, void** out) {
+// Exchange area of the cooperative kernels.  First choice: a fixed 64 MiB of uncached device
+// memory (reads and writes bypass the per-XCD L2, so blocks exchange data without cache
+// write-back/invalidate); it is never freed before the process ends.  Larger requests fall
+// back to ordinary memory, with which the kernels use agent-scope fences.
+int lra_coop_area(lra_ctx* c, size_t bytes, void** out, bool* uncached) {
+  static const size_t kUncached = (size_t)64 << 20;
+  if (!c->coop_uc_tried) {
+    c->coop_uc_tried = true;
+    void* p = nullptr;
+    if (hipExtMallocWithFlags(&p, kUncached, hipDeviceMallocUncached) == hipSuccess) {
+      c->coop_uc = p;
+      c->coop_uc_size = kUncached;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  if (c->coop_uc && bytes <= c->coop_uc_size) {
+    *out = c->coop_uc;
+    *uncached = true;
+    return LRA_OK;
+  }
   if (c->coop_size < bytes) {
     if (c->coop) {
       LRA_HIP(hipStreamSynchronize(c->stream));
@@ -165,6 +185,7 @@ int lra_coop_area(lra_ctx* c, size_t bytes, void** out) {
     c->coop_size = want;
   }
   *out = c->coop;
+  *uncached = false;
   return LRA_OK;
 }
 

@@ -49,9 +49,13 @@ struct lra_ctx {
   size_t ws_size[kSlots] = {};
   // small pinned host staging area for scalars read back synchronously
   void* host_pinned = nullptr;
-  // per-context cooperative exchange area
-  void* coop = nullptr;
+  // per-context cooperative exchange area (uncached device memory when available: allocated
+  // once, never freed before the process ends)
+  void* coop = nullptr;       // ordinary memory, grown on demand
   size_t coop_size = 0;
+  void* coop_uc = nullptr;    // uncached, fixed size, never freed
+  size_t coop_uc_size = 0;
+  bool coop_uc_tried = false;
   int* d_err = nullptr;  // device error word (timeout etc.)
 };
 
@@ -67,7 +71,7 @@ template <class T>
 static inline int lra_wsT(lra_ctx* ctx, int slot, size_t count, T** out) {
   return lra_ws(ctx, slot, count * sizeof(T), (void**)out);
 }
-int lra_coop_area(lra_ctx* ctx, size_t bytes, void** out);
+int lra_coop_area(lra_ctx* ctx, size_t bytes, void** out, bool* uncached);
 int lra_check_device_error(lra_ctx* ctx);
 
 // ---------------------------------------------------------------------------------------
@@ -154,13 +158,15 @@ __device__ __forceinline__ double lra_wave_sum(double v) {
 __device__ __forceinline__ bool lra_grid_barrier(unsigned* counter, unsigned target,
                                                  int* err, int* abort_flag) {
   __shared__ int s_ok;
-  __threadfence();
+  // make this wave's global writes visible device-wide (L2 write-back), then arrive
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    // relaxed polling: no cache invalidation per poll
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         ok = 0;
         break;
@@ -173,10 +179,12 @@ __device__ __forceinline__ bool lra_grid_barrier(unsigned* counter, unsigned tar
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    // one acquire for the block: invalidates this CU's L1 and the XCD's L2 lines so the
+    // block's subsequent loads see the other blocks' data
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
   return s_ok != 0;
 }
 

@@ -24,14 +24,18 @@
 
 namespace {
 
-struct Slot {
-  double val;
-  int pos;
-  int owner;
-  int lc;
-  int flag;
-  int pad[2];
-};
+// Step exchange between blocks: a 16-byte slot {candidate norm, tag} per block and step parity,
+// tag = step+1 | position << 28 | local column << 52 | flag << 63, written with one 16-byte
+// store after the block's candidate column has been written (and acknowledged), read with one
+// 16-byte load, so a reader that sees the step's tag also sees the norm and the column.
+constexpr uint64_t TAG_MASK = (1ull << 28) - 1;
+constexpr uint64_t POS_NONE = (1ull << 24) - 1;
+constexpr uint64_t LC_NONE = (1ull << 11) - 1;
+LRA_DEV uint64_t pack_tag(int step1, int pos, int lc, int flag) {
+  const uint64_t pp = pos == INT_MAX ? POS_NONE : (uint64_t)pos;
+  const uint64_t ll = lc < 0 ? LC_NONE : (uint64_t)lc;
+  return ((uint64_t)step1 & TAG_MASK) | (pp << 28) | (ll << 52) | ((uint64_t)(flag != 0) << 63);
+}
 
 struct QrcpDev {
   double* A;
@@ -45,8 +49,8 @@ struct QrcpDev {
   i64* k_out;
   int* early_out;
   int G, cpw, tpc, cand_stride;
-  unsigned* bar;
-  Slot* slots;
+  int uncached;       // exchange area bypasses the caches (no write-back/invalidate needed)
+  uint64_t* slots;    // 2G slots of 16 bytes: {norm bits, tag}
   double* cand;
   int* err;
   int* abort_flag;
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
   __shared__ double s_wval[NW];
   __shared__ int s_wpos[NW], s_wlc[NW], s_wflag[NW];
   __shared__ double s_tau, s_beta, s_ptol, s_winval;
-  __shared__ int s_winowner, s_winlc, s_winpos, s_flagprev, s_blc;
+  __shared__ int s_winowner, s_winlc, s_winpos, s_flagprev, s_blc;  // s_blc: exchange ok
 
   // s_x holds the current reflector by ABSOLUTE row: zeros above row rk and in the padding,
   // so the dot/update loops below need no per-row predicates.
@@ -146,47 +150,100 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     const bool done_steps = (s == p.kmax);
 
     if (p.G > 1) {
-      // publish the block candidate and its column rows rk..m-1
       const int rk = p.offset + s;
-      if (!done_steps) {
-        double* dst = p.cand + ((size_t)(s & 1) * p.G + g) * p.cand_stride;
-        if (have && active && lc == clc) {
-          double* d = dst + (rbase - rk);
-          const int lo = rk - rbase, hi = m - rbase;
+      const int len = m - rk;
+      const size_t sl = (size_t)(s & 1) * p.G + g;
+      // stage the block candidate's rows rk..m-1 in s_x (absolute rows; the winner's data
+      // overwrites them below, rows < rk and >= m stay zero)
+      if (!done_steps && have && active && lc == clc) {
+        double* d = s_x + rbase;
+        const int lo = rk - rbase, hi = m - rbase;
 #pragma unroll
-          for (int i = 0; i < RPT; ++i)
-            if (i >= lo && i < hi) d[i] = a[i];
-        }
-      }
-      if (tid == 0) {
-        Slot sl;
-        sl.val = cv; sl.pos = cp; sl.owner = g; sl.lc = clc; sl.flag = cflag;
-        p.slots[(size_t)(s & 1) * p.G + g] = sl;
-      }
-      if (!lra_grid_barrier(p.bar, (unsigned)(s + 1) * (unsigned)p.G, p.err, p.abort_flag)) {
-        aborted = true;
-        break;
-      }
-      if (wave == 0) {
-        double v = -1.0;
-        int vp = INT_MAX, vo = -1, vl = -1, fl = 0;
-        for (int t = lane; t < p.G; t += 64) {
-          const Slot sl = p.slots[(size_t)(s & 1) * p.G + t];
-          fl |= sl.flag;
-          if (sl.val > v || (sl.val == v && sl.pos < vp)) { v = sl.val; vp = sl.pos; vo = sl.owner; vl = sl.lc; }
-        }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const double ov = __shfl_xor(v, d, 64);
-          const int op = __shfl_xor(vp, d, 64);
-          const int oo = __shfl_xor(vo, d, 64);
-          const int ol = __shfl_xor(vl, d, 64);
-          if (ov > v || (ov == v && op < vp)) { v = ov; vp = op; vo = oo; vl = ol; }
-          fl |= __shfl_xor(fl, d, 64);
-        }
-        if (lane == 0) { s_winval = v; s_winpos = vp; s_winowner = vo; s_winlc = vl; s_flagprev = fl; }
+        for (int i = 0; i < RPT; ++i)
+          if (i >= lo && i < hi) d[i] = a[i];
       }
       __syncthreads();
+      if (wave == 0) {
+        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+        volatile u64x2* slots = reinterpret_cast<volatile u64x2*>(p.slots);
+        // publish: column data, wait until it is acknowledged, then {norm, tag}
+        if (!done_steps) {
+          double* dst = p.cand + sl * p.cand_stride;
+          for (int t = lane; t < len; t += 64) dst[t] = s_x[rk + t];
+        }
+        if (p.uncached) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) {
+          u64x2 w;
+          w.x = (uint64_t)__double_as_longlong(cv);
+          w.y = pack_tag(s + 1, cp, clc, cflag);
+          slots[sl] = w;
+        }
+        // wait until every block has published this step (bounded)
+        const uint64_t want = (uint64_t)(s + 1) & TAG_MASK;
+        const size_t base = (size_t)(s & 1) * p.G;
+        unsigned spins = 0;
+        int ok = 1;
+        u64x2 mine[4];  // G <= 256 slots, up to 4 per lane
+        for (;;) {
+          bool ready = true;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            if (t < p.G) {
+              mine[u] = slots[base + t];
+              ready &= ((mine[u].y & TAG_MASK) == want);
+            }
+          }
+          if (__all(ready)) break;
+          if ((++spins & 63) == 0) {
+            if (__hip_atomic_load(p.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+            if (spins > LRA_SPIN_LIMIT) {
+              if (lane == 0) {
+                __hip_atomic_store(p.err, (int)LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              ok = 0;
+              break;
+            }
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double v = -1.0;
+        int vp = INT_MAX, vo = -1, vl = -1, fl = 0;
+        if (ok) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * u;
+            if (t >= p.G) continue;
+            const uint64_t tg = mine[u].y;
+            const double sv = __longlong_as_double((long long)mine[u].x);
+            const uint64_t pp = (tg >> 28) & POS_NONE;
+            const int spos = pp == POS_NONE ? INT_MAX : (int)pp;
+            const uint64_t ll = (tg >> 52) & LC_NONE;
+            fl |= (int)(tg >> 63);
+            if (sv > v || (sv == v && spos < vp)) {
+              v = sv; vp = spos; vo = t; vl = ll == LC_NONE ? -1 : (int)ll;
+            }
+          }
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const double ov = __shfl_xor(v, d, 64);
+            const int op = __shfl_xor(vp, d, 64);
+            const int oo = __shfl_xor(vo, d, 64);
+            const int ol = __shfl_xor(vl, d, 64);
+            if (ov > v || (ov == v && op < vp)) { v = ov; vp = op; vo = oo; vl = ol; }
+            fl |= __shfl_xor(fl, d, 64);
+          }
+        }
+        if (lane == 0) {
+          s_winval = v; s_winpos = vp; s_winowner = vo; s_winlc = vl; s_flagprev = fl;
+          s_blc = ok;
+        }
+      }
+      __syncthreads();
+      if (!s_blc) { aborted = true; break; }
     } else {
       if (tid == 0) { s_winval = cv; s_winpos = cp; s_winowner = 0; s_winlc = clc; s_flagprev = cflag; }
       __syncthreads();
@@ -228,7 +285,12 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     if (tid == 0 && rk > 0) s_x[rk - 1] = 0.0;  // previous step's v(1) = 1
     if (p.G > 1) {
       const double* src = p.cand + ((size_t)(s & 1) * p.G + win_owner) * p.cand_stride;
-      for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = src[t];
+      if (p.uncached) {
+        const volatile double* vsrc = src;
+        for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = vsrc[t];
+      } else {
+        for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = src[t];
+      }
     } else {
       if (have && active && lc == win_lc) {
         double* d = s_x + rbase;
@@ -472,6 +534,10 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
     if (q.early_out) LRA_HIP(hipMemsetAsync(q.early_out, 0, sizeof(int), ctx->stream));
     return LRA_OK;
   }
+  if (q.n >= (int)POS_NONE) {
+    lra_set_error("qrcp: n=%d exceeds the position field", q.n);
+    return LRA_E_UNSUPPORTED;
+  }
   if (q.nb > NB_MAX) {
     lra_set_error("qrcp: nb=%d exceeds %d", q.nb, NB_MAX);
     return LRA_E_UNSUPPORTED;
@@ -490,17 +556,23 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   d.G = sh.G; d.cpw = sh.cpw; d.tpc = sh.tpc;
   d.cand_stride = ((q.m + 7) / 8) * 8;
   d.err = ctx->d_err;
-  // cooperative area: [bar u32 x 16][abort int x 16][slots 2*G][cand 2*G*stride]
-  const size_t slot_bytes = sizeof(Slot) * 2 * (size_t)sh.G;
+  if (sh.G > 256) {
+    lra_set_error("qrcp: %d blocks exceed the 256-slot exchange", sh.G);
+    return LRA_E_UNSUPPORTED;
+  }
+  // exchange area: [header 256 B: abort flag][slots 2G x 16 B][cand 2G x stride doubles]
+  const size_t slot_bytes = 16 * 2 * (size_t)sh.G;
   const size_t cand_bytes = sizeof(double) * 2 * (size_t)sh.G * d.cand_stride;
   void* area = nullptr;
-  LRA_CHECK(lra_coop_area(ctx, 256 + slot_bytes + cand_bytes, &area));
+  bool uc = false;
+  LRA_CHECK(lra_coop_area(ctx, 256 + slot_bytes + cand_bytes, &area, &uc));
   char* base = (char*)area;
-  d.bar = (unsigned*)base;
-  d.abort_flag = (int*)(base + 64);
-  d.slots = (Slot*)(base + 256);
+  d.uncached = uc ? 1 : 0;
+  d.abort_flag = (int*)base;
+  d.slots = (uint64_t*)(base + 256);
   d.cand = (double*)(base + 256 + slot_bytes);
-  LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, 64));
+  // zero the header and every slot (a stale tag of an earlier launch must never match)
+  LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + slot_bytes) / 4));
   hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream, sh.G > 1);
   if (e != hipSuccess) {
     lra_set_error("qrcp launch (m=%d n=%d tpc=%d rpt=%d G=%d): %s", q.m, q.n, sh.tpc, sh.rpt,
