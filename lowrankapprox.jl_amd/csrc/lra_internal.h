@@ -56,6 +56,7 @@ struct lra_ctx {
   void* coop_uc = nullptr;    // uncached, fixed size, never freed
   size_t coop_uc_size = 0;
   bool coop_uc_tried = false;
+  uint32_t qrcp_gen = 0;      // launch generation of the cooperative QRCP (chunk tags)
   int* d_err = nullptr;  // device error word (timeout etc.)
 };
 

@@ -50,6 +50,7 @@ struct QrcpDev {
   int* early_out;
   int G, cpw, tpc, cand_stride;
   int uncached;       // exchange area bypasses the caches (no write-back/invalidate needed)
+  uint32_t gen;       // launch generation, part of every chunk tag
   uint64_t* slots;    // 2G slots of 16 bytes: {norm bits, tag}
   double* cand;
   int* err;
@@ -65,6 +66,8 @@ LRA_DEV double lapy2(double x, double y) {
   const double q = z / w;
   return w * sqrt(1.0 + q * q);
 }
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 template <int RPT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
@@ -164,15 +167,20 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
       }
       __syncthreads();
       if (wave == 0) {
-        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
         volatile u64x2* slots = reinterpret_cast<volatile u64x2*>(p.slots);
-        // publish: column data, wait until it is acknowledged, then {norm, tag}
+        // publish: the column as 16-byte {value, step tag} chunks (each chunk validates
+        // itself, so no acknowledgement wait is needed), then the {norm, tag} slot
+        const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
         if (!done_steps) {
-          double* dst = p.cand + sl * p.cand_stride;
-          for (int t = lane; t < len; t += 64) dst[t] = s_x[rk + t];
+          volatile u64x2* dst = reinterpret_cast<volatile u64x2*>(p.cand) + sl * p.cand_stride;
+          for (int t = lane; t < len; t += 64) {
+            u64x2 w;
+            w.x = (uint64_t)__double_as_longlong(s_x[rk + t]);
+            w.y = ctag;
+            dst[t] = w;
+          }
         }
-        if (p.uncached) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) {
           u64x2 w;
           w.x = (uint64_t)__double_as_longlong(cv);
@@ -284,12 +292,21 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     const int win_owner = s_winowner, win_lc = s_winlc, win_pos = s_winpos;
     if (tid == 0 && rk > 0) s_x[rk - 1] = 0.0;  // previous step's v(1) = 1
     if (p.G > 1) {
-      const double* src = p.cand + ((size_t)(s & 1) * p.G + win_owner) * p.cand_stride;
-      if (p.uncached) {
-        const volatile double* vsrc = src;
-        for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = vsrc[t];
-      } else {
-        for (int t = tid; t < len; t += BLOCK) s_x[rk + t] = src[t];
+      const volatile u64x2* src = reinterpret_cast<const volatile u64x2*>(p.cand) +
+                                  ((size_t)(s & 1) * p.G + win_owner) * p.cand_stride;
+      const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
+      for (int t = tid; t < len; t += BLOCK) {
+        u64x2 w = src[t];
+        unsigned spins = 0;
+        while (w.y != ctag && ++spins < LRA_SPIN_LIMIT) {  // chunk not landed yet
+          __builtin_amdgcn_s_sleep(1);
+          w = src[t];
+        }
+        if (w.y != ctag) {
+          __hip_atomic_store(p.err, (int)LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_x[rk + t] = __longlong_as_double((long long)w.x);
       }
     } else {
       if (have && active && lc == win_lc) {
@@ -362,10 +379,15 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
       if (have && pos == s) pos = win_pos;  // LAPACK swaps columns k and pvt
       if (active) {
         const double* vx = s_x + rbase;
-        double w = 0.0;
+        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) w += vx[i] * a[i];
-        w = lra_group_sum(w, tpc);
+        for (int i = 0; i < RPT; i += 4) {
+          w0 += vx[i] * a[i];
+          if (i + 1 < RPT) w1 += vx[i + 1] * a[i + 1];
+          if (i + 2 < RPT) w2 += vx[i + 2] * a[i + 2];
+          if (i + 3 < RPT) w3 += vx[i + 3] * a[i + 3];
+        }
+        double w = lra_group_sum((w0 + w1) + (w2 + w3), tpc);
         const double tw = tau * w;
         // re-read v from LDS below instead of keeping RPT values live across the two loops
         asm volatile("" ::: "memory");
@@ -562,7 +584,7 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   }
   // exchange area: [header 256 B: abort flag][slots 2G x 16 B][cand 2G x stride doubles]
   const size_t slot_bytes = 16 * 2 * (size_t)sh.G;
-  const size_t cand_bytes = sizeof(double) * 2 * (size_t)sh.G * d.cand_stride;
+  const size_t cand_bytes = 16 * 2 * (size_t)sh.G * d.cand_stride;  // {value, tag} chunks
   void* area = nullptr;
   bool uc = false;
   LRA_CHECK(lra_coop_area(ctx, 256 + slot_bytes + cand_bytes, &area, &uc));
@@ -571,8 +593,10 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   d.abort_flag = (int*)base;
   d.slots = (uint64_t*)(base + 256);
   d.cand = (double*)(base + 256 + slot_bytes);
-  // zero the header and every slot (a stale tag of an earlier launch must never match)
+  // zero the header and every slot (a stale slot tag of an earlier launch must never match);
+  // chunk tags carry the launch generation instead
   LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + slot_bytes) / 4));
+  d.gen = ++ctx->qrcp_gen;
   hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream, sh.G > 1);
   if (e != hipSuccess) {
     lra_set_error("qrcp launch (m=%d n=%d tpc=%d rpt=%d G=%d): %s", q.m, q.n, sh.tpc, sh.rpt,
