@@ -186,6 +186,12 @@ int lra_snormdiff_id(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_
                      int64_t k, const int64_t* sk, const int64_t* rd, const double* T,
                      int64_t ldt, const lra_opts* opts, const double* x0, double* result);
 
+/* snorm of a matrix-free operator (snorm(L::LinearOperator), src/snorm.jl:14-45 with
+ * src/linop.jl:105-113): `apply` is called with nrhs = 1 for L x (trans 0) and L' y
+ * (trans 1); isherm != 0 iterates on L itself. */
+int lra_snorm_op(lra_ctx* ctx, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                 int isherm, const lra_opts* opts, const double* x0, double* result);
+
 /* ---- whole-pipeline entry points (mirror the reference API) -------------------------- */
 
 /* idfact(trans, A, opts) (src/id.jl:434-447): sketchfact(:left, trans, A) (adaptive loop,
@@ -199,6 +205,23 @@ int lra_idfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, i
  * src/linop.jl:105-113): the sketch is one application of L' to ell vectors. */
 int lra_idfact_op(lra_ctx* ctx, int trans, int64_t m, int64_t n, lra_apply_fn apply,
                   void* user, const lra_opts* opts, lra_id_out* out);
+
+/* sketchfact(:left, trans, A, opts) with pqrfact_retval = "" (src/sketch.jl:52-66 and the
+ * adaptive drivers :223-240, :313-330, :545-562, :674-690; sketch = none is
+ * pqrfact_backend! on op(A), src/pqr.jl:323-327): column pivots of op(A) into p (device,
+ * N = columns of op(A) entries, 1-based) and the rank into *k (host).  Round r of the
+ * adaptive loop draws its random sketch from Philox stream stream_base + r. */
+int lra_sketchfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A,
+                   int64_t lda, const lra_opts* opts, uint64_t stream_base, int64_t* p,
+                   int64_t* k);
+
+/* curfact(A, opts) (src/cur.jl:532-564; sketch = none: curfact_none, :574-606): row and
+ * column skeletons.  rows and cols (device, min(m,n) entries each) receive k 1-based
+ * indices, *k (host).  An exactly symmetric A (ishermitian for real A) takes one column
+ * sketchfact and rows = cols; otherwise two chained sketchfacts, the second drawing from
+ * Philox streams 2^32 + round. */
+int lra_curfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                const lra_opts* opts, int64_t* rows, int64_t* cols, int64_t* k);
 
 /* Batched ID of many independent blocks (MI355X extension; each block behaves as idfact
  * with a fresh Gaussian sketch of stream `b` under opts->seed): one persistent fused launch

@@ -22,8 +22,9 @@ import numpy as np
 from . import _lib
 from ._lib import LraError, check
 
-__all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "geqp3_adap", "laqps",
-           "id_T", "gaussian", "to_device", "LraError", "context"]
+__all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "geqp3_adap",
+           "laqps", "id_T", "maxdet_swapcols", "gaussian", "to_device", "LraError", "context",
+           "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -128,13 +129,19 @@ def to_device(A, dtype=None):
         A = A.to(torch.float64)
     if not A.is_cuda:
         A = A.cuda()
-    if not (A.stride(0) == 1 and A.stride(1) >= max(1, A.shape[0])):
+    colmajor = ((A.shape[0] <= 1 or A.stride(0) == 1) and
+                (A.shape[1] <= 1 or A.stride(1) >= max(1, A.shape[0])))
+    if not colmajor:
         A = A.t().contiguous().t()
     return A
 
 
 def _ld(A) -> int:
-    return max(1, A.stride(1)) if A.dim() == 2 else max(1, A.shape[0])
+    """Leading dimension of a column-major tensor (a single column's stride(1) is arbitrary in
+    torch, so it is not used)."""
+    if A.dim() == 2 and A.shape[1] > 1:
+        return max(1, A.stride(1))
+    return max(1, A.shape[0])
 
 
 def empty_colmajor(m: int, n: int, dtype=None):
@@ -246,6 +253,19 @@ def id_T(R, k: int):
     return T[:k, :]
 
 
+def maxdet_swapcols(p, T, k: int, tol: float = 0.0, niter: int = -1) -> int:
+    """Determinant-maximising swaps on (p, T) in place (maxdet_swapcols!, src/pqr.jl:445-501).
+    p: int64 CUDA vector (n, 1-based), T: column-major k x (n-k) CUDA tensor.  Returns the
+    iteration count."""
+    ctx = context()
+    n = p.shape[0]
+    it = _lib.i64(0)
+    ldt = _ld(T) if T.numel() else max(1, k)
+    check(_lib.load().lra_maxdet_swapcols_f64(ctx.h, k, n, _ptr(p), _ptr(T), ldt, tol, niter,
+                                              C.byref(it)), "lra_maxdet_swapcols_f64")
+    return int(it.value)
+
+
 # ----------------------------------------------------------------------------------------
 # IDPackedV / idfact
 
@@ -318,6 +338,7 @@ def _copy_device(dst: int, src: int, rows: int, cols: int, lds: int, ldd: int) -
                               ctypes.c_size_t(8 * lds), ctypes.c_size_t(8 * rows),
                               ctypes.c_size_t(cols), ctypes.c_int(3), ctypes.c_void_p(st))
     if rc != 0:
+        hip.hipGetLastError()  # do not leave the error for the next launch check
         raise LraError(f"hipMemcpy2DAsync failed ({rc})")
 
 
@@ -340,8 +361,9 @@ def id(trans, A=None, opts: LRAOptions = None, **kw):
 
 
 class LinearOperator:
-    """Matrix-free operator (src/linop.jl:9-40): `mul(X) -> L*X`, `mulc(X) -> L'*X` on
-    column-major CUDA tensors."""
+    """Matrix-free operator (LinearOperator, src/linop.jl:9-40): `mul(X) -> L*X` and
+    `mulc(X) -> L'*X` on float64 CUDA tensors (X is n x nrhs, resp. m x nrhs).  The library
+    calls them through the C callback `lra_apply_fn` on torch's current stream."""
 
     def __init__(self, m: int, n: int, mul, mulc=None, hermitian: bool = False):
         self.m, self.n = m, n
@@ -349,12 +371,134 @@ class LinearOperator:
         self.mulc = mulc if mulc is not None else mul
         self.hermitian = hermitian
 
+    @classmethod
+    def from_matrix(cls, A, hermitian: bool = None):
+        """LinOp(A) (src/linop.jl:42-60) of a dense matrix."""
+        A = to_device(A)
+        if hermitian is None:
+            hermitian = _ishermitian_dev(A)
+        return cls(A.shape[0], A.shape[1], lambda X: A @ X, lambda X: A.t() @ X, hermitian)
+
     @property
     def shape(self):
         return (self.m, self.n)
 
-    def idfact(self, trans, opts, **kw):
-        raise LraError("matrix-free idfact: not built yet")
+    def _callback(self):
+        """lra_apply_fn: Y (ldy) = op(L) X (ldx); raw device pointers are copied into and out
+        of torch tensors (a failing operator returns a non-zero status, never raises)."""
+        def cb(user, trans, nrhs, X, ldx, Y, ldy, stream):
+            try:
+                rin = self.n if trans == 0 else self.m
+                rout = self.m if trans == 0 else self.n
+                Xt = empty_colmajor(rin, nrhs)
+                if rin > 0 and nrhs > 0:
+                    _copy_device(Xt.data_ptr(), X, rin, nrhs, ldx, rin)
+                Yt = self.mul(Xt) if trans == 0 else self.mulc(Xt)
+                Yt = to_device(Yt)
+                if tuple(Yt.shape) != (rout, nrhs):
+                    return 2
+                if rout > 0 and nrhs > 0:
+                    _copy_device(Y, Yt.data_ptr(), rout, nrhs, _ld(Yt), ldy)
+                return 0
+            except Exception as e:  # noqa: BLE001 -- reported to the library as a status
+                self._error = e
+                return 1
+        self._error = None
+        return _lib.APPLY_FN(cb)
+
+    def _check(self, status: int, what: str) -> None:
+        if status != 0 and getattr(self, "_error", None) is not None:
+            raise LraError(f"{what}: the operator raised {self._error!r}") from self._error
+        check(status, what)
+
+    def idfact(self, trans="n", opts: "LRAOptions" = None, **kw) -> "IDPackedV":
+        """idfact on the operator (src/id.jl:434-447 through src/linop.jl:105-113)."""
+        torch = _torch()
+        opts = (opts or LRAOptions()).copy(**kw)
+        ctx = context()
+        N = self.n if trans == "n" else self.m
+        sk = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+        rd = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+        out = _lib.lra_id_out()
+        out.sk, out.rd = _ptr(sk), _ptr(rd)
+        out.T, out.ldt = None, 0
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_idfact_op(ctx.h, _lib.TRANS[trans], self.m, self.n, cb,
+                                              None, C.byref(copts), C.byref(out)),
+                    "lra_idfact_op")
+        k = int(out.k)
+        return IDPackedV(sk[:k], rd[: N - k], _borrow_T(out, k, N - k), int(out.rounds))
+
+    def snorm(self, opts: "LRAOptions" = None, x0=None, **kw) -> float:
+        """snorm on the operator (src/snorm.jl:14-45)."""
+        opts = (opts or LRAOptions()).copy(**kw)
+        ctx = context()
+        x = to_device(np.asarray(x0, dtype=np.float64)) if x0 is not None else None
+        res = C.c_double(0.0)
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_snorm_op(ctx.h, self.m, self.n, cb, None,
+                                             int(self.hermitian), C.byref(copts), _ptr(x),
+                                             C.byref(res)), "lra_snorm_op")
+        return float(res.value)
+
+
+def sketchfact(trans, A=None, opts: "LRAOptions" = None, stream_base: int = 0, **kw):
+    """sketchfact(:left, trans, A, opts) with pqrfact_retval = "" (src/sketch.jl:52-66):
+    (column pivots of op(A), 1-based, device; rank k)."""
+    torch = _torch()
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    N = n if trans == "n" else m
+    p = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+    k = _lib.i64(0)
+    copts = opts.to_c()
+    check(_lib.load().lra_sketchfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
+                                     C.byref(copts), stream_base, _ptr(p), C.byref(k)),
+          "lra_sketchfact")
+    return p[:N], int(k.value)
+
+
+@dataclasses.dataclass
+class CURPackedU:
+    """CURPackedU (src/cur.jl:16-24): row and column skeletons (1-based, device)."""
+    rows: object
+    cols: object
+
+    def __getitem__(self, key):
+        if key == "rows": return self.rows
+        if key == "cols": return self.cols
+        if key == "k": return int(self.rows.shape[0])
+        raise KeyError(key)
+
+
+def curfact(A, opts: "LRAOptions" = None, **kw) -> CURPackedU:
+    """curfact(A, opts) (src/cur.jl:532-564)."""
+    torch = _torch()
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    mn = max(min(m, n), 1)
+    rows = torch.empty(mn, dtype=torch.int64, device="cuda")
+    cols = torch.empty(mn, dtype=torch.int64, device="cuda")
+    k = _lib.i64(0)
+    copts = opts.to_c()
+    check(_lib.load().lra_curfact(ctx.h, m, n, _ptr(A), _ld(A), C.byref(copts), _ptr(rows),
+                                  _ptr(cols), C.byref(k)), "lra_curfact")
+    k = int(k.value)
+    return CURPackedU(rows[:k], cols[:k])
+
+
+def cur(A, opts: "LRAOptions" = None, **kw):
+    """cur (src/cur.jl:566-569): (rows, cols)."""
+    U = curfact(A, opts, **kw)
+    return U.rows, U.cols
 
 
 # ----------------------------------------------------------------------------------------

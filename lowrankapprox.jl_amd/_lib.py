@@ -87,10 +87,16 @@ SIGNATURES = [
                                 C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_snormdiff_id", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, P_I, P_I, P_D, i64,
                                    C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
+    ("lra_snorm_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
+                               C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_idfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                              C.POINTER(lra_id_out)]),
     ("lra_idfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p,
                                 C.POINTER(lra_opts), C.POINTER(lra_id_out)]),
+    ("lra_sketchfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                                 C.c_uint64, P_I, C.POINTER(i64)]),
+    ("lra_curfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_I, P_I,
+                              C.POINTER(i64)]),
     ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
                                      C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
                                      P_I, P_I, C.POINTER(i64)]),

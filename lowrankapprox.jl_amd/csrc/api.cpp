@@ -12,6 +12,8 @@
 
 #include "lra_internal.h"
 
+#include <functional>
+
 static thread_local char g_err[1024] = "";
 
 void lra_set_error(const char* fmt, ...) {
@@ -389,9 +391,11 @@ static double prec_floor(int prec) {
 //   - rtol <  floor: a round that certifies rank >= target above the floor doubles as usual;
 //     the first round that cannot is recomputed in FP64 with the same Omega, and every later
 //     round stays FP64, so the accepted factorization is exactly the FP64 drop-in's.
-static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
-                           const lra_opts* o, SketchFactOut* out) {
-  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(A)
+// `make(ell, stream, prec, B)` writes the ell x N sketch of round `stream` into B (ld = ell).
+using SketchFn = std::function<int(i64, uint64_t, int, double*)>;
+
+static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opts* o,
+                          uint64_t stream_base, SketchFactOut* out) {
   const bool gauss = o->sketch == LRA_SKETCH_RANDN;
   int prec = gauss ? o->precision : LRA_PREC_F64;
   const bool adap = o->sketchfact_adap || o->rank < 0;
@@ -404,8 +408,7 @@ static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A,
     if (ell <= 0) { lra_set_error("sketch order %lld", (long long)ell); return LRA_E_UNSUPPORTED; }
     double* B = nullptr;
     LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)ell * std::max<i64>(N, 1), &B));
-    LRA_CHECK(lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, (uint64_t)rnd,
-                             prec, B, ell));
+    LRA_CHECK(make(ell, stream_base + (uint64_t)rnd, prec, B));
     const double floor = prec_floor(prec);
     const bool lowp = prec != LRA_PREC_F64;
     double* Bsave = nullptr;
@@ -438,12 +441,21 @@ static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A,
     if (accept) {
       out->B = B; out->Bsave = Bsave; out->ell = ell; out->N = N; out->jpvt = jpvt;
       out->tau = tau; out->k = hk; out->rounds = rnd + 1; out->prec = prec;
-      out->stream_id = rnd;
+      out->stream_id = stream_base + rnd;
       return LRA_OK;
     }
     target *= 2;
     ++rnd;
   }
+}
+
+static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                           const lra_opts* o, SketchFactOut* out, uint64_t stream_base = 0) {
+  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(A)
+  SketchFn make = [=](i64 ell, uint64_t stream, int prec, double* B) {
+    return lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, stream, prec, B, ell);
+  };
+  return sketchfact_run(c, N, make, o, stream_base, out);
 }
 
 // Low-precision sketch selected the skeleton: recompute T in FP64 from A's skeleton columns
@@ -495,6 +507,10 @@ static int mixed_refine_T(lra_ctx* c, int trans, i64 m, i64 n, const double* A, 
 // ---------------------------------------------------------------------------------------
 // idfact (src/id.jl:434-447)
 
+using RefineFn = std::function<int(const i64*, const i64*, double*, i64)>;
+static int finish_id(lra_ctx* c, const SketchFactOut& f, i64 N, const lra_opts* op,
+                     lra_id_out* out, const RefineFn& refine);
+
 extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
                           int64_t lda, const lra_opts* opts, lra_id_out* out) {
   LRA_ARG(c != nullptr, 1);
@@ -529,6 +545,18 @@ extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const dou
   } else {
     LRA_CHECK(sketchfact_left(c, trans, m, n, A, lda, &o, &f));
   }
+  return finish_id(c, f, N, &o, out, [&](const i64* sk, const i64* rd, double* T, i64 ldt) {
+    return mixed_refine_T(c, trans, m, n, A, lda, f, sk, rd, T, ldt, &o);
+  });
+}
+
+// The tail of idfact shared by the dense and matrix-free drivers: sk/rd from the pivots,
+// T = R11 \ R12 (maxdet_t), the mixed-precision refinement when the accepted round was
+// low precision, and the determinant-maximising swaps when maxdet_tol >= 0
+// (pqrback_postproc, src/pqr.jl:420-436).
+static int finish_id(lra_ctx* c, const SketchFactOut& f, i64 N, const lra_opts* op,
+                     lra_id_out* out, const RefineFn& refine) {
+  const lra_opts& o = *op;
   const i64 k = f.k;
   LRA_CHECK(lra_launch_split_ids(c->stream, N, k, f.jpvt, out->sk, out->rd));
   if (!out->T) {
@@ -541,8 +569,7 @@ extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const dou
   if (k > 0 && N > k) {
     LRA_ARG(out->ldt >= k, 8);
     LRA_CHECK(lra_launch_id_T(c->stream, k, N, f.B, f.ell, out->T, out->ldt));
-    if (f.prec != LRA_PREC_F64 && f.Bsave)
-      LRA_CHECK(mixed_refine_T(c, trans, m, n, A, lda, f, out->sk, out->rd, out->T, out->ldt, &o));
+    if (f.prec != LRA_PREC_F64 && f.Bsave) LRA_CHECK(refine(out->sk, out->rd, out->T, out->ldt));
     if (o.maxdet_tol >= 0) {
       // strong-RRQR swaps (pqrback_postproc with maxdet, src/pqr.jl:426-433)
       i64* p = nullptr;
@@ -557,6 +584,174 @@ extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const dou
   }
   out->k = k;
   out->rounds = f.rounds;
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// matrix-free idfact (idfact(L::LinearOperator), src/id.jl:434-447 with the LinOp products
+// of src/linop.jl:105-113): the Gaussian sketch Omega*op(L) is formed as (op(L)' Omega')',
+// one application of the caller's operator to ell vectors per adaptive round.
+
+extern "C" int lra_idfact_op(lra_ctx* c, int trans, int64_t m, int64_t n, lra_apply_fn apply,
+                             void* user, const lra_opts* opts, lra_id_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(apply != nullptr, 5);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(out != nullptr, 8);
+  if (chkopts(opts) != LRA_OK) return -7;
+  LRA_HIP(hipSetDevice(c->device));
+  lra_opts o = *opts;
+  o.pqrfact_retval = LRA_RET_T;
+  if (o.sketch != LRA_SKETCH_RANDN) {
+    lra_set_error("matrix-free idfact takes the Gaussian sketch (sketch=%d needs matrix entries)",
+                  o.sketch);
+    return LRA_E_UNSUPPORTED;
+  }
+  o.precision = LRA_PREC_F64;  // the operator is applied in FP64; no matrix for a refinement
+  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(L)
+  const i64 P = trans == LRA_TRANS_N ? m : n;  // rows of op(L)
+  const int tr = trans == LRA_TRANS_N ? 1 : 0; // op(L)' = L' for :n, L for :c
+  SketchFn make = [&](i64 ell, uint64_t stream, int, double* B) -> int {
+    double *Om = nullptr, *X = nullptr, *Y = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)std::max<i64>(ell * P, 1), &Om));
+    LRA_CHECK(lra_wsT(c, WS_OP0, (size_t)std::max<i64>(P * ell, 1), &X));
+    LRA_CHECK(lra_wsT(c, WS_OP1, (size_t)std::max<i64>(N * ell, 1), &Y));
+    LRA_CHECK(lra_launch_gaussian_fill(c->stream, o.seed, stream, ell, P, Om, ell));
+    LRA_CHECK(lra_launch_transpose(c->stream, ell, P, Om, ell, X, std::max<i64>(P, 1)));
+    const int st = apply(user, tr, ell, X, std::max<i64>(P, 1), Y, std::max<i64>(N, 1),
+                         (void*)c->stream);
+    if (st != 0) {
+      lra_set_error("operator apply returned %d", st);
+      return LRA_E_STATE;
+    }
+    return lra_launch_transpose(c->stream, N, ell, Y, std::max<i64>(N, 1), B, ell);
+  };
+  SketchFactOut f{};
+  LRA_CHECK(sketchfact_run(c, N, make, &o, 0, &f));
+  return finish_id(c, f, N, &o, out, [](const i64*, const i64*, double*, i64) { return 0; });
+}
+
+// ---------------------------------------------------------------------------------------
+// sketchfact (src/sketch.jl:52-66) with pqrfact_retval = "": column pivots and rank of
+// op(A); sketch = none is pqrfact_backend! on op(A) itself (src/pqr.jl:323-327,343-346).
+
+static int pivots_of(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                     const lra_opts* o, uint64_t stream_base, SketchFactOut* f) {
+  const i64 N = trans == LRA_TRANS_N ? n : m;
+  if (o->sketch != LRA_SKETCH_NONE) return sketchfact_left(c, trans, m, n, A, lda, o, f, stream_base);
+  const i64 P = trans == LRA_TRANS_N ? m : n;
+  double* B = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)std::max<i64>(P * N, 1), &B));
+  if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, A, lda, B, std::max<i64>(m, 1)));
+  else LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, B, std::max<i64>(n, 1)));
+  i64* jpvt = nullptr;
+  double* tau = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_JPVT, (size_t)std::max<i64>(N, 1), &jpvt));
+  LRA_CHECK(lra_wsT(c, WS_TAU, (size_t)std::max<i64>(std::min(P, N), 1), &tau));
+  i64 k = 0;
+  LRA_CHECK(lra_geqp3_adap_f64(c, P, N, B, std::max<i64>(P, 1), o->rank, o->atol, o->rtol, o->nb,
+                               jpvt, tau, &k));
+  *f = SketchFactOut{};
+  f->B = B; f->ell = P; f->N = N; f->jpvt = jpvt; f->tau = tau; f->k = k; f->rounds = 1;
+  f->prec = LRA_PREC_F64;
+  return LRA_OK;
+}
+
+extern "C" int lra_sketchfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                              int64_t lda, const lra_opts* opts, uint64_t stream_base,
+                              int64_t* p, int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(A != nullptr || m * n == 0, 5);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(p != nullptr, 9);
+  LRA_ARG(k != nullptr, 10);
+  if (chkopts(opts) != LRA_OK) return -7;
+  LRA_HIP(hipSetDevice(c->device));
+  lra_opts o = *opts;
+  o.precision = LRA_PREC_F64;
+  SketchFactOut f{};
+  LRA_CHECK(pivots_of(c, trans, m, n, A, lda, &o, stream_base, &f));
+  const i64 N = trans == LRA_TRANS_N ? n : m;
+  if (N > 0)
+    LRA_HIP(hipMemcpyAsync(p, f.jpvt, sizeof(i64) * N, hipMemcpyDeviceToDevice, c->stream));
+  *k = f.k;
+  return LRA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// curfact (src/cur.jl:532-564): row and column skeletons from two chained sketchfacts.
+
+extern "C" int lra_curfact(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t lda,
+                           const lra_opts* opts, int64_t* rows, int64_t* cols, int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(A != nullptr || m * n == 0, 4);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 5);
+  LRA_ARG(opts != nullptr, 6);
+  LRA_ARG(rows != nullptr || std::min(m, n) == 0, 7);
+  LRA_ARG(cols != nullptr || std::min(m, n) == 0, 8);
+  LRA_ARG(k != nullptr, 9);
+  if (chkopts(opts) != LRA_OK) return -6;
+  LRA_HIP(hipSetDevice(c->device));
+  lra_opts o = *opts;
+  o.pqrfact_retval = 0;
+  o.precision = LRA_PREC_F64;
+  *k = 0;
+  if (m == 0 || n == 0) return LRA_OK;
+  // ishermitian(A) (exactly symmetric for real A): one column sketchfact, rows = cols
+  bool herm = false;
+  if (m == n) {
+    int* flag = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_CUR0, 64, &flag));
+    LRA_CHECK(lra_launch_is_symmetric(c->stream, n, A, lda, flag));
+    int h = 0;
+    LRA_CHECK(read_scalars(c, flag, &h, 1));
+    herm = h != 0;
+  }
+  SketchFactOut f{};
+  if (herm) {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_N, m, n, A, lda, &o, 0, &f));
+    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
+    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
+    *k = f.k;
+    return LRA_OK;
+  }
+  // the second sketchfact draws from Philox streams 2^32 + round
+  const uint64_t base2 = 1ull << 32;
+  i64 kr = 0, kc = 0;
+  double* X = nullptr;
+  if (m >= n) {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, m, n, A, lda, &o, 0, &f));
+    kr = f.k;
+    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * kr, hipMemcpyDeviceToDevice, c->stream));
+    // A[rows, :] as the transpose of X = A[rows, :]' (n x kr), factored with trans = c
+    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(n * kr, 1) + 8, &X));
+    if (kr > 0) LRA_CHECK(lra_launch_gather_rows_t(c->stream, n, kr, A, lda, rows, X, n));
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, n, kr, X, std::max<i64>(n, 1), &o, base2, &f));
+    kc = f.k;
+    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * std::min(kr, kc), hipMemcpyDeviceToDevice,
+                           c->stream));
+  } else {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_N, m, n, A, lda, &o, 0, &f));
+    kc = f.k;
+    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * kc, hipMemcpyDeviceToDevice, c->stream));
+    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(m * kc, 1) + 8, &X));
+    if (kc > 0) LRA_CHECK(lra_launch_gather_cols(c->stream, m, kc, A, lda, cols, X, m));
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, m, kc, X, std::max<i64>(m, 1), &o, base2, &f));
+    kr = f.k;
+    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * std::min(kr, kc), hipMemcpyDeviceToDevice,
+                           c->stream));
+  }
+  *k = std::min(kr, kc);
+  LRA_HIP(hipStreamSynchronize(c->stream));
   return LRA_OK;
 }
 

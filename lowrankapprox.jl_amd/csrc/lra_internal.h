@@ -44,7 +44,7 @@ struct lra_ctx {
   int num_cu = 0;
   int lds_bytes = 0;
   // workspace slots: grown on demand, never shrunk until destroy
-  static const int kSlots = 24;
+  static const int kSlots = 32;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
   // small pinned host staging area for scalars read back synchronously
@@ -64,8 +64,10 @@ struct lra_ctx {
 enum WsSlot {
   WS_OMEGA = 0, WS_SKETCH, WS_SKETCH2, WS_SPLITK, WS_JPVT, WS_TAU, WS_SCAL, WS_R,
   WS_T, WS_TMP0, WS_TMP1, WS_TMP2, WS_TMP3, WS_LP0, WS_LP1, WS_BATCH0, WS_BATCH1,
-  WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0
+  WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
+  WS_COUNT
 };
+static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
 
 int lra_ws(lra_ctx* ctx, int slot, size_t bytes, void** out);
 template <class T>
@@ -255,3 +257,5 @@ int lra_launch_transpose(hipStream_t st, i64 m, i64 n, const double* A, i64 lda,
 int lra_launch_copy2d(hipStream_t st, i64 m, i64 n, const double* A, i64 lda, double* out,
                       i64 ldo);
 int lra_launch_split_ids(hipStream_t st, i64 n, i64 k, const i64* jpvt, i64* sk, i64* rd);
+// *flag = 1 when the n x n matrix A is exactly symmetric, else 0 (device int)
+int lra_launch_is_symmetric(hipStream_t st, i64 n, const double* A, i64 lda, int* flag);

@@ -127,3 +127,29 @@ int lra_launch_split_ids(hipStream_t st, i64 n, i64 k, const i64* jpvt, i64* sk,
   LRA_LAUNCH_CHECK();
   return LRA_OK;
 }
+
+// flag = 0 when the n x n matrix A is not exactly symmetric (ishermitian for real A); the
+// caller sets flag = 1 first.  Each thread compares one (i, j), i > j.
+__global__ void not_symmetric_kernel(i64 n, const double* __restrict__ A, i64 lda, int* flag) {
+  const i64 j = blockIdx.y;
+  for (i64 i = j + 1 + blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n;
+       i += (i64)gridDim.x * blockDim.x)
+    if (A[i + j * lda] != A[j + i * lda])
+      __hip_atomic_store(flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ void set_flag_kernel(int* flag, int v) { *flag = v; }
+
+int lra_launch_is_symmetric(hipStream_t st, i64 n, const double* A, i64 lda, int* flag) {
+  hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(1), 0, st, flag, 1);
+  LRA_LAUNCH_CHECK();
+  if (n <= 1) return LRA_OK;
+  if (n > 65535) {
+    lra_set_error("symmetry test: n=%lld exceeds the grid", (long long)n);
+    return LRA_E_UNSUPPORTED;
+  }
+  const unsigned bx = (unsigned)std::min<i64>(64, (n + 255) / 256);
+  hipLaunchKernelGGL(not_symmetric_kernel, dim3(bx, (unsigned)n), dim3(256), 0, st, n, A, lda,
+                     flag);
+  LRA_LAUNCH_CHECK();
+  return LRA_OK;
+}

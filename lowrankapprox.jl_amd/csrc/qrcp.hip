@@ -5,18 +5,24 @@
 //
 // Design (gfx950): the ell x n sketch is spread over a co-resident grid of G workgroups;
 // every column lives in the registers of `tpc` lanes (lane lic holds rows lic*RPT ..
-// lic*RPT+RPT-1, so every per-row LDS offset is an immediate), so a pivot
-// step never touches HBM.  Each step: every workgroup publishes its best remaining column
-// (norm, position, data), one grid barrier, every workgroup picks the same global winner
-// (largest norm, lowest position -- idamax's first-index rule), computes the same reflector
-// redundantly in LDS, and applies it to its own columns at once (right-looking, unblocked).
-// Exact arithmetic is identical to dlaqps' deferred-update scheme; the block structure of
-// geqp3_adap_main! is emulated so the rank-termination test runs at the same points:
-// blocks of min(nb, k-j+1) columns that end early at the first step whose norm downdate
-// needed a recompute (dlaqps' lsticc), and after each block the |R(jn-1,jn-1)| <= ptol test
-// returns the first small diagonal of the block.  Columns are never moved during the
-// factorization; each carries its LAPACK position (jpvt order) and is written to it at the
-// end.
+// lic*RPT+RPT-1, so every per-row LDS offset is an immediate), so a pivot step never
+// touches HBM.  One step s (pivot row rk):
+//   A  each workgroup finds its best remaining column (largest norm, lowest position --
+//      idamax's first-index rule) with DPP wave reductions and one barrier;
+//   B  that column finishes the previous step's update and builds its own Householder
+//      reflector (dlarfg) in registers, then publishes {tau, beta, v} as tagged chunks;
+//   C  wave 0 publishes {norm, position} in its slot and polls all slots while the other
+//      columns finish the previous step's rank-1 update (the update is lazy: the norm
+//      downdate only needs v'a_j and the new row rk, so the axpy hides in the exchange);
+//   D  the block structure of geqp3_adap_main! is emulated so the rank-termination test
+//      runs at the same points: blocks of min(nb, k-j+1) columns that end early at the
+//      first step whose norm downdate needed a recompute (dlaqps' lsticc), and after each
+//      block the |R(jn-1,jn-1)| <= ptol test returns the first small diagonal of the block;
+//   E  every workgroup copies the global winner's reflector into LDS;
+//   F  dot products v'a_j, position swap and the LAWN 176 norm downdate.
+// Exact arithmetic is that of dlaqps' deferred-update scheme.  Columns are never moved
+// during the factorization; each carries its LAPACK position (jpvt order) and is written
+// to it at the end.
 #include "lra_internal.h"
 
 #include <algorithm>
@@ -26,8 +32,10 @@ namespace {
 
 // Step exchange between blocks: a 16-byte slot {candidate norm, tag} per block and step parity,
 // tag = step+1 | position << 28 | local column << 52 | flag << 63, written with one 16-byte
-// store after the block's candidate column has been written (and acknowledged), read with one
-// 16-byte load, so a reader that sees the step's tag also sees the norm and the column.
+// store.  The candidate's reflector is published beside it as a record of 16-byte
+// {value, chunk tag} chunks: chunk 0 = tau, chunk 1 = beta, chunk 2+i = v(rk+1+i).  Every chunk
+// carries (launch generation, step+1), so a reader validates each chunk by itself and no
+// writer ever waits for an acknowledgement.
 constexpr uint64_t TAG_MASK = (1ull << 28) - 1;
 constexpr uint64_t POS_NONE = (1ull << 24) - 1;
 constexpr uint64_t LC_NONE = (1ull << 11) - 1;
@@ -48,11 +56,11 @@ struct QrcpDev {
   double* vn2g;
   i64* k_out;
   int* early_out;
-  int G, cpw, tpc, cand_stride;
+  int G, cpw, tpc, ltpc, cand_stride;  // tpc = 1 << ltpc lanes per column
   int uncached;       // exchange area bypasses the caches (no write-back/invalidate needed)
   uint32_t gen;       // launch generation, part of every chunk tag
   uint64_t* slots;    // 2G slots of 16 bytes: {norm bits, tag}
-  double* cand;
+  double* cand;       // 2G records of cand_stride 16-byte chunks
   int* err;
   int* abort_flag;
 };
@@ -69,31 +77,107 @@ LRA_DEV double lapy2(double x, double y) {
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
-template <int RPT, int BLOCK>
+// ---- wave-level helpers on DPP lane permutations (VALU latency instead of LDS permutes) ----
+// quad_perm [1,0,3,2] = lane^1, quad_perm [2,3,0,1] = lane^2, row_half_mirror pairs the two
+// quads of each 8 lanes, row_mirror the two halves of each 16 lanes.  Every pattern is an
+// involution, so a+b is formed as b+a in the partner lane: all lanes of a group end with
+// bit-identical results.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HMIRROR = 0x141, DPP_MIRROR = 0x140;
+
+template <int CTRL>
+LRA_DEV double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+LRA_DEV int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+LRA_DEV double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over the `width` consecutive lanes of a group (width a power of two <= 64); call with
+// the whole wave active
+LRA_DEV double group_sum(double v, int width) {
+  if (width >= 2) v += dpp_f64<DPP_XOR1>(v);
+  if (width >= 4) v += dpp_f64<DPP_XOR2>(v);
+  if (width >= 8) v += dpp_f64<DPP_HMIRROR>(v);
+  if (width >= 16) v += dpp_f64<DPP_MIRROR>(v);
+  if (width >= 32) v += __shfl_xor(v, 16, 64);
+  if (width >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+// wave-uniform maximum (NaN-free inputs)
+LRA_DEV double wave_max(double v) {
+  v = fmax(v, dpp_f64<DPP_XOR1>(v));
+  v = fmax(v, dpp_f64<DPP_XOR2>(v));
+  v = fmax(v, dpp_f64<DPP_HMIRROR>(v));
+  v = fmax(v, dpp_f64<DPP_MIRROR>(v));
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+              fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+LRA_DEV int wave_min_i32(int v) {
+  v = min(v, dpp_i32<DPP_XOR1>(v));
+  v = min(v, dpp_i32<DPP_XOR2>(v));
+  v = min(v, dpp_i32<DPP_HMIRROR>(v));
+  v = min(v, dpp_i32<DPP_MIRROR>(v));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+// Wave argmax with idamax's rule (largest value, then lowest key); lanes with value < 0 take
+// no part.  Returns the winning lane or -1; `width` lanes may share one (value, key) pair.
+LRA_DEV int wave_argmax(double v, int key, int width, double* vmax) {
+  const double mx = wave_max(v);
+  *vmax = mx;
+  if (mx < 0.0) return -1;
+  uint64_t msk = __ballot(v == mx);
+  if (__popcll(msk) > (unsigned)width) {  // a tie between different keys: lowest key wins
+    const int kmin = wave_min_i32(v == mx ? key : INT_MAX);
+    msk = __ballot(v == mx && key == kmin);
+  }
+  return __builtin_amdgcn_readfirstlane(__ffsll((long long)msk) - 1);
+}
+
+LRA_DEV void raise_error(const QrcpDev& p, int code) {
+  __hip_atomic_store(p.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int RPT, int BLOCK, bool COOP>
 __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
   constexpr int NW = BLOCK / 64;
+  constexpr int XS = 64 * RPT;  // rows held by one column group (>= m)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tpc = p.tpc;
-  const int gpw = 64 / tpc;
+  const int tpc = p.tpc, ltpc = p.ltpc;
+  const int lgpw = 6 - ltpc;  // columns per wave = 1 << lgpw
   const int lic = lane & (tpc - 1);
-  const int lc = wave * gpw + lane / tpc;
+  const int lc = (wave << lgpw) + (lane >> ltpc);
   const int g = blockIdx.x;
   const int col = g * p.cpw + lc;
   const bool have = (lc < p.cpw) && (col < p.n);
   const int m = p.m;
   const int rbase = lic * RPT;  // first row held by this lane
+  const int glead = lane & ~(tpc - 1);  // first lane of this column's group
   const double tol3z = 1.0536712127723509e-08;  // sqrt(2^-53) = sqrt(dlamch('E'))
 
-  __shared__ double s_x[64 * RPT];
+  // s_x[par] holds the reflector of step s (par = s & 1) by ABSOLUTE row: zeros above row
+  // rk and in the padding, so the dot/update loops need no per-row predicates.  Two buffers:
+  // the rank-1 update of step s is finished lazily during step s+1's exchange.
+  __shared__ double s_x[2][XS];
   __shared__ double s_diag[NB_MAX];
   __shared__ double s_wval[NW];
   __shared__ int s_wpos[NW], s_wlc[NW], s_wflag[NW];
-  __shared__ double s_tau, s_beta, s_ptol, s_winval;
-  __shared__ int s_winowner, s_winlc, s_winpos, s_flagprev, s_blc;  // s_blc: exchange ok
+  __shared__ double s_tau, s_beta, s_ptol;
+  __shared__ double s_pv[NW];
+  __shared__ int s_pp[NW], s_po[NW], s_pl[NW], s_pf[NW], s_pok[NW];
 
-  // s_x holds the current reflector by ABSOLUTE row: zeros above row rk and in the padding,
-  // so the dot/update loops below need no per-row predicates.
-  for (int t = tid; t < 64 * RPT; t += BLOCK) s_x[t] = 0.0;
+  for (int t = tid; t < 2 * XS; t += BLOCK) (&s_x[0][0])[t] = 0.0;
   double a[RPT];
   {
     const double* src = p.A + (i64)col * p.lda + rbase;
@@ -109,7 +193,7 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     double ss = 0.0;
 #pragma unroll
     for (int i = 0; i < RPT; ++i) ss += a[i] * a[i];
-    ss = lra_group_sum(ss, tpc);
+    ss = group_sum(ss, tpc);
     vn1 = vn2 = sqrt(ss);
     jp = col + 1;
   } else if (have) {
@@ -122,25 +206,27 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
   int jb = min(p.nb, p.kmax);
   int k_result = p.kmax;
   int early = 0;
-  bool aborted = false;
+  bool pending = false;  // rank-1 update of the previous step not applied yet
+  double tw_pending = 0.0;
 
-  int s = 0;
-  for (;; ++s) {
-    // ---------------- local candidate (largest norm, then lowest position) ------------
-    double bv = -1.0;
-    int bp = INT_MAX, blc = -1;
-    if (active) { bv = vn1; bp = pos; blc = lc; }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const double ov = __shfl_xor(bv, d, 64);
-      const int op = __shfl_xor(bp, d, 64);
-      const int ol = __shfl_xor(blc, d, 64);
-      if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; blc = ol; }
+  for (int s = 0;; ++s) {
+    const int par = s & 1;
+    const int rk = p.offset + s;
+    const bool done_steps = (s == p.kmax);
+    // ---------------- A: block candidate (largest norm, then lowest position) -----------
+    {
+      double wm;
+      const int wl = wave_argmax(active ? vn1 : -1.0, active ? pos : INT_MAX, tpc, &wm);
+      const int wf = __any(myflag) ? 1 : 0;
+      const int wposv = __builtin_amdgcn_readlane(pos, wl < 0 ? 0 : wl);
+      if (lane == 0) {
+        s_wval[wave] = wm;
+        s_wpos[wave] = wl < 0 ? INT_MAX : wposv;
+        s_wlc[wave] = wl < 0 ? -1 : (wave << lgpw) + (wl >> ltpc);
+        s_wflag[wave] = wf;
+      }
     }
-    const int wflag = __any(myflag) ? 1 : 0;
-    if (lane == 0) { s_wval[wave] = bv; s_wpos[wave] = bp; s_wlc[wave] = blc; s_wflag[wave] = wflag; }
     __syncthreads();
-    // every thread reduces the NW wave candidates the same way
     double cv = s_wval[0];
     int cp = s_wpos[0], clc = s_wlc[0], cflag = s_wflag[0];
 #pragma unroll
@@ -150,67 +236,125 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
       if (ov > cv || (ov == cv && op < cp)) { cv = ov; cp = op; clc = s_wlc[w]; }
       cflag |= s_wflag[w];
     }
-    const bool done_steps = (s == p.kmax);
+    myflag = 0;
 
-    if (p.G > 1) {
-      const int rk = p.offset + s;
-      const int len = m - rk;
-      const size_t sl = (size_t)(s & 1) * p.G + g;
-      // stage the block candidate's rows rk..m-1 in s_x (absolute rows; the winner's data
-      // overwrites them below, rows < rk and >= m stay zero)
-      if (!done_steps && have && active && lc == clc) {
-        double* d = s_x + rbase;
-        const int lo = rk - rbase, hi = m - rbase;
+    // ---------------- B: the candidate column builds its reflector (dlarfg) -------------
+    // The candidate first finishes the previous step's update.  G > 1: the reflector is
+    // published as a record; G == 1: it goes straight to s_x[par].
+    const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
+    const size_t sl = (size_t)par * p.G + g;
+    if (!done_steps && clc >= 0 && wave == (clc >> lgpw)) {
+      const bool cand = have && lc == clc;
+      if (cand && pending) {
+        const double* vo = s_x[par ^ 1] + rbase;
 #pragma unroll
-        for (int i = 0; i < RPT; ++i)
-          if (i >= lo && i < hi) d[i] = a[i];
+        for (int i = 0; i < RPT; ++i) a[i] -= tw_pending * vo[i];
+        pending = false;
       }
-      __syncthreads();
-      if (wave == 0) {
-        volatile u64x2* slots = reinterpret_cast<volatile u64x2*>(p.slots);
-        // publish: the column as 16-byte {value, step tag} chunks (each chunk validates
-        // itself, so no acknowledgement wait is needed), then the {norm, tag} slot
-        const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
-        if (!done_steps) {
-          volatile u64x2* dst = reinterpret_cast<volatile u64x2*>(p.cand) + sl * p.cand_stride;
-          for (int t = lane; t < len; t += 64) {
-            u64x2 w;
-            w.x = (uint64_t)__double_as_longlong(s_x[rk + t]);
-            w.y = ctag;
-            dst[t] = w;
+      const int lo = rk - rbase;
+      double ss = 0.0, al = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        ss += (i > lo) ? a[i] * a[i] : 0.0;
+        al = (i == lo) ? a[i] : al;
+      }
+      ss = group_sum(ss, tpc);
+      const double alpha0 = __shfl(al, glead | (rk / RPT), 64);
+      double alpha = alpha0, xnorm = sqrt(ss), beta = alpha0, tau = 0.0, scal = 1.0, fac = 1.0;
+      const int len = m - rk;
+      if (len > 1 && xnorm != 0.0) {
+        beta = -copysign(lapy2(alpha, xnorm), alpha);
+        const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+        int knt = 0;
+        if (fabs(beta) < safmin) {
+          const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+          do {
+            ++knt;
+            fac *= rsafmn;
+            beta *= rsafmn;
+            alpha *= rsafmn;
+          } while (fabs(beta) < safmin && knt < 20);
+          double s2 = 0.0;
+#pragma unroll
+          for (int i = 0; i < RPT; ++i) {
+            const double xi = a[i] * fac;
+            s2 += (i > lo) ? xi * xi : 0.0;
           }
+          s2 = group_sum(s2, tpc);
+          xnorm = sqrt(s2);
+          beta = -copysign(lapy2(alpha, xnorm), alpha);
+        }
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+        for (int j = 0; j < knt; ++j) beta *= safmin;
+      }
+      // the candidate's v goes to s_x[par] (rows <= rk get zeros; row rk becomes 1 in E),
+      // unconditionally, so no per-row predicate or branch is needed
+      if (cand) {
+        double* d = s_x[par] + rbase;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) d[i] = (i > lo) ? (a[i] * fac) * scal : 0.0;
+      }
+      // the candidate group's first lane holds tau and beta
+      const int cl = (clc & ((1 << lgpw) - 1)) << ltpc;
+      const double ctau = readlane_f64(tau, cl), cbeta = readlane_f64(beta, cl);
+      if (lane == 0) { s_tau = ctau; s_beta = cbeta; }
+      if (COOP) {
+        // the whole wave publishes {tau, beta, v(rk+1..m-1)} as coalesced tagged chunks
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        volatile u64x2* rec = reinterpret_cast<volatile u64x2*>(p.cand) + sl * p.cand_stride;
+        const double* xs = s_x[par];
+        for (int t = lane; t < len + 1; t += 64) {
+          u64x2 w;
+          const double val = t == 0 ? ctau : (t == 1 ? cbeta : xs[rk + t - 1]);
+          w.x = (uint64_t)__double_as_longlong(val);
+          w.y = ctag;
+          rec[t] = w;
         }
         if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0) {
-          u64x2 w;
-          w.x = (uint64_t)__double_as_longlong(cv);
-          w.y = pack_tag(s + 1, cp, clc, cflag);
-          slots[sl] = w;
-        }
-        // wait until every block has published this step (bounded)
-        const uint64_t want = (uint64_t)(s + 1) & TAG_MASK;
-        const size_t base = (size_t)(s & 1) * p.G;
+      }
+    }
+
+    // ---------------- C: exchange (G > 1) while the other columns finish their update ----
+    // Wave w polls slots [64w, 64w+64) and reduces them; the wave results meet in LDS.
+    if (COOP && !done_steps) {
+      volatile u64x2* slots = reinterpret_cast<volatile u64x2*>(p.slots);
+      if (tid == 0) {
+        if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        u64x2 w;
+        w.x = (uint64_t)__double_as_longlong(cv);
+        w.y = pack_tag(s + 1, cp, clc, cflag);
+        slots[sl] = w;
+      }
+      if (pending) {
+        const double* vo = s_x[par ^ 1] + rbase;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) a[i] -= tw_pending * vo[i];
+        pending = false;
+      }
+      const uint64_t want = (uint64_t)(s + 1) & TAG_MASK;
+      const int t = (wave << 6) + lane;
+      const bool has_slot = t < p.G;
+      const volatile u64x2* src = slots + (size_t)par * p.G + t;
+      u64x2 mine;
+      mine.x = 0;
+      mine.y = 0;
+      int ok = 1;
+      if ((wave << 6) < p.G) {  // wave-uniform: this wave has slots to watch (bounded wait)
         unsigned spins = 0;
-        int ok = 1;
-        u64x2 mine[4];  // G <= 256 slots, up to 4 per lane
         for (;;) {
           bool ready = true;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int t = lane + 64 * u;
-            if (t < p.G) {
-              mine[u] = slots[base + t];
-              ready &= ((mine[u].y & TAG_MASK) == want);
-            }
+          if (has_slot) {
+            mine = *src;
+            ready = (mine.y & TAG_MASK) == want;
           }
           if (__all(ready)) break;
           if ((++spins & 63) == 0) {
             if (__hip_atomic_load(p.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
             if (spins > LRA_SPIN_LIMIT) {
-              if (lane == 0) {
-                __hip_atomic_store(p.err, (int)LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              }
+              if (lane == 0) raise_error(p, LRA_E_TIMEOUT);
               ok = 0;
               break;
             }
@@ -218,54 +362,70 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
           __builtin_amdgcn_s_sleep(1);
         }
         if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        double v = -1.0;
-        int vp = INT_MAX, vo = -1, vl = -1, fl = 0;
-        if (ok) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int t = lane + 64 * u;
-            if (t >= p.G) continue;
-            const uint64_t tg = mine[u].y;
-            const double sv = __longlong_as_double((long long)mine[u].x);
-            const uint64_t pp = (tg >> 28) & POS_NONE;
-            const int spos = pp == POS_NONE ? INT_MAX : (int)pp;
-            const uint64_t ll = (tg >> 52) & LC_NONE;
-            fl |= (int)(tg >> 63);
-            if (sv > v || (sv == v && spos < vp)) {
-              v = sv; vp = spos; vo = t; vl = ll == LC_NONE ? -1 : (int)ll;
-            }
-          }
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const double ov = __shfl_xor(v, d, 64);
-            const int op = __shfl_xor(vp, d, 64);
-            const int oo = __shfl_xor(vo, d, 64);
-            const int ol = __shfl_xor(vl, d, 64);
-            if (ov > v || (ov == v && op < vp)) { v = ov; vp = op; vo = oo; vl = ol; }
-            fl |= __shfl_xor(fl, d, 64);
-          }
-        }
-        if (lane == 0) {
-          s_winval = v; s_winpos = vp; s_winowner = vo; s_winlc = vl; s_flagprev = fl;
-          s_blc = ok;
-        }
       }
-      __syncthreads();
-      if (!s_blc) { aborted = true; break; }
-    } else {
-      if (tid == 0) { s_winval = cv; s_winpos = cp; s_winowner = 0; s_winlc = clc; s_flagprev = cflag; }
-      __syncthreads();
+      double v = -1.0;
+      int vp = INT_MAX, vl = -1, fl = 0;
+      if (ok && has_slot) {
+        const uint64_t tg = mine.y;
+        const uint64_t pp = (tg >> 28) & POS_NONE;
+        const uint64_t ll = (tg >> 52) & LC_NONE;
+        v = __longlong_as_double((long long)mine.x);
+        vp = pp == POS_NONE ? INT_MAX : (int)pp;
+        vl = ll == LC_NONE ? -1 : (int)ll;
+        fl = (int)(tg >> 63);
+      }
+      double vmax;
+      const int wl = wave_argmax(v, vp, 1, &vmax);
+      const int anyfl = __any(fl) ? 1 : 0;
+      const int wpos = __builtin_amdgcn_readlane(vp, wl < 0 ? 0 : wl);
+      const int wlc = __builtin_amdgcn_readlane(vl, wl < 0 ? 0 : wl);
+      if (lane == 0) {
+        s_pv[wave] = vmax;
+        s_pp[wave] = wl < 0 ? INT_MAX : wpos;
+        s_po[wave] = wl < 0 ? -1 : (wave << 6) + wl;
+        s_pl[wave] = wl < 0 ? -1 : wlc;
+        s_pf[wave] = anyfl;
+        s_pok[wave] = ok;
+      }
+    } else if (pending) {
+      const double* vo = s_x[par ^ 1] + rbase;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) a[i] -= tw_pending * vo[i];
+      pending = false;
     }
-    myflag = 0;
+    __syncthreads();
+    // the global winner: every thread reduces the wave results the same way
+    double win_val = cv;
+    int win_pos = cp, win_owner = g, win_lc = clc, flagprev = cflag, wok = 1;
+    if (COOP && !done_steps) {
+      win_val = s_pv[0]; win_pos = s_pp[0]; win_owner = s_po[0]; win_lc = s_pl[0];
+      flagprev = s_pf[0]; wok = s_pok[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) {
+        const double ov = s_pv[w];
+        const int op = s_pp[w];
+        if (ov > win_val || (ov == win_val && op < win_pos)) {
+          win_val = ov; win_pos = op; win_owner = s_po[w]; win_lc = s_pl[w];
+        }
+        flagprev |= s_pf[w];
+        wok &= s_pok[w];
+      }
+    }
+    if (!wok) return;  // bounded wait expired or another block failed: no write-out
+    if (!done_steps && (win_owner < 0 || win_lc < 0)) {
+      // no candidate anywhere (only possible with NaN norms) is an error, not a pivot
+      if (tid == 0) raise_error(p, LRA_E_STATE);
+      return;
+    }
 
-    // ---------------- block bookkeeping for the step just finished (s-1) --------------
+    // ---------------- D: block bookkeeping for the step just finished (s-1) -------------
     if (s == 0) {
-      if (tid == 0) s_ptol = fmax(p.atol, p.rtol * s_winval);
-      __syncthreads();
+      if (tid == 0) s_ptol = fmax(p.atol, p.rtol * win_val);
+      // (read after the next barrier)
     } else {
       const int last = s - 1;
       const bool size_end = (last - j0 + 1 == jb);
-      const bool blk_end = size_end || (s_flagprev != 0) || done_steps;
+      const bool blk_end = size_end || (flagprev != 0) || done_steps;
       if (blk_end) {
         if (p.mode == 1) {
           k_result = s;
@@ -286,151 +446,121 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     }
     if (done_steps) { k_result = s; break; }
 
-    // ---------------- winner column -> LDS ---------------------------------------------
-    const int rk = p.offset + s;
+    // ---------------- E: the winner's reflector -> s_x[par] ------------------------------
     const int len = m - rk;
-    const int win_owner = s_winowner, win_lc = s_winlc, win_pos = s_winpos;
-    if (tid == 0 && rk > 0) s_x[rk - 1] = 0.0;  // previous step's v(1) = 1
-    if (p.G > 1) {
-      const volatile u64x2* src = reinterpret_cast<const volatile u64x2*>(p.cand) +
-                                  ((size_t)(s & 1) * p.G + win_owner) * p.cand_stride;
-      const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
-      for (int t = tid; t < len; t += BLOCK) {
-        u64x2 w = src[t];
-        unsigned spins = 0;
-        while (w.y != ctag && ++spins < LRA_SPIN_LIMIT) {  // chunk not landed yet
-          __builtin_amdgcn_s_sleep(1);
-          w = src[t];
-        }
-        if (w.y != ctag) {
-          __hip_atomic_store(p.err, (int)LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_x[rk + t] = __longlong_as_double((long long)w.x);
-      }
-    } else {
-      if (have && active && lc == win_lc) {
-        double* d = s_x + rbase;
-        const int lo = rk - rbase, hi = m - rbase;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i)
-          if (i >= lo && i < hi) d[i] = a[i];
-      }
-    }
-    __syncthreads();
-
-    // ---------------- Householder reflector (dlarfg) in wave 0 --------------------------
-    if (wave == 0) {
-      double* x = s_x + rk;
-      double alpha = x[0];
-      double ss = 0.0;
-      for (int t = 1 + lane; t < len; t += 64) ss += x[t] * x[t];
-      ss = lra_wave_sum(ss);
-      double xnorm = sqrt(ss);
-      double tau = 0.0, beta = alpha, scal = 1.0;
-      if (len > 1 && xnorm != 0.0) {
-        beta = -copysign(lapy2(alpha, xnorm), alpha);
-        const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
-        int knt = 0;
-        if (fabs(beta) < safmin) {
-          const double rsafmn = 1.0 / safmin;
-          do {
-            ++knt;
-            for (int t = 1 + lane; t < len; t += 64) x[t] *= rsafmn;
-            beta *= rsafmn;
-            alpha *= rsafmn;
-          } while (fabs(beta) < safmin && knt < 20);
-          double s2 = 0.0;
-          for (int t = 1 + lane; t < len; t += 64) s2 += x[t] * x[t];
-          s2 = lra_wave_sum(s2);
-          xnorm = sqrt(s2);
-          beta = -copysign(lapy2(alpha, xnorm), alpha);
-        }
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-        for (int j = 0; j < knt; ++j) beta *= safmin;
-      }
-      if (lane == 0) { s_tau = tau; s_beta = beta; }
-      // scale x into v (v(1) = 1 implicit, stored at s_x[0] below)
-      for (int t = 1 + lane; t < len; t += 64) x[t] *= scal;
-    }
-    __syncthreads();
+    double* xs = s_x[par];
     if (tid == 0) {
-      s_x[rk] = 1.0;
-      if (g == 0) p.tau[s] = s_tau;
-      s_diag[s - j0] = s_beta;
+      if (rk >= 1) xs[rk - 1] = 0.0;  // rows of the reflector two steps back
+      if (rk >= 2) xs[rk - 2] = 0.0;
+      xs[rk] = 1.0;
+    }
+    if (COOP && win_owner != g) {  // the owner already holds its reflector in LDS
+      const volatile u64x2* rec = reinterpret_cast<const volatile u64x2*>(p.cand) +
+                                  ((size_t)par * p.G + win_owner) * p.cand_stride;
+      for (int t = tid; t < len + 1; t += BLOCK) {
+        u64x2 w = rec[t];
+        unsigned spins = 0;
+        while (w.y != ctag) {  // chunk not landed yet (bounded wait)
+          if ((++spins & 63) == 0 &&
+              (spins > LRA_SPIN_LIMIT ||
+               __hip_atomic_load(p.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            break;
+          __builtin_amdgcn_s_sleep(1);
+          if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          w = rec[t];
+        }
+        if (w.y != ctag) raise_error(p, LRA_E_TIMEOUT);
+        const double val = __longlong_as_double((long long)w.x);
+        if (t == 0) s_tau = val;
+        else if (t == 1) s_beta = val;
+        else xs[rk + t - 1] = val;
+      }
     }
     __syncthreads();
     const double tau = s_tau, beta = s_beta;
+    if (tid == 0) {
+      if (g == 0) p.tau[s] = tau;
+      s_diag[s - j0] = beta;  // read after the next step's barriers
+    }
 
-    // ---------------- apply H to the block's columns, position swap, norm downdate ------
+    // ---------------- F: apply H to the block's columns, position swap, norm downdate ----
     const bool is_win = have && active && (g == win_owner) && (lc == win_lc);
     if (is_win) {
-      const double* vx = s_x + rbase;
-      const int lo = rk - rbase, hi = m - rbase;
+      const double* vx = xs + rbase;  // zero in the padding rows
+      const int lo = rk - rbase;
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
         if (i == lo) a[i] = beta;
-        else if (i > lo && i < hi) a[i] = vx[i];
+        else if (i > lo) a[i] = vx[i];
       }
       active = false;
       pos = s;
     } else {
       if (have && pos == s) pos = win_pos;  // LAPACK swaps columns k and pvt
+    }
+    // every lane runs the dot (group sums need the whole wave); inactive lanes discard it
+    {
+      const double* vx = xs + rbase;
+      double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPT; i += 4) {
+        w0 += vx[i] * a[i];
+        if (i + 1 < RPT) w1 += vx[i + 1] * a[i + 1];
+        if (i + 2 < RPT) w2 += vx[i + 2] * a[i + 2];
+        if (i + 3 < RPT) w3 += vx[i + 3] * a[i + 3];
+      }
+      // re-read v from LDS below instead of keeping RPT values live across the reductions
+      asm volatile("" ::: "memory");
+      const double w = group_sum((w0 + w1) + (w2 + w3), tpc);
+      const double tw = tau * w;
+      // new value of row rk (v(rk) = 1): lives in lane rk / RPT of the group
+      const int irk = rk % RPT;
+      double ar = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) ar = (i == irk) ? a[i] : ar;
+      ar = __shfl(ar, glead | (rk / RPT), 64);
       if (active) {
-        const double* vx = s_x + rbase;
-        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
-#pragma unroll
-        for (int i = 0; i < RPT; i += 4) {
-          w0 += vx[i] * a[i];
-          if (i + 1 < RPT) w1 += vx[i + 1] * a[i + 1];
-          if (i + 2 < RPT) w2 += vx[i + 2] * a[i + 2];
-          if (i + 3 < RPT) w3 += vx[i + 3] * a[i + 3];
-        }
-        double w = lra_group_sum((w0 + w1) + (w2 + w3), tpc);
-        const double tw = tau * w;
-        // re-read v from LDS below instead of keeping RPT values live across the two loops
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) a[i] -= tw * vx[i];
-        if (rk + 1 < p.lastrk) {
-          // new value of row rk lives in lane rk / RPT of the group, register rk % RPT
-          const int irk = rk % RPT;
-          double xr = 0.0;
-#pragma unroll
-          for (int i = 0; i < RPT; ++i) xr = (i == irk) ? a[i] : xr;
-          xr = __shfl(xr, (lane & ~(tpc - 1)) | (rk / RPT), 64);
-          if (vn1 != 0.0) {
-            double temp = fabs(xr) / vn1;
-            temp = (1.0 + temp) * (1.0 - temp);
-            temp = temp < 0.0 ? 0.0 : temp;
-            const double r = vn1 / vn2;
-            const double temp2 = temp * r * r;
-            if (temp2 <= tol3z) {
-              double ss = 0.0;
-              const int lo = rk - rbase;
-#pragma unroll
-              for (int i = 0; i < RPT; ++i) ss += (i > lo) ? a[i] * a[i] : 0.0;
-              ss = lra_group_sum(ss, tpc);
-              vn1 = vn2 = sqrt(ss);
-              myflag = 1;
-            } else {
-              vn1 *= sqrt(temp);
-            }
-          }
+        pending = true;
+        tw_pending = tw;
+        if (rk + 1 < p.lastrk && vn1 != 0.0) {
+          const double xr = ar - tw;
+          double temp = fabs(xr) / vn1;
+          temp = (1.0 + temp) * (1.0 - temp);
+          temp = temp < 0.0 ? 0.0 : temp;
+          const double r = vn1 / vn2;
+          const double temp2 = temp * r * r;
+          if (temp2 <= tol3z) myflag = 1;
+          else vn1 *= sqrt(temp);
         }
       }
+      // columns whose norm must be recomputed finish the update now (group-uniform)
+      const bool recompute = active && myflag;
+      if (__any(recompute)) {
+        if (recompute) {
+#pragma unroll
+          for (int i = 0; i < RPT; ++i) a[i] -= tw * vx[i];
+          pending = false;
+        }
+        double ss = 0.0;
+        const int lo = rk - rbase;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) ss += (i > lo) ? a[i] * a[i] : 0.0;
+        ss = group_sum(ss, tpc);
+        if (recompute) vn1 = vn2 = sqrt(ss);
+      }
     }
-    // s_x is rewritten next step only after the next __syncthreads
+    // s_x[par] is rewritten two steps later, after two more barriers
   }
 
-  if (aborted) return;
+  // (every exit of the loop comes after phase C, so no update is pending here)
   // ---------------- write-out to LAPACK positions --------------------------------------
-  // (with G > 1 every block loaded its columns before step 0's barrier, so the in-place
+  // (with G > 1 every block loaded its columns before step 0's exchange, so the in-place
   // permuted write cannot overwrite a column somebody still has to read)
   if (have) {
     double* dst = p.A + (i64)pos * p.lda + rbase;
-    const int hi = m - rbase;
+    int mm = m;
+    asm volatile("" : "+s"(mm));  // keep the row masks below from being computed early
+    const int hi = mm - rbase;
 #pragma unroll
     for (int i = 0; i < RPT; ++i)
       if (i < hi) dst[i] = a[i];
@@ -445,39 +575,43 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
   }
 }
 
-template <int RPT, int BLOCK>
-hipError_t launch_rpt(const QrcpDev& d, hipStream_t st, bool coop) {
+template <int RPT, int BLOCK, bool COOP>
+hipError_t launch_rpt(const QrcpDev& d, hipStream_t st) {
   dim3 grid(d.G), block(BLOCK);
-  if (coop) {
+  if constexpr (COOP) {
     void* args[] = {(void*)&d};
-    return hipLaunchCooperativeKernel((const void*)qrcp_kernel<RPT, BLOCK>, grid, block, args,
-                                      0, st);
+    return hipLaunchCooperativeKernel((const void*)qrcp_kernel<RPT, BLOCK, true>, grid, block,
+                                      args, 0, st);
+  } else {
+    hipLaunchKernelGGL((qrcp_kernel<RPT, BLOCK, false>), grid, block, 0, st, d);
+    return hipGetLastError();
   }
-  hipLaunchKernelGGL((qrcp_kernel<RPT, BLOCK>), grid, block, 0, st, d);
-  return hipGetLastError();
 }
 
 #define LRA_RPT_256(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48) \
   X(52) X(56) X(60) X(64) X(68) X(72) X(76) X(80)
-#define LRA_RPT_512(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48)
-#define LRA_RPT_1024(X) X(4) X(8) X(12) X(16) X(20)
+#define LRA_RPT_512(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44)
+#define LRA_RPT_1024(X) X(4) X(8)
 
-hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st, bool coop) {
-  if (block == 1024) {
+// Single-block shapes (512 and 1024 threads, or 256 with G == 1) run without the exchange;
+// cooperative grids use the 256-thread variant with it.
+hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st) {
+  const bool coop = d.G > 1;
+  if (block == 1024 && !coop) {
     switch (rpt) {
-#define CASE(R) case R: return launch_rpt<R, 1024>(d, st, coop);
+#define CASE(R) case R: return launch_rpt<R, 1024, false>(d, st);
       LRA_RPT_1024(CASE)
 #undef CASE
     }
-  } else if (block == 512) {
+  } else if (block == 512 && !coop) {
     switch (rpt) {
-#define CASE(R) case R: return launch_rpt<R, 512>(d, st, coop);
+#define CASE(R) case R: return launch_rpt<R, 512, false>(d, st);
       LRA_RPT_512(CASE)
 #undef CASE
     }
-  } else {
+  } else if (block == 256) {
     switch (rpt) {
-#define CASE(R) case R: return launch_rpt<R, 256>(d, st, coop);
+#define CASE(R) case R: return coop ? launch_rpt<R, 256, true>(d, st) : launch_rpt<R, 256, false>(d, st);
       LRA_RPT_256(CASE)
 #undef CASE
     }
@@ -485,26 +619,13 @@ hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st, bool
   return hipErrorInvalidValue;
 }
 
-int occupancy_any(int rpt, int block, int* nb) {
+// co-resident 256-thread blocks per CU of the cooperative variant
+int occupancy_coop(int rpt, int* nb) {
   hipError_t e = hipErrorInvalidValue;
-  if (block == 1024) {
-    switch (rpt) {
-#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 1024>, 1024, 0); break;
-      LRA_RPT_1024(CASE)
+  switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 256, true>, 256, 0); break;
+    LRA_RPT_256(CASE)
 #undef CASE
-    }
-  } else if (block == 512) {
-    switch (rpt) {
-#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 512>, 512, 0); break;
-      LRA_RPT_512(CASE)
-#undef CASE
-    }
-  } else {
-    switch (rpt) {
-#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 256>, 256, 0); break;
-      LRA_RPT_256(CASE)
-#undef CASE
-    }
   }
   return e == hipSuccess ? 0 : 1;
 }
@@ -522,11 +643,11 @@ static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
     int rpt = (m + tpc - 1) / tpc;
     rpt = (rpt + 3) / 4 * 4;
     if (rpt < 4) rpt = 4;
-    if (rpt <= 20 && n <= 1024 / tpc) {
+    if (rpt <= 8 && n <= 1024 / tpc) {
       *out = {tpc, rpt, 1024, 1, 1024 / tpc};
       return true;
     }
-    if (rpt <= 48 && n <= 512 / tpc) {
+    if (rpt <= 44 && n <= 512 / tpc) {
       *out = {tpc, rpt, 512, 1, 512 / tpc};
       return true;
     }
@@ -539,7 +660,7 @@ static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
     const int cpw = 256 / tpc;
     const int G = (n + cpw - 1) / cpw;
     int per_cu = 0;
-    if (occupancy_any(rpt, 256, &per_cu) != 0 || per_cu < 1) continue;
+    if (occupancy_coop(rpt, &per_cu) != 0 || per_cu < 1) continue;
     if (G == 1 || G <= per_cu * ctx->num_cu) {
       *out = {tpc, rpt, 256, G, cpw};
       return true;
@@ -576,13 +697,16 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   d.atol = q.atol; d.rtol = q.rtol; d.jpvt = q.jpvt; d.tau = q.tau; d.vn1g = q.vn1;
   d.vn2g = q.vn2; d.k_out = q.k_out; d.early_out = q.early_out;
   d.G = sh.G; d.cpw = sh.cpw; d.tpc = sh.tpc;
-  d.cand_stride = ((q.m + 7) / 8) * 8;
+  d.ltpc = 0;
+  while ((1 << d.ltpc) < sh.tpc) ++d.ltpc;
+  // tau, beta and reflector rows up to the column group's padded height tpc*rpt
+  d.cand_stride = ((sh.tpc * sh.rpt + 2 + 7) / 8) * 8;
   d.err = ctx->d_err;
   if (sh.G > 256) {
     lra_set_error("qrcp: %d blocks exceed the 256-slot exchange", sh.G);
     return LRA_E_UNSUPPORTED;
   }
-  // exchange area: [header 256 B: abort flag][slots 2G x 16 B][cand 2G x stride doubles]
+  // exchange area: [header 256 B: abort flag][slots 2G x 16 B][records 2G x stride x 16 B]
   const size_t slot_bytes = 16 * 2 * (size_t)sh.G;
   const size_t cand_bytes = 16 * 2 * (size_t)sh.G * d.cand_stride;  // {value, tag} chunks
   void* area = nullptr;
@@ -597,7 +721,7 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   // chunk tags carry the launch generation instead
   LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + slot_bytes) / 4));
   d.gen = ++ctx->qrcp_gen;
-  hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream, sh.G > 1);
+  hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream);
   if (e != hipSuccess) {
     lra_set_error("qrcp launch (m=%d n=%d tpc=%d rpt=%d G=%d): %s", q.m, q.n, sh.tpc, sh.rpt,
                   sh.G, hipGetErrorString(e));

@@ -164,9 +164,17 @@ struct Op {
   i64 ldt;
   const double* B;  // second dense operand (snormdiff of two matrices) or nullptr
   i64 ldb;
+  lra_apply_fn apply = nullptr;  // matrix-free operator (replaces A when set)
+  void* user = nullptr;
 };
 
 int op_apply(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) {
+  if (o.apply) {
+    const int st = o.apply(o.user, 0, 1, x, std::max<i64>(o.n, 1), y, std::max<i64>(o.m, 1),
+                           (void*)c->stream);
+    if (st != 0) { lra_set_error("operator apply returned %d", st); return LRA_E_STATE; }
+    return LRA_OK;
+  }
   LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.A, o.lda, nullptr, x, 1.0, y, 0.0));
   if (o.B) LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.B, o.ldb, nullptr, x, -1.0, y, 1.0));
   if (o.k > 0) {
@@ -180,6 +188,12 @@ int op_apply(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) {
 }
 
 int op_apply_t(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) {
+  if (o.apply) {
+    const int st = o.apply(o.user, 1, 1, x, std::max<i64>(o.m, 1), y, std::max<i64>(o.n, 1),
+                           (void*)c->stream);
+    if (st != 0) { lra_set_error("operator apply returned %d", st); return LRA_E_STATE; }
+    return LRA_OK;
+  }
   LRA_CHECK(lra_gemv_t(c, o.m, o.n, o.A, o.lda, nullptr, x, 1.0, y, 0.0));
   if (o.B) LRA_CHECK(lra_gemv_t(c, o.m, o.n, o.B, o.ldb, nullptr, x, -1.0, y, 1.0));
   if (o.k > 0) {
@@ -277,4 +291,21 @@ extern "C" int lra_snormdiff_id(lra_ctx* c, int64_t m, int64_t n, const double* 
   Op o{m, n, A, lda, k, sk, rd, T, ldt, nullptr, 0};
   if (k == 0) o.k = -1;
   return power_iteration(c, o, 0, opts, x0, result);
+}
+
+// snorm of a matrix-free operator (snorm(L::LinearOperator), src/snorm.jl:14-45): the power
+// iteration applies the caller's L and L' to one vector per step.
+extern "C" int lra_snorm_op(lra_ctx* c, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                            int isherm, const lra_opts* opts, const double* x0, double* result) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(apply != nullptr, 4);
+  LRA_ARG(!isherm || m == n, 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(result != nullptr, 9);
+  Op o{m, n, nullptr, 1, -1, nullptr, nullptr, nullptr, 0, nullptr, 0};
+  o.apply = apply;
+  o.user = user;
+  return power_iteration(c, o, isherm, opts, x0, result);
 }

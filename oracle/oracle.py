@@ -706,9 +706,10 @@ def pheigvals(A, opts: LRAOptions = None, rand: Rand = None, **kw):
     return w
 
 
-def curfact(A, opts: LRAOptions = None, rand: Rand = None, **kw):
+def curfact(A, opts: LRAOptions = None, rand: Rand = None, rand2: Rand = None, **kw):
     """curfact (src/cur.jl:532-564); returns (rows, cols) 1-based (Hermitian/symmetric:
-    rows == cols)."""
+    rows == cols).  rand2, when given, supplies the second sketchfact's randomness (default:
+    rand, continuing as the reference's global generator would)."""
     opts = (opts or LRAOptions()).copy(**kw)
     opts.pqrfact_retval = ""
     chkopts(opts)
@@ -721,15 +722,16 @@ def curfact(A, opts: LRAOptions = None, rand: Rand = None, **kw):
         cols = F.p[:F.k]
         return cols, cols
     m, n = A.shape
+    rand2 = rand2 or rand
     if m >= n:
         F = sketchfact("left", "c", A, opts, rand)
         rows = F.p[:F.k]
-        F = sketchfact("left", "n", A[rows - 1, :], opts, rand)
+        F = sketchfact("left", "n", A[rows - 1, :], opts, rand2)
         cols = F.p[:F.k]
     else:
         F = sketchfact("left", "n", A, opts, rand)
         cols = F.p[:F.k]
-        F = sketchfact("left", "c", A[:, cols - 1], opts, rand)
+        F = sketchfact("left", "c", A[:, cols - 1], opts, rand2)
         rows = F.p[:F.k]
     k = min(len(rows), len(cols))
     return rows[:k], cols[:k]
