@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 
 namespace {
 
@@ -147,6 +149,32 @@ LRA_DEV int wave_argmax(double v, int key, int width, double* vmax) {
 LRA_DEV void raise_error(const QrcpDev& p, int code) {
   __hip_atomic_store(p.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(p.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 16-byte exchange store.  Uncached memory: a plain store (no wait for its completion; the
+// chunk tag validates it on the reader's side).  Cached fallback: a volatile store, made
+// visible by the caller's release fence.
+LRA_DEV void put_chunk(u64x2* dst, u64x2 w, int uncached) {
+  if (uncached) *dst = w;
+  else *(volatile u64x2*)dst = w;
+}
+
+// re-read a chunk until its tag is `tag` (bounded; on expiry or abort the error is raised)
+LRA_DEV u64x2 wait_chunk(const QrcpDev& p, const u64x2* src, uint64_t tag) {
+  const volatile u64x2* v = (const volatile u64x2*)src;
+  u64x2 w = *v;
+  unsigned spins = 0;
+  while (w.y != tag) {
+    if ((++spins & 63) == 0 &&
+        (spins > LRA_SPIN_LIMIT ||
+         __hip_atomic_load(p.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      break;
+    __builtin_amdgcn_s_sleep(1);
+    if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    w = *v;
+  }
+  if (w.y != tag) raise_error(p, LRA_E_TIMEOUT);
+  return w;
 }
 
 template <int RPT, int BLOCK, bool COOP>
@@ -304,15 +332,16 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        volatile u64x2* rec = reinterpret_cast<volatile u64x2*>(p.cand) + sl * p.cand_stride;
+        u64x2* rec = reinterpret_cast<u64x2*>(p.cand) + sl * p.cand_stride;
         const double* xs = s_x[par];
         for (int t = lane; t < len + 1; t += 64) {
           u64x2 w;
           const double val = t == 0 ? ctau : (t == 1 ? cbeta : xs[rk + t - 1]);
           w.x = (uint64_t)__double_as_longlong(val);
           w.y = ctag;
-          rec[t] = w;
+          put_chunk(rec + t, w, p.uncached);
         }
+        asm volatile("" ::: "memory");  // the stores stay ahead of the exchange below
         if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       }
     }
@@ -326,7 +355,8 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
         u64x2 w;
         w.x = (uint64_t)__double_as_longlong(cv);
         w.y = pack_tag(s + 1, cp, clc, cflag);
-        slots[sl] = w;
+        put_chunk(reinterpret_cast<u64x2*>(p.slots) + sl, w, p.uncached);
+        asm volatile("" ::: "memory");
       }
       if (pending) {
         const double* vo = s_x[par ^ 1] + rbase;
@@ -455,25 +485,35 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
       xs[rk] = 1.0;
     }
     if (COOP && win_owner != g) {  // the owner already holds its reflector in LDS
-      const volatile u64x2* rec = reinterpret_cast<const volatile u64x2*>(p.cand) +
-                                  ((size_t)par * p.G + win_owner) * p.cand_stride;
-      for (int t = tid; t < len + 1; t += BLOCK) {
-        u64x2 w = rec[t];
-        unsigned spins = 0;
-        while (w.y != ctag) {  // chunk not landed yet (bounded wait)
-          if ((++spins & 63) == 0 &&
-              (spins > LRA_SPIN_LIMIT ||
-               __hip_atomic_load(p.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-            break;
-          __builtin_amdgcn_s_sleep(1);
-          if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          w = rec[t];
+      const u64x2* rec = reinterpret_cast<const u64x2*>(p.cand) +
+                         ((size_t)par * p.G + win_owner) * p.cand_stride;
+      // up to 4 chunks per thread in flight at once, then validated (a chunk that has not
+      // landed yet is re-read with a bounded wait)
+      for (int t0 = tid; t0 < len + 1; t0 += 4 * BLOCK) {
+        u64x2 w[4];
+        if (p.uncached) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int t = t0 + q * BLOCK;
+            w[q] = t < len + 1 ? rec[t] : u64x2{0, 0};
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int t = t0 + q * BLOCK;
+            if (t < len + 1) w[q] = *(const volatile u64x2*)(rec + t);
+          }
         }
-        if (w.y != ctag) raise_error(p, LRA_E_TIMEOUT);
-        const double val = __longlong_as_double((long long)w.x);
-        if (t == 0) s_tau = val;
-        else if (t == 1) s_beta = val;
-        else xs[rk + t - 1] = val;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int t = t0 + q * BLOCK;
+          if (t >= len + 1) continue;
+          if (w[q].y != ctag) w[q] = wait_chunk(p, rec + t, ctag);
+          const double val = __longlong_as_double((long long)w[q].x);
+          if (t == 0) s_tau = val;
+          else if (t == 1) s_beta = val;
+          else xs[rk + t - 1] = val;
+        }
       }
     }
     __syncthreads();
@@ -593,8 +633,10 @@ hipError_t launch_rpt(const QrcpDev& d, hipStream_t st) {
 #define LRA_RPT_512(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44)
 #define LRA_RPT_1024(X) X(4) X(8)
 
+#define LRA_RPT_512C(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40)
+
 // Single-block shapes (512 and 1024 threads, or 256 with G == 1) run without the exchange;
-// cooperative grids use the 256-thread variant with it.
+// cooperative grids use 256- or 512-thread blocks with it.
 hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st) {
   const bool coop = d.G > 1;
   if (block == 1024 && !coop) {
@@ -609,6 +651,12 @@ hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st) {
       LRA_RPT_512(CASE)
 #undef CASE
     }
+  } else if (block == 512 && coop) {
+    switch (rpt) {
+#define CASE(R) case R: return launch_rpt<R, 512, true>(d, st);
+      LRA_RPT_512C(CASE)
+#undef CASE
+    }
   } else if (block == 256) {
     switch (rpt) {
 #define CASE(R) case R: return coop ? launch_rpt<R, 256, true>(d, st) : launch_rpt<R, 256, false>(d, st);
@@ -619,13 +667,21 @@ hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
-// co-resident 256-thread blocks per CU of the cooperative variant
-int occupancy_coop(int rpt, int* nb) {
+// co-resident blocks per CU of the cooperative variant
+int occupancy_coop(int rpt, int block, int* nb) {
   hipError_t e = hipErrorInvalidValue;
-  switch (rpt) {
+  if (block == 256) {
+    switch (rpt) {
 #define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 256, true>, 256, 0); break;
-    LRA_RPT_256(CASE)
+      LRA_RPT_256(CASE)
 #undef CASE
+    }
+  } else if (block == 512) {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_kernel<R, 512, true>, 512, 0); break;
+      LRA_RPT_512C(CASE)
+#undef CASE
+    }
   }
   return e == hipSuccess ? 0 : 1;
 }
@@ -637,6 +693,17 @@ int occupancy_coop(int rpt, int* nb) {
 struct QrcpShape {
   int tpc, rpt, block, G, cpw;
 };
+
+// Estimated microseconds per pivot step of a cooperative shape, fitted to measured steps on
+// MI355X (n = 8192, ell = 40..520): a fixed exchange + barrier cost, plus the serial work of
+// a lane, which grows with the rows it holds (two waves per SIMD with 512-thread blocks hide
+// part of it), plus a penalty for grids too small to spread the columns thin.
+static double coop_step_cost(int rpt, int block, int G) {
+  const double rows = rpt > 12 ? rpt - 12 : 0;
+  double c = block == 256 ? 6.3 + 0.10 * rows : 7.2 + 0.085 * rows;
+  for (int g = G; g < 256; g *= 2) c += 0.5;
+  return c;
+}
 
 static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
   for (int tpc = 1; tpc <= 64; tpc <<= 1) {
@@ -652,21 +719,36 @@ static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
       return true;
     }
   }
-  for (int tpc = 1; tpc <= 64; tpc <<= 1) {
-    int rpt = (m + tpc - 1) / tpc;
-    rpt = (rpt + 3) / 4 * 4;
-    if (rpt < 4) rpt = 4;
-    if (rpt > 80) continue;
-    const int cpw = 256 / tpc;
-    const int G = (n + cpw - 1) / cpw;
-    int per_cu = 0;
-    if (occupancy_coop(rpt, &per_cu) != 0 || per_cu < 1) continue;
-    if (G == 1 || G <= per_cu * ctx->num_cu) {
-      *out = {tpc, rpt, 256, G, cpw};
-      return true;
+  // LRA_QRCP_SHAPE=block:tpc forces a cooperative shape (when it fits)
+  int fblock = 0, ftpc = 0;
+  if (const char* e = getenv("LRA_QRCP_SHAPE")) {
+    if (sscanf(e, "%d:%d", &fblock, &ftpc) != 2) fblock = ftpc = 0;
+  }
+  bool found = false;
+  double best = 1e30;
+  for (int block : {256, 512}) {
+    const int rpt_max = block == 256 ? 80 : 40;
+    for (int tpc = 1; tpc <= 64; tpc <<= 1) {
+      if (fblock && (block != fblock || tpc != ftpc)) continue;
+      int rpt = (m + tpc - 1) / tpc;
+      rpt = (rpt + 3) / 4 * 4;
+      if (rpt < 4) rpt = 4;
+      if (rpt > rpt_max) continue;
+      const int cpw = block / tpc;
+      const int G = (n + cpw - 1) / cpw;
+      if (G > 256) continue;  // at most one block per CU is co-resident with these registers
+      int per_cu = 0;
+      if (occupancy_coop(rpt, block, &per_cu) != 0 || per_cu < 1) continue;
+      if (G > 1 && G > per_cu * ctx->num_cu) continue;
+      const double cost = coop_step_cost(rpt, block, G);
+      if (!found || cost < best) {
+        best = cost;
+        *out = {tpc, rpt, block, G, cpw};
+        found = true;
+      }
     }
   }
-  return false;
+  return found;
 }
 
 int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
@@ -702,8 +784,8 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   // tau, beta and reflector rows up to the column group's padded height tpc*rpt
   d.cand_stride = ((sh.tpc * sh.rpt + 2 + 7) / 8) * 8;
   d.err = ctx->d_err;
-  if (sh.G > 256) {
-    lra_set_error("qrcp: %d blocks exceed the 256-slot exchange", sh.G);
+  if (sh.G > sh.block) {
+    lra_set_error("qrcp: %d blocks exceed the %d-slot exchange", sh.G, sh.block);
     return LRA_E_UNSUPPORTED;
   }
   // exchange area: [header 256 B: abort flag][slots 2G x 16 B][records 2G x stride x 16 B]
