@@ -8,12 +8,13 @@
 // lic*RPT+RPT-1, so every per-row LDS offset is an immediate), so a pivot step never
 // touches HBM.  One step s (pivot row rk):
 //   A  each workgroup finds its best remaining column (largest norm, lowest position --
-//      idamax's first-index rule) with DPP wave reductions and one barrier;
+//      idamax's first-index rule) with DPP wave reductions and one barrier, and publishes
+//      {norm, position} in its slot;
 //   B  that column finishes the previous step's update and builds its own Householder
 //      reflector (dlarfg) in registers, then publishes {tau, beta, v} as tagged chunks;
-//   C  wave 0 publishes {norm, position} in its slot and polls all slots while the other
-//      columns finish the previous step's rank-1 update (the update is lazy: the norm
-//      downdate only needs v'a_j and the new row rk, so the axpy hides in the exchange);
+//   C  every wave polls 64 of the slots while the other columns finish the previous step's
+//      rank-1 update (the update is lazy: the norm downdate only needs v'a_j and the new
+//      row rk, so the axpy hides in the exchange);
 //   D  the block structure of geqp3_adap_main! is emulated so the rank-termination test
 //      runs at the same points: blocks of min(nb, k-j+1) columns that end early at the
 //      first step whose norm downdate needed a recompute (dlaqps' lsticc), and after each
@@ -265,12 +266,22 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
       cflag |= s_wflag[w];
     }
     myflag = 0;
+    const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
+    const size_t sl = (size_t)par * p.G + g;
+    // publish {norm, position} at once: the reflector record follows (its chunk tags let a
+    // reader wait for it), so building it is off the exchange's critical path
+    if (COOP && !done_steps && tid == 0) {
+      if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      u64x2 w;
+      w.x = (uint64_t)__double_as_longlong(cv);
+      w.y = pack_tag(s + 1, cp, clc, cflag);
+      put_chunk(reinterpret_cast<u64x2*>(p.slots) + sl, w, p.uncached);
+      asm volatile("" ::: "memory");
+    }
 
     // ---------------- B: the candidate column builds its reflector (dlarfg) -------------
     // The candidate first finishes the previous step's update.  G > 1: the reflector is
     // published as a record; G == 1: it goes straight to s_x[par].
-    const uint64_t ctag = ((uint64_t)p.gen << 32) | (uint64_t)(s + 1);
-    const size_t sl = (size_t)par * p.G + g;
     if (!done_steps && clc >= 0 && wave == (clc >> lgpw)) {
       const bool cand = have && lc == clc;
       if (cand && pending) {
@@ -350,14 +361,6 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
     // Wave w polls slots [64w, 64w+64) and reduces them; the wave results meet in LDS.
     if (COOP && !done_steps) {
       volatile u64x2* slots = reinterpret_cast<volatile u64x2*>(p.slots);
-      if (tid == 0) {
-        if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        u64x2 w;
-        w.x = (uint64_t)__double_as_longlong(cv);
-        w.y = pack_tag(s + 1, cp, clc, cflag);
-        put_chunk(reinterpret_cast<u64x2*>(p.slots) + sl, w, p.uncached);
-        asm volatile("" ::: "memory");
-      }
       if (pending) {
         const double* vo = s_x[par ^ 1] + rbase;
 #pragma unroll

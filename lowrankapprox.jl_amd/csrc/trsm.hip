@@ -36,11 +36,108 @@ __global__ __launch_bounds__(256) void id_T_kernel(i64 k, i64 nr, const double* 
     for (i64 r = g; r < k; r += ng) T[r + (c0 + c) * ldt] = x[r];
 }
 
+// Blocked back substitution on MFMA: one workgroup per slab of NC right-hand sides, the
+// (k rounded up to 16) x NC slab in LDS (leading dimension kp+1 keeps the 16 lanes of an
+// MFMA operand on distinct banks).  From the bottom, per block of 16 rows: wave 0 solves
+// the 16 x 16 diagonal triangle lane-per-column (dtrsm's division and column-oriented
+// update order inside the block), then all waves subtract R(0:r0, block) * X(block) from the
+// rows above with v_mfma_f64_16x16x4f64.  Rows beyond k are padding with R = I and zero
+// right-hand sides, so they solve to zero and never touch the real rows.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int NC>
+__global__ __launch_bounds__(256) void id_T_mfma_kernel(i64 k, i64 nr, const double* __restrict__ R,
+                                                        i64 ldr, double* __restrict__ T, i64 ldt) {
+  extern __shared__ double slab[];  // (kp+1) x NC, column-major
+  __shared__ double s_rbb[16][17];
+  const int kp = (int)((k + 15) / 16) * 16;
+  const int ld = kp + 1;
+  const i64 c0 = (i64)blockIdx.x * NC;
+  const int ncols = (int)std::min<i64>(NC, nr - c0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // load the slab (padding rows and missing columns are zero)
+  for (int e = tid; e < kp * NC; e += 256) {
+    const int r = e % kp, c = e / kp;
+    slab[r + c * ld] = (r < k && c < ncols) ? R[r + (k + c0 + c) * ldr] : 0.0;
+  }
+  for (int r0 = kp - 16; r0 >= 0; r0 -= 16) {
+    {  // stage the diagonal block R(r0:r0+16, r0:r0+16) (identity in the padding)
+      const int i = tid & 15, j = tid >> 4;
+      const i64 gi = r0 + i, gj = r0 + j;
+      s_rbb[i][j] = (gi < k && gj < k) ? R[gi + gj * ldr] : (gi == gj ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (wave == 0 && lane < NC) {
+      double* x = slab + r0 + lane * ld;
+      for (int i = 15; i >= 0; --i) {
+        const double xi = x[i] / s_rbb[i][i];
+        x[i] = xi;
+        for (int r = 0; r < i; ++r) x[r] -= s_rbb[r][i] * xi;
+      }
+    }
+    __syncthreads();
+    // S(0:r0, :) -= R(0:r0, r0:r0+16) * X(r0:r0+16, :)
+    const int ntiles = r0 / 16;
+    for (int tt = wave; tt < ntiles * (NC / 16); tt += 4) {
+      const int t = tt / (NC / 16), ct = tt % (NC / 16);
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const i64 gi = (i64)t * 16 + (lane & 15), gj = r0 + kk + (lane >> 4);
+        const double a = gj < k ? R[gi + gj * ldr] : 0.0;  // gi < r0 <= gj: never diagonal
+        const double b = slab[(r0 + kk + (lane >> 4)) + (ct * 16 + (lane & 15)) * ld];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* d = slab + (t * 16 + (lane >> 4) + 4 * r) + (ct * 16 + (lane & 15)) * ld;
+        *d -= acc[r];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < (int)k * NC; e += 256) {
+    const int r = e % (int)k, c = e / (int)k;
+    if (c < ncols) T[r + (c0 + c) * ldt] = slab[r + c * ld];
+  }
+}
+
 }  // namespace
 
 int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt) {
   const i64 nr = n - k;
   if (k <= 0 || nr <= 0) return LRA_OK;
+  {
+    // MFMA blocked solve when a slab of 32 (or 16) right-hand sides fits in LDS
+    const i64 kp = (k + 15) / 16 * 16;
+    const size_t cap = 150 * 1024;
+    for (int nc : {32, 16}) {
+      const size_t bytes = (size_t)(kp + 1) * nc * sizeof(double);
+      if (bytes > cap) continue;
+      const i64 blocks = (nr + nc - 1) / nc;
+      static bool attr_set = false;
+      if (!attr_set) {  // dynamic LDS beyond 64 KiB
+        (void)hipFuncSetAttribute((const void*)id_T_mfma_kernel<32>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+        (void)hipFuncSetAttribute((const void*)id_T_mfma_kernel<16>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+        (void)hipGetLastError();
+        attr_set = true;
+      }
+      if (nc == 32)
+        hipLaunchKernelGGL(id_T_mfma_kernel<32>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
+                           nr, R, ldr, T, ldt);
+      else
+        hipLaunchKernelGGL(id_T_mfma_kernel<16>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
+                           nr, R, ldr, T, ldt);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        lra_set_error("id_T launch: %s", hipGetErrorString(e));
+        return LRA_E_HIP;
+      }
+      return LRA_OK;
+    }
+  }
   const size_t lds_cap = 64 * 1024;  // dynamic LDS budget per block
   int nc = 32;  // power of two <= 32 (divides the 256 threads)
   while (nc > 1 && (size_t)k * nc * sizeof(double) > lds_cap) nc >>= 1;

@@ -147,7 +147,8 @@ def test_laqps_block_vs_oracle(gpu, orc, m, n, offset, nb):
     assert np.allclose(host(v1)[kb_o:], vn1_o[kb_o:], rtol=1e-7)
 
 
-@pytest.mark.parametrize("k,n", [(26, 1024), (426, 8192), (1, 5), (300, 4096), (64, 65)])
+@pytest.mark.parametrize("k,n", [(26, 1024), (426, 8192), (1, 5), (300, 4096), (64, 65),
+                                 (15, 48), (17, 50), (16, 16 + 33), (436, 8192), (1100, 1500)])
 def test_id_T_vs_scipy(gpu, k, n):
     rng = np.random.default_rng(k + n)
     R = np.triu(rng.standard_normal((k, n)))
