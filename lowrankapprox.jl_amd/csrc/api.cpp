@@ -235,15 +235,17 @@ static int sketch_gaussian_impl(lra_ctx* c, int side, int trans, i64 m, i64 n, c
   g.alpha = 1.0; g.beta = 0.0; g.C = B; g.ldc = ldb; g.bgather = nullptr; g.prec = prec;
   if (side == LRA_SIDE_LEFT) {
     // B (ell x q) = Omega (ell x p) * op(A)
-    const double* om = omega;
-    if (!om) {
-      double* w = nullptr;
-      LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)ell * p, &w));
-      LRA_CHECK(lra_launch_gaussian_fill(c->stream, seed, stream_id, ell, p, w, ell));
-      om = w;
-    }
     g.M = ell; g.N = q; g.K = p;
-    g.A = om; g.lda = ell; g.transA = 0;
+    if (omega) {
+      g.A = omega; g.lda = ell; g.transA = 0;
+    } else {
+      // generated Omega is stored transposed (p x ell): both GEMM operands k-contiguous
+      double* w = nullptr;
+      const i64 ldw = (p + 1) / 2 * 2;  // even: 16-byte aligned k pairs
+      LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)ldw * ell, &w));
+      LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, seed, stream_id, ell, p, w, ldw));
+      g.A = w; g.lda = ldw; g.transA = 1;
+    }
     g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 0 : 1;
   } else {
     // B (p x ell) = op(A) (p x q) * Omega (q x ell)
@@ -615,12 +617,11 @@ extern "C" int lra_idfact_op(lra_ctx* c, int trans, int64_t m, int64_t n, lra_ap
   const i64 P = trans == LRA_TRANS_N ? m : n;  // rows of op(L)
   const int tr = trans == LRA_TRANS_N ? 1 : 0; // op(L)' = L' for :n, L for :c
   SketchFn make = [&](i64 ell, uint64_t stream, int, double* B) -> int {
-    double *Om = nullptr, *X = nullptr, *Y = nullptr;
-    LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)std::max<i64>(ell * P, 1), &Om));
+    double *X = nullptr, *Y = nullptr;
     LRA_CHECK(lra_wsT(c, WS_OP0, (size_t)std::max<i64>(P * ell, 1), &X));
     LRA_CHECK(lra_wsT(c, WS_OP1, (size_t)std::max<i64>(N * ell, 1), &Y));
-    LRA_CHECK(lra_launch_gaussian_fill(c->stream, o.seed, stream, ell, P, Om, ell));
-    LRA_CHECK(lra_launch_transpose(c->stream, ell, P, Om, ell, X, std::max<i64>(P, 1)));
+    // X = Omega' (P x ell), Omega the ell x P Philox matrix of this round
+    LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, o.seed, stream, ell, P, X, std::max<i64>(P, 1)));
     const int st = apply(user, tr, ell, X, std::max<i64>(P, 1), Y, std::max<i64>(N, 1),
                          (void*)c->stream);
     if (st != 0) {

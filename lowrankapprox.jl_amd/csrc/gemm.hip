@@ -16,6 +16,8 @@
 #include "lra_internal.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <type_traits>
 #include <mutex>
@@ -248,6 +250,143 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
     }
 }
 
+// FP64 GEMM for the case where both operands are k-contiguous in memory (op(A) = A' with A
+// stored K x M, op(B) = B stored K x N, no gather): the sketch B = Omega * A.  Every thread
+// moves 16-byte pairs (two consecutive k) from HBM into registers for the next k-tile while
+// the MFMAs consume the current one from a second LDS buffer, so each k-tile costs one
+// barrier.  LDS rows hold BK = 16 doubles padded to 17, which keeps the 16 lanes of an MFMA
+// operand read on distinct banks.
+template <int BM, int BN, int WAVES_M, int WAVES_N>
+__global__ __launch_bounds__(256) void gemm_kc_f64_kernel(GemmDev p) {
+  constexpr int BK = 16, LDK = 17;
+  constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int CA = BM * BK / 2, CB = BN * BK / 2;  // 16-byte chunks per tile
+  constexpr int LA = (CA + 255) / 256, LB = (CB + 255) / 256;
+  static_assert(WAVES_M * WAVES_N == 4, "4 waves");
+  __shared__ double sA[2][BM * LDK];
+  __shared__ double sB[2][BN * LDK];
+
+  const int b = blockIdx.x;
+  const int xcd = b & 7, idx = b >> 3;
+  const int grp = (idx / p.MT) * 8 + xcd;
+  const int mt = idx % p.MT;
+  if (grp >= p.NG) return;
+  const int nt = grp % p.NT, split = grp / p.NT;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int kbeg = split * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WAVES_N, wn = wave % WAVES_N;
+
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  d2 ra[LA], rb[LB];
+  // chunk e of a tile -> (row e / 8, k pair e % 8)
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      const int r = e >> 3, c = (e & 7) * 2;
+      const int gm = m0 + r, gk = k0 + c;
+      d2 v = {0.0, 0.0};
+      if (e < CA && gm < p.M) {
+        const double* src = p.A + gk + (i64)gm * p.lda;
+        if (gk + 1 < kend) v = *reinterpret_cast<const d2*>(src);
+        else if (gk < kend) v.x = src[0];
+      }
+      ra[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      const int r = e >> 3, c = (e & 7) * 2;
+      const int gn = n0 + r, gk = k0 + c;
+      d2 v = {0.0, 0.0};
+      if (e < CB && gn < p.N) {
+        const double* src = p.B + gk + (i64)gn * p.ldb;
+        if (gk + 1 < kend) v = *reinterpret_cast<const d2*>(src);
+        else if (gk < kend) v.x = src[0];
+      }
+      rb[t] = v;
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      if (e < CA) {
+        const int o = (e >> 3) * LDK + (e & 7) * 2;
+        sA[buf][o] = ra[t].x;
+        sA[buf][o + 1] = ra[t].y;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      if (e < CB) {
+        const int o = (e >> 3) * LDK + (e & 7) * 2;
+        sB[buf][o] = rb[t].x;
+        sB[buf][o + 1] = rb[t].y;
+      }
+    }
+  };
+
+  d4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int am0 = wm * WTM, bn0 = wn * WTN;
+
+  // (the pointer alignment of the 16-byte loads: lda, ldb, kchunk and k0 are even)
+  if (kbeg < kend) {
+    load_tiles(kbeg);
+    store_tiles(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) load_tiles(k0 + BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        double af[TM], bfv[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) af[i] = sA[cur][(am0 + i * 16 + l16) * LDK + kk + lq];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bfv[j] = sB[cur][(bn0 + j * 16 + l16) * LDK + kk + lq];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bfv[j], acc[i][j], 0, 0, 0);
+      }
+      if (more) store_tiles(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + bn0 + j * 16 + l16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + am0 + i * 16 + lq + 4 * r;
+        if (row < p.M && col < p.N) {
+          const double v = acc[i][j][r];
+          if (p.splits > 1) {
+            p.partial[(i64)split * p.M * p.N + row + (i64)col * p.M] = v;
+          } else {
+            double* c = p.C + row + (i64)col * p.ldc;
+            *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+          }
+        }
+      }
+    }
+}
+
 __global__ void splitk_reduce_kernel(int M, int N, int splits, const double* part, double alpha,
                                      double beta, double* C, i64 ldc) {
   const i64 total = (i64)M * N;
@@ -264,7 +403,17 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, const double* par
 // ---- configurations -------------------------------------------------------------------
 struct Cfg { int id, BM, BN; };
 // id -> (BM, BN, WAVES_M, WAVES_N)
-static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 256}, {4, 128, 128}};
+static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 256}, {4, 128, 128},
+                            {5, 16, 256}, {6, 32, 256}, {7, 48, 256}, {8, 64, 256}, {9, 128, 128},
+                            {10, 64, 128}};
+constexpr int kNumCfg = 11;
+// ids 5..10: the k-contiguous FP64 kernel (gemm_kc_f64_kernel)
+bool kc_cfg(int id) { return id >= 5; }
+bool kc_ok(const GemmArgs& g) {
+  return g.prec == LRA_PREC_F64 && g.transA == 1 && g.transB == 0 && g.bgather == nullptr &&
+         (g.lda % 2) == 0 && (g.ldb % 2) == 0 && ((uintptr_t)g.A % 16) == 0 &&
+         ((uintptr_t)g.B % 16) == 0;
+}
 
 template <int PREC>
 hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
@@ -274,12 +423,19 @@ hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
     case 2: hipLaunchKernelGGL((gemm_kernel<PREC, 48, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
     case 3: hipLaunchKernelGGL((gemm_kernel<PREC, 64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
     case 4: hipLaunchKernelGGL((gemm_kernel<PREC, 128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 5: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<16, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 6: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<32, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 7: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<48, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 8: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 9: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 10: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_prec(int prec, int id, const GemmDev& d, int blocks, hipStream_t st) {
+  if (kc_cfg(id) && prec != LRA_PREC_F64) return hipErrorInvalidValue;
   switch (prec) {
     case LRA_PREC_F64: return launch_cfg<LRA_PREC_F64>(id, d, blocks, st);
     case LRA_PREC_F32: return launch_cfg<LRA_PREC_F32>(id, d, blocks, st);
@@ -368,11 +524,13 @@ int run(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits) {
 }
 
 void candidates(const GemmArgs& g, int num_cu, std::vector<TuneVal>* out) {
-  for (int cfg = 0; cfg < 5; ++cfg) {
+  const bool kc = kc_ok(g);
+  for (int cfg = 0; cfg < kNumCfg; ++cfg) {
     const Cfg& c = kCfgs[cfg];
+    if (kc_cfg(cfg) && !kc) continue;
     // skip configs that waste most of the M tile
     if (c.BM > 16 && g.M <= c.BM / 2) continue;
-    if (cfg == 4 && g.M < 96) continue;
+    if ((cfg == 4 || cfg == 9) && g.M < 96) continue;
     const i64 tiles = ((g.M + c.BM - 1) / c.BM) * ((g.N + c.BN - 1) / c.BN);
     for (int s = 1; s <= 32; s *= 2) {
       if (s > 1 && tiles * s > 8ll * num_cu) break;
@@ -407,6 +565,13 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
   if (g_autotune < 0) {
     const char* env = getenv("LRA_AUTOTUNE");
     g_autotune = (env && env[0] == '0') ? 0 : 1;
+  }
+  // LRA_GEMM_CFG=id:splits forces a configuration (when it applies to the operands)
+  if (const char* f = getenv("LRA_GEMM_CFG")) {
+    int id = -1, sp = 1;
+    if (sscanf(f, "%d:%d", &id, &sp) == 2 && id >= 0 && id < kNumCfg && sp >= 1 &&
+        (!kc_cfg(id) || kc_ok(g)))
+      return run(ctx, g, id, sp);
   }
   TuneKey key{g.prec, g.transA, g.transB, bucket(g.M), bucket(g.N), bucket(g.K)};
   TuneVal choice;

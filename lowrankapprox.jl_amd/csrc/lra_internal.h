@@ -197,6 +197,9 @@ __device__ __forceinline__ bool lra_grid_barrier(unsigned* counter, unsigned tar
 // rng.hip
 int lra_launch_gaussian_fill(hipStream_t st, uint64_t seed, uint64_t stream_id, i64 rows,
                              i64 cols, double* out, i64 ld);
+// the rows x cols Philox matrix stored transposed (cols x rows, leading dimension ld)
+int lra_launch_gaussian_fill_t(hipStream_t st, uint64_t seed, uint64_t stream_id, i64 rows,
+                               i64 cols, double* out, i64 ld);
 int lra_launch_gaussian_fill_f32(hipStream_t st, uint64_t seed, uint64_t stream_id, i64 rows,
                                  i64 cols, float* out, i64 ld);
 int lra_launch_srft_rand(hipStream_t st, uint64_t seed, uint64_t stream_id, i64 m, i64 ell,

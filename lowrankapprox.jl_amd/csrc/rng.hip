@@ -22,6 +22,22 @@ __global__ void gaussian_fill_kernel(uint64_t seed, uint64_t stream, i64 rows, i
   }
 }
 
+// The same matrix stored transposed: out[j + i*ld] = element (i, j) = normal #(j*rows + i).
+// One thread per output element (consecutive threads write consecutive addresses); each
+// evaluates the pair that holds its normal.
+__global__ void gaussian_fill_t_kernel(uint64_t seed, uint64_t stream, i64 rows, i64 cols,
+                                       double* out, i64 ld) {
+  const i64 total = rows * cols;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < total;
+       t += (i64)gridDim.x * blockDim.x) {
+    const i64 j = t % cols, i = t / cols;
+    const i64 e = j * rows + i;
+    double z0, z1;
+    lra_normal2(seed, stream, (uint64_t)(e >> 1), &z0, &z1);
+    out[j + i * ld] = (e & 1) ? z1 : z0;
+  }
+}
+
 __global__ void gaussian_fill_f32_kernel(uint64_t seed, uint64_t stream, i64 rows, i64 cols,
                                          float* out, i64 ld) {
   const i64 total = rows * cols;
@@ -128,6 +144,15 @@ int lra_launch_gaussian_fill(hipStream_t st, uint64_t seed, uint64_t stream_id, 
   if (rows <= 0 || cols <= 0) return LRA_OK;
   hipLaunchKernelGGL(gaussian_fill_kernel, dim3(grid_for((rows * cols + 1) / 2)), dim3(256),
                      0, st, seed, stream_id, rows, cols, out, ld);
+  LRA_LAUNCH_CHECK();
+  return LRA_OK;
+}
+
+int lra_launch_gaussian_fill_t(hipStream_t st, uint64_t seed, uint64_t stream_id, i64 rows,
+                               i64 cols, double* out, i64 ld) {
+  if (rows <= 0 || cols <= 0) return LRA_OK;
+  hipLaunchKernelGGL(gaussian_fill_t_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, st, seed,
+                     stream_id, rows, cols, out, ld);
   LRA_LAUNCH_CHECK();
   return LRA_OK;
 }

@@ -41,6 +41,22 @@ def test_sketch_gemm_f64_variants(gpu, side, trans):
     assert np.abs(B - ref).max() < 1e-11
 
 
+@pytest.mark.parametrize("cfg", ["5:1", "6:1", "7:2", "8:1", "9:1", "10:4", "0:1", "4:2"])
+@pytest.mark.parametrize("M,N,K", [(40, 1000, 1031), (136, 513, 800), (7, 300, 2)])
+def test_sketch_gemm_forced_configs(gpu, monkeypatch, cfg, M, N, K):
+    """Every GEMM configuration (the k-contiguous FP64 kernel 5..10 and the general one 0..4),
+    forced through LRA_GEMM_CFG, on a generated-Omega sketch with ragged M, N, K (odd K
+    exercises the single-element tail of the 16-byte loads)."""
+    rng = np.random.default_rng(M * N + K)
+    A = np.asfortranarray(rng.standard_normal((K, N)))
+    monkeypatch.setenv("LRA_GEMM_CFG", cfg)
+    print(f"forced gemm cfg={cfg} M={M} N={N} K={K}", flush=True)
+    B = host(gpu.sketch("left", "n", A, M, seed=9, stream=3))
+    Om = host(gpu.gaussian(M, K, 9, 3))
+    ref = Om @ A
+    assert np.abs(B - ref).max() <= 1e-12 * np.abs(Om).max() * np.abs(A).max() * K
+
+
 @pytest.mark.parametrize("prec,tol", [("f32", 2e-5), ("bf16", 3e-2), ("bf16x3", 2e-5)])
 def test_sketch_gemm_lowprec(gpu, prec, tol):
     rng = np.random.default_rng(11)
