@@ -192,6 +192,17 @@ int lra_snormdiff_id(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_
 int lra_snorm_op(lra_ctx* ctx, int64_t m, int64_t n, lra_apply_fn apply, void* user,
                  int isherm, const lra_opts* opts, const double* x0, double* result);
 
+/* Householder QR of a tall m x n matrix (m >= n) in place, LAPACK geqrf layout (R on and
+ * above the diagonal, reflectors below, tau[n]) -- the qr! of pqrfact/psvdfact/pheigfact
+ * (src/pqr.jl:298, src/psvd.jl:244,258, src/pheig.jl:282).  Reflectors follow dlarfg, so the
+ * signs of R's diagonal are dgeqrf's. */
+int lra_geqrf_f64(lra_ctx* ctx, int64_t m, int64_t n, double* A, int64_t lda, double* tau);
+
+/* Q (m x k, ldq) = first k columns of H_1...H_k from lra_geqrf_f64's output (LAPACK.orgqr!,
+ * src/pqr.jl:427). */
+int lra_orgqr_f64(lra_ctx* ctx, int64_t m, int64_t k, const double* A, int64_t lda,
+                  const double* tau, double* Q, int64_t ldq);
+
 /* ---- whole-pipeline entry points (mirror the reference API) -------------------------- */
 
 /* idfact(trans, A, opts) (src/id.jl:434-447): sketchfact(:left, trans, A) (adaptive loop,

@@ -24,7 +24,8 @@ from ._lib import LraError, check
 
 __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "geqp3_adap",
            "laqps", "id_T", "maxdet_swapcols", "gaussian", "to_device", "LraError", "context",
-           "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff"]
+           "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
+           "orgqr", "qr"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -241,6 +242,40 @@ def laqps(A, offset: int, nb: int, jpvt, vn1, vn2):
                                     _ptr(jpvt), _ptr(tau), _ptr(vn1), _ptr(vn2), None, None,
                                     max(1, n)), "lra_laqps_f64")
     return tau[: int(kb.value)], int(kb.value)
+
+
+def geqrf(A):
+    """Householder QR in place (LAPACK geqrf layout); returns tau.  A: column-major float64
+    CUDA tensor, m >= n."""
+    torch = _torch()
+    ctx = context()
+    m, n = A.shape
+    tau = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+    check(_lib.load().lra_geqrf_f64(ctx.h, m, n, _ptr(A), _ld(A), _ptr(tau)), "lra_geqrf_f64")
+    return tau[:n]
+
+
+def orgqr(A, tau, k: int = None):
+    """Thin Q (m x k) from a geqrf factorization."""
+    ctx = context()
+    m = A.shape[0]
+    k = tau.shape[0] if k is None else k
+    Q = empty_colmajor(m, k)
+    check(_lib.load().lra_orgqr_f64(ctx.h, m, k, _ptr(A), _ld(A), _ptr(tau), _ptr(Q),
+                                    max(1, m)), "lra_orgqr_f64")
+    return Q
+
+
+def qr(A):
+    """(Q, R) of a tall matrix (qr! of the reference's factorizations)."""
+    torch = _torch()
+    X = to_device(A).clone() if not isinstance(A, np.ndarray) else to_device(A)
+    X = X.t().contiguous().t() if X.stride(0) != 1 else X
+    m, n = X.shape
+    tau = geqrf(X)
+    Q = orgqr(X, tau, n)
+    R = torch.triu(X[:n, :])
+    return Q, R
 
 
 def id_T(R, k: int):

@@ -89,6 +89,8 @@ SIGNATURES = [
                                    C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_snorm_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
                                C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
+    ("lra_geqrf_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D]),
+    ("lra_orgqr_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, P_D, i64]),
     ("lra_idfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                              C.POINTER(lra_id_out)]),
     ("lra_idfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p,

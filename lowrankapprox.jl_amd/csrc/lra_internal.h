@@ -44,7 +44,7 @@ struct lra_ctx {
   int num_cu = 0;
   int lds_bytes = 0;
   // workspace slots: grown on demand, never shrunk until destroy
-  static const int kSlots = 32;
+  static const int kSlots = 48;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
   // small pinned host staging area for scalars read back synchronously
@@ -65,6 +65,7 @@ enum WsSlot {
   WS_OMEGA = 0, WS_SKETCH, WS_SKETCH2, WS_SPLITK, WS_JPVT, WS_TAU, WS_SCAL, WS_R,
   WS_T, WS_TMP0, WS_TMP1, WS_TMP2, WS_TMP3, WS_LP0, WS_LP1, WS_BATCH0, WS_BATCH1,
   WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
+  WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
   WS_COUNT
 };
 static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
@@ -226,6 +227,11 @@ struct GemmArgs {
   int prec;
 };
 int lra_gemm(lra_ctx* ctx, const GemmArgs& g);
+
+// qr.hip: blocked Householder QR (geqrf layout) and the explicit thin Q
+int lra_geqrf_impl(lra_ctx* ctx, i64 m, i64 n, double* A, i64 lda, double* tau);
+int lra_orgqr_impl(lra_ctx* ctx, i64 m, i64 k, const double* A, i64 lda, const double* tau,
+                   double* Q, i64 ldq);
 
 // qrcp.hip
 struct QrcpCall {

@@ -1,0 +1,38 @@
+"""GPU Householder QR (geqrf layout + explicit Q) against LAPACK (scipy) on the same input."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+pytestmark = pytest.mark.gpu
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("m,n", [(300, 40), (1000, 77), (8192, 436), (256, 256), (5, 1),
+                                 (70000, 33), (2000, 1)])
+def test_qr_vs_lapack(gpu, m, n):
+    """R matches dgeqrf's (same reflector convention, so the same diagonal signs) to rounding;
+    Q is orthonormal and Q R reproduces A."""
+    rng = np.random.default_rng(m + n)
+    A = np.asfortranarray(rng.standard_normal((m, n)) * np.logspace(0, -8, n))
+    print(f"qr m={m} n={n}", flush=True)
+    Q, R = gpu.qr(A)
+    Q, R = host(Q), host(R)
+    Rl = sla.qr(A, mode="r")[0][:n, :]
+    scale = np.abs(Rl).max(axis=0)  # column scales
+    assert np.all(np.abs(R - Rl) <= 1e-10 * scale[None, :] + 1e-13 * np.abs(Rl).max())
+    assert np.abs(Q.T @ Q - np.eye(n)).max() <= 1e-12 * max(1, np.sqrt(m) / 10)
+    assert np.abs(Q @ R - A).max() <= 1e-13 * np.abs(A).max() * np.sqrt(m)
+
+
+def test_qr_rank_deficient_column(gpu):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((500, 20))
+    A[:, 7] = 0.0
+    A = np.asfortranarray(A)
+    Q, R = gpu.qr(A)
+    Q, R = host(Q), host(R)
+    assert np.abs(Q @ R - A).max() <= 1e-12
+    assert np.abs(Q.T @ Q - np.eye(20)).max() <= 1e-12
