@@ -88,6 +88,42 @@ typedef struct lra_id_out {
   int64_t rounds; /* host: adaptive rounds used */
 } lra_id_out;
 
+/* Partial QR (PQRFactors / PartialQR, src/pqr.jl:25-40, 86-100): op(A)[:, p] ~ Q R with
+ * Q (P x k), R = [R11, R11 T] (k x N), p (N, 1-based) and T (k x (N-k)).  Buffers left NULL
+ * are library-owned (valid until the next call on the context) and filled in. */
+typedef struct lra_pqr_out {
+  double* Q;
+  int64_t ldq;
+  double* R;
+  int64_t ldr;
+  int64_t* p;
+  double* T;
+  int64_t ldt;
+  int64_t k;      /* host: rank */
+  int64_t rounds; /* host: adaptive rounds of the ID */
+} lra_pqr_out;
+
+/* Partial SVD (PartialSVD, src/psvd.jl:16-24): A ~ U diag(S) Vt with U (m x k), S (k),
+ * Vt (k x n).  NULL buffers are library-owned. */
+typedef struct lra_psvd_out {
+  double* U;
+  int64_t ldu;
+  double* S;
+  double* Vt;
+  int64_t ldvt;
+  int64_t k; /* host */
+} lra_psvd_out;
+
+/* Partial Hermitian eigendecomposition (PartialHermEigen, src/pheig.jl:16-22): A ~ V diag(w) V'
+ * with w ascending (kn negative, then kp positive), V (n x k).  NULL buffers are
+ * library-owned. */
+typedef struct lra_pheig_out {
+  double* values;
+  double* vectors;
+  int64_t ldv;
+  int64_t k, kn, kp; /* host */
+} lra_pheig_out;
+
 /* Matrix-free operator (LinearOperator mul!/mulc!, src/linop.jl:9-40,105-113):
  * Y (rows x nrhs, ldy) = op(L) * X (cols x nrhs, ldx) with op = L (trans 0) or L' (trans 1),
  * device buffers, work enqueued on `stream` (a hipStream_t). Return 0 on success. */
@@ -233,6 +269,32 @@ int lra_sketchfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* 
  * Philox streams 2^32 + round. */
 int lra_curfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
                 const lra_opts* opts, int64_t* rows, int64_t* cols, int64_t* k);
+
+/* pqrfact(trans, A, opts) (src/pqr.jl:285-320): the ID of op(A), then the thin QR of its
+ * skeleton columns; pqrfact_retval bits choose Q, R, T.  sketch = none is the rank-adaptive
+ * QRCP of op(A) itself (pqrfact_none, :323-327). */
+int lra_pqrfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
+                const lra_opts* opts, lra_pqr_out* out);
+
+/* psvdfact(A, opts) / psvdvals (src/psvd.jl:238-300): the ID, the thin QR of the skeleton,
+ * the SVD of the k x n core through its QR and a one-sided Jacobi SVD, truncated by
+ * psvdrank (:302-308).  psvdvals writes the k singular values to S (device). */
+int lra_psvdfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                 const lra_opts* opts, lra_psvd_out* out);
+int lra_psvdvals(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                 const lra_opts* opts, double* S, int64_t* k);
+
+/* pheigfact(A, opts) / pheigvals for a Hermitian (symmetric) A (src/pheig.jl:276-314). */
+int lra_pheigfact(lra_ctx* ctx, int64_t n, const double* A, int64_t lda, const lra_opts* opts,
+                  lra_pheig_out* out);
+int lra_pheigvals(lra_ctx* ctx, int64_t n, const double* A, int64_t lda, const lra_opts* opts,
+                  double* values, int64_t* k);
+
+/* One-sided Jacobi SVD of X (m x n, m >= n), the core of psvdfact/pheigfact: X = U diag(s) V'
+ * with s decreasing; X is overwritten (MI355X extension; replaces svd!/eigen! on the small
+ * core, src/psvd.jl:245, src/pheig.jl:284). */
+int lra_gesvj_f64(lra_ctx* ctx, int64_t m, int64_t n, double* X, int64_t ldx, double* U,
+                  int64_t ldu, double* s, double* V, int64_t ldv);
 
 /* Batched ID of many independent blocks (MI355X extension; each block behaves as idfact
  * with a fresh Gaussian sketch of stream `b` under opts->seed): one persistent fused launch

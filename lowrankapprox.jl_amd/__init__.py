@@ -25,7 +25,8 @@ from ._lib import LraError, check
 __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "geqp3_adap",
            "laqps", "id_T", "maxdet_swapcols", "gaussian", "to_device", "LraError", "context",
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
-           "orgqr", "qr"]
+           "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
+           "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -477,6 +478,185 @@ class LinearOperator:
                                              int(self.hermitian), C.byref(copts), _ptr(x),
                                              C.byref(res)), "lra_snorm_op")
         return float(res.value)
+
+
+# ----------------------------------------------------------------------------------------
+# partial factorizations built on the ID
+
+def _own(ptr: int, rows: int, cols: int, ld: int):
+    """Copy a library-owned column-major device matrix into a new tensor."""
+    out = empty_colmajor(rows, cols)
+    if rows > 0 and cols > 0:
+        _copy_device(out.data_ptr(), int(ptr), rows, cols, ld, rows)
+    return out
+
+
+def _own_vec(ptr: int, n: int, dtype=None):
+    torch = _torch()
+    out = torch.empty(max(n, 1), dtype=torch.float64 if dtype is None else dtype, device="cuda")
+    if n > 0:
+        _copy_device(out.data_ptr(), int(ptr), n, 1, n, n)
+    return out[:n]
+
+
+@dataclasses.dataclass
+class PartialQR:
+    """PartialQR (src/pqr.jl:25-31): op(A)[:, p] ~ Q R."""
+    Q: object
+    R: object
+    p: object
+
+    @property
+    def k(self) -> int:
+        return int(self.R.shape[0])
+
+
+@dataclasses.dataclass
+class PQRFactors:
+    """PQRFactors (src/pqr.jl:33-40): Q, R, p, k and T (entries not requested are None)."""
+    Q: object
+    R: object
+    p: object
+    k: int
+    T: object
+
+
+def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
+    """pqrfact(trans, A, opts) (src/pqr.jl:285-320); PartialQR when pqrfact_retval is "qr"."""
+    torch = _torch()
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    N = n if trans == "n" else m
+    P = m if trans == "n" else n
+    out = _lib.lra_pqr_out()
+    copts = opts.to_c()
+    check(_lib.load().lra_pqrfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
+                                  C.byref(copts), C.byref(out)), "lra_pqrfact")
+    k = int(out.k)
+    rv = opts.pqrfact_retval
+    Q = _own(out.Q, P, k, int(out.ldq)) if "q" in rv and k > 0 else None
+    R = _own(out.R, k, N, int(out.ldr)) if "r" in rv and k > 0 else None
+    if "q" in rv and k == 0: Q = empty_colmajor(P, 0)
+    if "r" in rv and k == 0: R = empty_colmajor(0, N)
+    p = _own_vec(out.p, N, torch.int64)
+    T = _own(out.T, k, N - k, int(out.ldt)) if "t" in rv and k > 0 and N > k else (
+        empty_colmajor(k, N - k) if "t" in rv else None)
+    if "q" in rv and "r" in rv and "t" not in rv:
+        return PartialQR(Q, R, p)
+    return PQRFactors(Q, R, p, k, T)
+
+
+def pqr(trans, A=None, opts: LRAOptions = None, **kw):
+    """pqr (src/pqr.jl:310-315): (Q, R, p) with pqrfact_retval = "qr"."""
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    kw.setdefault("pqrfact_retval", "qr")
+    F = pqrfact(trans, A, opts, **kw)
+    return F.Q, F.R, F.p
+
+
+@dataclasses.dataclass
+class PartialSVD:
+    """PartialSVD (src/psvd.jl:16-24): A ~ U diag(S) Vt."""
+    U: object
+    S: object
+    Vt: object
+
+    @property
+    def V(self):
+        return self.Vt.t()
+
+    @property
+    def k(self) -> int:
+        return int(self.S.shape[0])
+
+
+def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
+    """psvdfact(A, opts) (src/psvd.jl:238-272)."""
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    out = _lib.lra_psvd_out()
+    copts = opts.to_c()
+    check(_lib.load().lra_psvdfact(ctx.h, m, n, _ptr(A), _ld(A), C.byref(copts), C.byref(out)),
+          "lra_psvdfact")
+    k = int(out.k)
+    return PartialSVD(_own(out.U, m, k, int(out.ldu)) if k else empty_colmajor(m, 0),
+                      _own_vec(out.S, k), _own(out.Vt, k, n, int(out.ldvt)) if k else
+                      empty_colmajor(0, n))
+
+
+def psvdvals(A, opts: LRAOptions = None, **kw):
+    """psvdvals(A, opts) (src/psvd.jl:274-290)."""
+    torch = _torch()
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    S = torch.empty(max(min(m, n), 1), dtype=torch.float64, device="cuda")
+    k = _lib.i64(0)
+    copts = opts.to_c()
+    check(_lib.load().lra_psvdvals(ctx.h, m, n, _ptr(A), _ld(A), C.byref(copts), _ptr(S),
+                                   C.byref(k)), "lra_psvdvals")
+    return S[:int(k.value)]
+
+
+def psvd(A, opts: LRAOptions = None, **kw):
+    """psvd (src/psvd.jl:296-299): (U, S, V)."""
+    F = psvdfact(A, opts, **kw)
+    return F.U, F.S, F.V
+
+
+@dataclasses.dataclass
+class PartialHermEigen:
+    """PartialHermEigen (src/pheig.jl:16-22): A ~ vectors diag(values) vectors'."""
+    values: object
+    vectors: object
+
+
+def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
+    """pheigfact(A, opts) (src/pheig.jl:276-297) for a symmetric A."""
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    n = A.shape[0]
+    if A.shape[0] != A.shape[1] or not _ishermitian_dev(A):
+        raise ValueError("matrix must be Hermitian")
+    out = _lib.lra_pheig_out()
+    copts = opts.to_c()
+    check(_lib.load().lra_pheigfact(ctx.h, n, _ptr(A), _ld(A), C.byref(copts), C.byref(out)),
+          "lra_pheigfact")
+    k = int(out.k)
+    return PartialHermEigen(_own_vec(out.values, k),
+                            _own(out.vectors, n, k, int(out.ldv)) if k else empty_colmajor(n, 0))
+
+
+def pheigvals(A, opts: LRAOptions = None, **kw):
+    """pheigvals(A, opts) (src/pheig.jl:298-314)."""
+    torch = _torch()
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    n = A.shape[0]
+    if A.shape[0] != A.shape[1] or not _ishermitian_dev(A):
+        raise ValueError("matrix must be Hermitian")
+    v = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+    k = _lib.i64(0)
+    copts = opts.to_c()
+    check(_lib.load().lra_pheigvals(ctx.h, n, _ptr(A), _ld(A), C.byref(copts), _ptr(v),
+                                    C.byref(k)), "lra_pheigvals")
+    return v[:int(k.value)]
+
+
+def pheig(A, opts: LRAOptions = None, **kw):
+    """pheig (src/pheig.jl:317-320): (values, vectors)."""
+    F = pheigfact(A, opts, **kw)
+    return F.values, F.vectors
 
 
 def sketchfact(trans, A=None, opts: "LRAOptions" = None, stream_base: int = 0, **kw):

@@ -53,6 +53,20 @@ class lra_id_out(C.Structure):
                 ("rounds", i64)]
 
 
+class lra_pqr_out(C.Structure):
+    _fields_ = [("Q", P_D), ("ldq", i64), ("R", P_D), ("ldr", i64), ("p", P_I), ("T", P_D),
+                ("ldt", i64), ("k", i64), ("rounds", i64)]
+
+
+class lra_psvd_out(C.Structure):
+    _fields_ = [("U", P_D), ("ldu", i64), ("S", P_D), ("Vt", P_D), ("ldvt", i64), ("k", i64)]
+
+
+class lra_pheig_out(C.Structure):
+    _fields_ = [("values", P_D), ("vectors", P_D), ("ldv", i64), ("k", i64), ("kn", i64),
+                ("kp", i64)]
+
+
 APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, i64, C.c_void_p, i64, C.c_void_p, i64,
                        C.c_void_p)
 
@@ -99,6 +113,17 @@ SIGNATURES = [
                                  C.c_uint64, P_I, C.POINTER(i64)]),
     ("lra_curfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_I, P_I,
                               C.POINTER(i64)]),
+    ("lra_pqrfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                              C.POINTER(lra_pqr_out)]),
+    ("lra_psvdfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                               C.POINTER(lra_psvd_out)]),
+    ("lra_psvdvals", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_D,
+                               C.POINTER(i64)]),
+    ("lra_pheigfact", C.c_int, [C.c_void_p, i64, P_D, i64, C.POINTER(lra_opts),
+                                C.POINTER(lra_pheig_out)]),
+    ("lra_pheigvals", C.c_int, [C.c_void_p, i64, P_D, i64, C.POINTER(lra_opts), P_D,
+                                C.POINTER(i64)]),
+    ("lra_gesvj_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64, P_D, P_D, i64]),
     ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
                                      C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
                                      P_I, P_I, C.POINTER(i64)]),
