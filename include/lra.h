@@ -239,6 +239,23 @@ int lra_geqrf_f64(lra_ctx* ctx, int64_t m, int64_t n, double* A, int64_t lda, do
 int lra_orgqr_f64(lra_ctx* ctx, int64_t m, int64_t k, const double* A, int64_t lda,
                   const double* tau, double* Q, int64_t ldq);
 
+/* C = alpha op(A) op(B) + beta C in FP64 on MFMA (trans 0 = N, 1 = T; BLAS argument
+ * order): the dgemm behind mul! on the dense factors (src/pqr.jl:86-175, src/psvd.jl:61-124,
+ * src/pheig.jl:54-100). */
+int lra_dgemm(lra_ctx* ctx, int transa, int transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+              double beta, double* C, int64_t ldc);
+
+/* Y = V X (trans 0: X n x nrhs, Y k x nrhs) or V' X (trans 1: X k x nrhs, Y n x nrhs) for the
+ * interpolation matrix V = [I T] P' of an ID (mul!/mulc! of IDPackedV, src/id.jl:94-200). */
+int lra_id_apply(lra_ctx* ctx, int trans, int64_t k, int64_t n, const int64_t* sk,
+                 const int64_t* rd, const double* T, int64_t ldt, int64_t nrhs,
+                 const double* X, int64_t ldx, double* Y, int64_t ldy);
+
+/* X (m x n) = diag(d) X (the diagonal factor of PartialSVD / PartialHermEigen products). */
+int lra_scale_rows(lra_ctx* ctx, int64_t m, int64_t n, const double* d, double* X,
+                   int64_t ldx);
+
 /* ---- whole-pipeline entry points (mirror the reference API) -------------------------- */
 
 /* idfact(trans, A, opts) (src/id.jl:434-447): sketchfact(:left, trans, A) (adaptive loop,

@@ -26,7 +26,7 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "laqps", "id_T", "maxdet_swapcols", "gaussian", "to_device", "LraError", "context",
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
-           "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig"]
+           "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -327,6 +327,75 @@ class IDPackedV:
         if key == "p": return torch.cat([self.sk, self.rd])
         raise KeyError(key)
 
+    @property
+    def n(self) -> int:
+        return self.k + int(self.rd.shape[0])
+
+    def apply(self, X, trans: str = "n"):
+        """V X (trans "n", X n x nrhs) or V' X (trans "c", X k x nrhs) for V = [I T] P'
+        (mul!/mulc! of IDPackedV, src/id.jl:94-200)."""
+        ctx = context()
+        X = to_device(X)
+        if X.dim() == 1:
+            X = X.reshape(-1, 1)
+        nrhs = X.shape[1]
+        rows = self.k if trans == "n" else self.n
+        Y = empty_colmajor(rows, nrhs)
+        ldt = _ld(self.T) if self.T.numel() else max(1, self.k)
+        check(_lib.load().lra_id_apply(ctx.h, 0 if trans == "n" else 1, self.k, self.n,
+                                       _ptr(self.sk), _ptr(self.rd), _ptr(self.T), ldt, nrhs,
+                                       _ptr(X), _ld(X), _ptr(Y), max(1, rows)), "lra_id_apply")
+        return Y
+
+    def matrix(self):
+        """The k x n interpolation matrix [I T] P' (Matrix(V), src/id.jl:66)."""
+        torch = _torch()
+        return self.apply(torch.eye(self.n, dtype=torch.float64, device="cuda"))
+
+
+def dgemm(A, B, transa: str = "n", transb: str = "n", alpha: float = 1.0, C=None,
+          beta: float = 0.0):
+    """alpha op(A) op(B) + beta C on the library's FP64 MFMA GEMM."""
+    ctx = context()
+    A, B = to_device(A), to_device(B)
+    ta, tb = int(transa != "n"), int(transb != "n")
+    m = A.shape[1] if ta else A.shape[0]
+    k = A.shape[0] if ta else A.shape[1]
+    n = B.shape[0] if tb else B.shape[1]
+    if C is None:
+        C = empty_colmajor(m, n)
+        beta = 0.0
+    check(_lib.load().lra_dgemm(ctx.h, ta, tb, m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B),
+                                beta, _ptr(C), _ld(C)), "lra_dgemm")
+    return C
+
+
+def _scale_rows(d, X):
+    ctx = context()
+    check(_lib.load().lra_scale_rows(ctx.h, X.shape[0], X.shape[1], _ptr(d), _ptr(X), _ld(X)),
+          "lra_scale_rows")
+    return X
+
+
+@dataclasses.dataclass
+class ID:
+    """ID (src/id.jl:212-235): A ~ C V with C = A[:, sk] (m x k) and V = [I T] P'."""
+    C: object
+    V: IDPackedV
+
+    @classmethod
+    def from_matrix(cls, A, V: IDPackedV, trans: str = "n"):
+        torch = _torch()
+        A = to_device(A)
+        X = A if trans == "n" else A.t()
+        return cls(X[:, V.sk - 1].t().contiguous().t(), V)
+
+    def apply(self, X, trans: str = "n"):
+        """A X = C (V X) (trans "n") or A' X = V' (C' X) (trans "c") (src/id.jl:236-331)."""
+        if trans == "n":
+            return dgemm(self.C, self.V.apply(X, "n"))
+        return self.V.apply(dgemm(self.C, X, transa="c"), "c")
+
 
 def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
     """idfact(trans, A, opts; kw...) (src/id.jl:434-447); idfact(A) means trans = :n."""
@@ -510,6 +579,12 @@ class PartialQR:
     def k(self) -> int:
         return int(self.R.shape[0])
 
+    def apply(self, X):
+        """op(A) X ~ Q R P' X = Q (R X[p, :]) (mul!, src/pqr.jl:86-131)."""
+        X = to_device(X)
+        Xp = X[self.p - 1, :].t().contiguous().t()
+        return dgemm(self.Q, dgemm(self.R, Xp))
+
 
 @dataclasses.dataclass
 class PQRFactors:
@@ -574,6 +649,12 @@ class PartialSVD:
     def k(self) -> int:
         return int(self.S.shape[0])
 
+    def apply(self, X, trans: str = "n"):
+        """A X = U (S (Vt X)) or A' X = V (S (U' X)) (mul!, src/psvd.jl:61-124)."""
+        if trans == "n":
+            return dgemm(self.U, _scale_rows(self.S, dgemm(self.Vt, X)))
+        return dgemm(self.Vt, _scale_rows(self.S, dgemm(self.U, X, transa="c")), transa="c")
+
 
 def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
     """psvdfact(A, opts) (src/psvd.jl:238-272)."""
@@ -617,6 +698,11 @@ class PartialHermEigen:
     """PartialHermEigen (src/pheig.jl:16-22): A ~ vectors diag(values) vectors'."""
     values: object
     vectors: object
+
+    def apply(self, X):
+        """A X = V (w (V' X)) (mul!, src/pheig.jl:54-100)."""
+        return dgemm(self.vectors, _scale_rows(self.values, dgemm(self.vectors, X,
+                                                                   transa="c")))
 
 
 def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:

@@ -124,6 +124,11 @@ SIGNATURES = [
     ("lra_pheigvals", C.c_int, [C.c_void_p, i64, P_D, i64, C.POINTER(lra_opts), P_D,
                                 C.POINTER(i64)]),
     ("lra_gesvj_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64, P_D, P_D, i64]),
+    ("lra_dgemm", C.c_int, [C.c_void_p, C.c_int, C.c_int, i64, i64, i64, C.c_double, P_D, i64,
+                            P_D, i64, C.c_double, P_D, i64]),
+    ("lra_id_apply", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_I, P_I, P_D, i64, i64, P_D, i64,
+                               P_D, i64]),
+    ("lra_scale_rows", C.c_int, [C.c_void_p, i64, i64, P_D, P_D, i64]),
     ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
                                      C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
                                      P_I, P_I, C.POINTER(i64)]),

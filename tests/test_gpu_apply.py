@@ -1,0 +1,53 @@
+"""Factor application (src/id.jl:94-331, src/pqr.jl:86-175, src/psvd.jl:61-124,
+src/pheig.jl:54-100) against dense numpy products of the same factors."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def test_idpackedv_and_id_apply(gpu, orc):
+    rng = np.random.default_rng(0)
+    A = orc.fourier_real(rng.random(180), rng.random(140))
+    V = gpu.idfact(A, seed=1)
+    sk, rd, T = host(V.sk), host(V.rd), host(V.T)
+    k, n = V.k, V.n
+    Vd = np.zeros((k, n))
+    Vd[:, sk - 1] = np.eye(k)
+    Vd[:, rd - 1] = T
+    assert np.allclose(host(V.matrix()), Vd, atol=1e-14)
+    X = rng.standard_normal((n, 5))
+    assert np.allclose(host(V.apply(X)), Vd @ X, atol=1e-12)
+    Yk = rng.standard_normal((k, 3))
+    assert np.allclose(host(V.apply(Yk, "c")), Vd.T @ Yk, atol=1e-12)
+    F = gpu.ID.from_matrix(A, V)
+    Cd = A[:, sk - 1]
+    assert np.allclose(host(F.apply(X)), Cd @ (Vd @ X), atol=1e-11)
+    Z = rng.standard_normal((180, 4))
+    assert np.allclose(host(F.apply(Z, "c")), Vd.T @ (Cd.T @ Z), atol=1e-11)
+
+
+def test_partial_factor_apply(gpu, orc):
+    rng = np.random.default_rng(1)
+    A = orc.fourier_real(rng.random(200), rng.random(150))
+    X = rng.standard_normal((150, 4))
+    Fq = gpu.pqrfact("n", A, seed=2)
+    Q, R, p = host(Fq.Q), host(Fq.R), host(Fq.p)
+    M = np.empty_like(A)
+    M[:, p - 1] = Q @ R
+    assert np.allclose(host(Fq.apply(X)), M @ X, atol=1e-11)
+    Fs = gpu.psvdfact(A, seed=3)
+    Md = (host(Fs.U) * host(Fs.S)) @ host(Fs.Vt)
+    assert np.allclose(host(Fs.apply(X)), Md @ X, atol=1e-11)
+    Y = rng.standard_normal((200, 2))
+    assert np.allclose(host(Fs.apply(Y, "c")), Md.T @ Y, atol=1e-11)
+    S = A[:150, :] @ A[:150, :].T
+    S = (S + S.T) / 2  # exactly symmetric
+    Fe = gpu.pheigfact(S, seed=4)
+    Me = (host(Fe.vectors) * host(Fe.values)) @ host(Fe.vectors).T
+    Xe = rng.standard_normal((150, 3))
+    assert np.allclose(host(Fe.apply(Xe)), Me @ Xe, atol=1e-9 * np.abs(S).max())
