@@ -451,11 +451,75 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
   }
 }
 
+// orthrows! (src/util.jl:86-97, thin = false) of B (ell x N, ld ell): the rows become the
+// orthonormal Q of B's LQ factorization, computed as the QR of B' (the same dlarfg steps as
+// gelqf on B); rows beyond min(ell, N) are zeroed.
+static int orthrows(lra_ctx* c, i64 ell, i64 N, double* B) {
+  const i64 k = std::min(ell, N);
+  if (k == 0) return LRA_OK;
+  double *X = nullptr, *Q = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_PI0, (size_t)N * ell + 2 * (size_t)ell + 16, &X));
+  double* tau = X + (size_t)N * ell;
+  LRA_CHECK(lra_wsT(c, WS_PI1, (size_t)N * k, &Q));
+  LRA_CHECK(lra_launch_transpose(c->stream, ell, N, B, ell, X, N));  // X = B' (N x ell)
+  LRA_CHECK(lra_geqrf_impl(c, N, k, X, N, tau));
+  LRA_CHECK(lra_orgqr_impl(c, N, k, X, N, tau, Q, N));
+  if (k < ell) LRA_HIP(hipMemsetAsync(B, 0, sizeof(double) * ell * N, c->stream));
+  // B(0:k, :) = Q'
+  if (k == ell) return lra_launch_transpose(c->stream, N, k, Q, N, B, ell);
+  double* Qt = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_PI0, (size_t)N * ell + 2 * (size_t)ell + 16, &Qt));
+  LRA_CHECK(lra_launch_transpose(c->stream, N, k, Q, N, Qt, k));
+  return lra_launch_copy2d(c->stream, k, N, Qt, k, B, ell);
+}
+
+// sketch_randn_ln / _lc power iterations (src/sketch.jl:129-174): Bp (ell x N) alternately
+// orthonormalised and multiplied by op(A)' and op(A); a Hermitian A skips the second
+// orthonormalisation (the two sketches swap).
+static int power_iterate(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                         i64 ell, double* B, i64 niter, bool herm) {
+  const i64 N = trans == LRA_TRANS_N ? n : m, P = trans == LRA_TRANS_N ? m : n;
+  double* Bq = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_PI2, (size_t)ell * std::max(N, P), &Bq));
+  for (i64 it = 0; it < niter; ++it) {
+    LRA_CHECK(orthrows(c, ell, N, B));
+    // Bq (ell x P) = Bp op(A)'
+    GemmArgs g;
+    g.bgather = nullptr; g.prec = LRA_PREC_F64; g.alpha = 1.0; g.beta = 0.0;
+    g.M = ell; g.N = P; g.K = N; g.A = B; g.lda = ell; g.transA = 0;
+    g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 1 : 0; g.C = Bq; g.ldc = ell;
+    LRA_CHECK(lra_gemm(c, g));
+    if (herm) {  // Bp, Bq = Bq, Bp (P == N)
+      LRA_CHECK(lra_launch_copy2d(c->stream, ell, N, Bq, ell, B, ell));
+      continue;
+    }
+    LRA_CHECK(orthrows(c, ell, P, Bq));
+    // Bp (ell x N) = Bq op(A)
+    g.M = ell; g.N = N; g.K = P; g.A = Bq; g.lda = ell; g.transA = 0;
+    g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 0 : 1; g.C = B; g.ldc = ell;
+    LRA_CHECK(lra_gemm(c, g));
+  }
+  return LRA_OK;
+}
+
 static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
                            const lra_opts* o, SketchFactOut* out, uint64_t stream_base = 0) {
   const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(A)
-  SketchFn make = [=](i64 ell, uint64_t stream, int prec, double* B) {
-    return lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, stream, prec, B, ell);
+  const i64 niter = o->sketch == LRA_SKETCH_RANDN ? o->sketch_randn_niter : 0;
+  bool herm = false;
+  if (niter > 0 && m == n) {  // ishermitian(A) (src/sketch.jl:132)
+    int* flag = nullptr;
+    LRA_CHECK(lra_wsT(c, WS_PI2, 64, &flag));
+    LRA_CHECK(lra_launch_is_symmetric(c->stream, n, A, lda, flag));
+    int h = 0;
+    LRA_CHECK(read_scalars(c, flag, &h, 1));
+    herm = h != 0;
+  }
+  SketchFn make = [=](i64 ell, uint64_t stream, int prec, double* B) -> int {
+    LRA_CHECK(lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, stream, prec, B,
+                             ell));
+    if (niter > 0) LRA_CHECK(power_iterate(c, trans, m, n, A, lda, ell, B, niter, herm));
+    return LRA_OK;
   };
   return sketchfact_run(c, N, make, o, stream_base, out);
 }

@@ -92,3 +92,26 @@ def test_sketch_none(gpu, orc):
     assert V.k == Vo.k and np.array_equal(V.sk.cpu().numpy(), Vo.sk)
     E = id_err(A, V.sk.cpu().numpy(), V.rd.cpu().numpy(), V.T.cpu().numpy())
     assert E < 100 * 5 * np.finfo(float).eps * np.linalg.norm(A)   # test/id.jl:20-32
+
+
+@pytest.mark.parametrize("herm", [False, True])
+def test_power_iteration_sketch(gpu, orc, herm):
+    """sketch_randn_niter = 1 (src/sketch.jl:129-174): orthrows! by the device QR, one pass
+    through A' and A (A itself when Hermitian)."""
+    rng = np.random.default_rng(11)
+    r = 120
+    sig = 10.0 ** (-8 * np.arange(r) / (r - 1))
+    U, _ = np.linalg.qr(rng.standard_normal((500, r)))
+    if herm:
+        A = np.asfortranarray((U * sig) @ U.T)
+        A = np.asfortranarray((A + A.T) / 2)
+    else:
+        V, _ = np.linalg.qr(rng.standard_normal((400, r)))
+        A = np.asfortranarray((U * sig) @ V.T)
+    print(f"power iteration herm={herm}", flush=True)
+    V, Vo = run_pair(gpu, orc, A, sketch_randn_niter=1, rtol=1e-6, seed=6)
+    assert V.k == Vo.k and V.rounds == Vo.rounds
+    assert pivots_match(V, Vo) >= min(20, V.k)
+    sk, rd, T = V.sk.cpu().numpy(), V.rd.cpu().numpy(), V.T.cpu().numpy()
+    e_g, e_o = id_err(A, sk, rd, T), id_err(A, Vo.sk, Vo.rd, Vo.T)
+    assert e_g <= 10 * max(e_o, 1e-15 * np.linalg.norm(A, 2))
