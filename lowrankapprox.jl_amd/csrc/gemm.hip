@@ -256,14 +256,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
 // the MFMAs consume the current one from a second LDS buffer, so each k-tile costs one
 // barrier.  LDS rows hold BK = 16 doubles padded to 17, which keeps the 16 lanes of an MFMA
 // operand read on distinct banks.
-template <int BM, int BN, int WAVES_M, int WAVES_N>
-__global__ __launch_bounds__(256) void gemm_kc_f64_kernel(GemmDev p) {
-  constexpr int BK = 16, LDK = 17;
+template <int BM, int BN, int WAVES_M, int WAVES_N, int BK = 16>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(GemmDev p) {
+  constexpr int NT = 64 * WAVES_M * WAVES_N;
+  constexpr int LDK = BK + 1, KP = BK / 2;  // KP: 16-byte k-pairs per tile row
   constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
   constexpr int TM = WTM / 16, TN = WTN / 16;
-  constexpr int CA = BM * BK / 2, CB = BN * BK / 2;  // 16-byte chunks per tile
-  constexpr int LA = (CA + 255) / 256, LB = (CB + 255) / 256;
-  static_assert(WAVES_M * WAVES_N == 4, "4 waves");
+  constexpr int CA = BM * KP, CB = BN * KP;  // 16-byte chunks per tile
+  constexpr int LA = (CA + NT - 1) / NT, LB = (CB + NT - 1) / NT;
   __shared__ double sA[2][BM * LDK];
   __shared__ double sB[2][BN * LDK];
 
@@ -281,12 +281,12 @@ __global__ __launch_bounds__(256) void gemm_kc_f64_kernel(GemmDev p) {
 
   typedef double d2 __attribute__((ext_vector_type(2)));
   d2 ra[LA], rb[LB];
-  // chunk e of a tile -> (row e / 8, k pair e % 8)
+  // chunk e of a tile -> (row e / KP, k pair e % KP)
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int t = 0; t < LA; ++t) {
-      const int e = tid + 256 * t;
-      const int r = e >> 3, c = (e & 7) * 2;
+      const int e = tid + NT * t;
+      const int r = e / KP, c = (e % KP) * 2;
       const int gm = m0 + r, gk = k0 + c;
       d2 v = {0.0, 0.0};
       if (e < CA && gm < p.M) {
@@ -298,8 +298,8 @@ __global__ __launch_bounds__(256) void gemm_kc_f64_kernel(GemmDev p) {
     }
 #pragma unroll
     for (int t = 0; t < LB; ++t) {
-      const int e = tid + 256 * t;
-      const int r = e >> 3, c = (e & 7) * 2;
+      const int e = tid + NT * t;
+      const int r = e / KP, c = (e % KP) * 2;
       const int gn = n0 + r, gk = k0 + c;
       d2 v = {0.0, 0.0};
       if (e < CB && gn < p.N) {
@@ -313,18 +313,18 @@ __global__ __launch_bounds__(256) void gemm_kc_f64_kernel(GemmDev p) {
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < LA; ++t) {
-      const int e = tid + 256 * t;
+      const int e = tid + NT * t;
       if (e < CA) {
-        const int o = (e >> 3) * LDK + (e & 7) * 2;
+        const int o = (e / KP) * LDK + (e % KP) * 2;
         sA[buf][o] = ra[t].x;
         sA[buf][o + 1] = ra[t].y;
       }
     }
 #pragma unroll
     for (int t = 0; t < LB; ++t) {
-      const int e = tid + 256 * t;
+      const int e = tid + NT * t;
       if (e < CB) {
-        const int o = (e >> 3) * LDK + (e & 7) * 2;
+        const int o = (e / KP) * LDK + (e % KP) * 2;
         sB[buf][o] = rb[t].x;
         sB[buf][o + 1] = rb[t].y;
       }
@@ -405,9 +405,11 @@ struct Cfg { int id, BM, BN; };
 // id -> (BM, BN, WAVES_M, WAVES_N)
 static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 256}, {4, 128, 128},
                             {5, 16, 256}, {6, 32, 256}, {7, 48, 256}, {8, 64, 256}, {9, 128, 128},
-                            {10, 64, 128}};
-constexpr int kNumCfg = 11;
-// ids 5..10: the k-contiguous FP64 kernel (gemm_kc_f64_kernel)
+                            {10, 64, 128}, {11, 128, 256}, {12, 128, 128}, {13, 64, 256},
+                            {14, 256, 128}};
+constexpr int kNumCfg = 15;
+// ids 5..14: the k-contiguous FP64 kernel (gemm_kc_f64_kernel; 11, 13, 14 with 512 threads,
+// 12 with 32-wide k-tiles)
 bool kc_cfg(int id) { return id >= 5; }
 bool kc_ok(const GemmArgs& g) {
   return g.prec == LRA_PREC_F64 && g.transA == 1 && g.transB == 0 && g.bgather == nullptr &&
@@ -429,6 +431,10 @@ hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
     case 8: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
     case 9: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
     case 10: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 11: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
+    case 12: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 32>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 13: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
+    case 14: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<256, 128, 4, 2>), dim3(blocks), dim3(512), 0, st, d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -474,7 +480,7 @@ int prepare(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits, GemmDev* d, in
   d->A = g.A; d->lda = g.lda; d->transA = g.transA;
   d->B = g.B; d->ldb = g.ldb; d->transB = g.transB;
   d->C = g.C; d->ldc = g.ldc; d->bgather = g.bgather;
-  const int bk = bk_of(g.prec);
+  const int bk = cfg == 12 ? 32 : bk_of(g.prec);
   i64 kchunk = 0;
   if (g.K <= 0) {
     splits = 1;
@@ -530,7 +536,8 @@ void candidates(const GemmArgs& g, int num_cu, std::vector<TuneVal>* out) {
     if (kc_cfg(cfg) && !kc) continue;
     // skip configs that waste most of the M tile
     if (c.BM > 16 && g.M <= c.BM / 2) continue;
-    if ((cfg == 4 || cfg == 9) && g.M < 96) continue;
+    if ((cfg == 4 || cfg == 9 || cfg == 11 || cfg == 12) && g.M < 96) continue;
+    if (cfg == 14 && g.M < 192) continue;
     const i64 tiles = ((g.M + c.BM - 1) / c.BM) * ((g.N + c.BN - 1) / c.BN);
     for (int s = 1; s <= 32; s *= 2) {
       if (s > 1 && tiles * s > 8ll * num_cu) break;
