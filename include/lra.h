@@ -313,15 +313,34 @@ int lra_pheigvals(lra_ctx* ctx, int64_t n, const double* A, int64_t lda, const l
 int lra_gesvj_f64(lra_ctx* ctx, int64_t m, int64_t n, double* X, int64_t ldx, double* U,
                   int64_t ldu, double* s, double* V, int64_t ldv);
 
-/* Batched ID of many independent blocks (MI355X extension; each block behaves as idfact
- * with a fresh Gaussian sketch of stream `b` under opts->seed): one persistent fused launch
- * (sketch GEMM + QRCP + T per block).  A[b], sk[b], rd[b], T[b] are device pointers given
- * in DEVICE arrays; k_out (device, nblk) receives the ranks.  Blocks whose rank reaches the
- * first sketch order are re-run through lra_idfact (status reports how many). */
+/* Batched ID of many independent blocks (MI355X extension; no counterpart in the reference,
+ * which factors blocks one idfact call at a time, src/id.jl:434-447).  Block b (m[b] x n[b],
+ * A[b] with lda[b]) gets exactly the result of lra_idfact on it alone with the same opts: its
+ * first adaptive round runs inside one persistent fused launch (sketch GEMM + pivoted QR +
+ * T), and a block whose rank reaches that round's target is completed by lra_idfact.  All
+ * per-block arrays are HOST arrays; the pointers in them are device addresses.  sk[b] and
+ * rd[b] hold n[b] entries, T[b] holds ldt[b] * n[b] doubles with ldt[b] >= the block's rank.
+ * k_out (host, nblk) receives the ranks, *n_fallback the number of blocks that went through
+ * lra_idfact. */
 int lra_idfact_batched(lra_ctx* ctx, int64_t nblk, const int64_t* m, const int64_t* n,
                        const double* const* A, const int64_t* lda, const lra_opts* opts,
                        int64_t* const* sk, int64_t* const* rd, double* const* T,
                        const int64_t* ldt, int64_t* k_out, int64_t* n_fallback);
+
+/* The same in three phases, for callers that replay one batch (MI355X extension):
+ * _plan validates the blocks, uploads their descriptors and draws Omega (allocates);
+ * _launch enqueues the counter reset + persistent kernel on the context's stream (or replays
+ * the graph made by _capture), allocating nothing; _finish waits and completes the blocks left
+ * to lra_idfact. */
+typedef struct lra_batch_plan lra_batch_plan;
+int lra_idfact_batched_plan(lra_ctx* ctx, int64_t nblk, const int64_t* m, const int64_t* n,
+                            const double* const* A, const int64_t* lda, const lra_opts* opts,
+                            int64_t* const* sk, int64_t* const* rd, double* const* T,
+                            const int64_t* ldt, lra_batch_plan** plan);
+int lra_idfact_batched_capture(lra_batch_plan* plan);
+int lra_idfact_batched_launch(lra_batch_plan* plan);
+int lra_idfact_batched_finish(lra_batch_plan* plan, int64_t* k_out, int64_t* n_fallback);
+int lra_idfact_batched_plan_destroy(lra_batch_plan* plan);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,8 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "laqps", "id_T", "maxdet_swapcols", "gaussian", "to_device", "LraError", "context",
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
-           "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm"]
+           "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
+           "BatchedID", "idfact_batched"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -421,6 +422,107 @@ def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
     k = int(out.k)
     T = _borrow_T(out, k, N - k)
     return IDPackedV(sk[:k], rd[: N - k], T, int(out.rounds))
+
+
+class BatchedID:
+    """A planned batched idfact of many independent blocks (lra_idfact_batched_plan): every
+    block gets the result of idfact on it alone with the same options; the first adaptive
+    round of all blocks runs as one persistent fused launch (sketch GEMM + pivoted QR + T),
+    and blocks whose rank reaches that round's target are completed by idfact.
+
+    plan = BatchedID(blocks, opts); plan.capture() (optional hipGraph); plan.launch();
+    ids = plan.finish()  ->  list of IDPackedV.  `ldt` is the row capacity reserved for each
+    T (default: the largest sketch order of the batched rounds, which bounds every rank they
+    return).
+    """
+
+    def __init__(self, blocks, opts: LRAOptions = None, ldt: Optional[int] = None, **kw):
+        torch = _torch()
+        opts = (opts or LRAOptions()).copy(**kw)
+        opts.pqrfact_retval = "t"
+        self.opts = opts
+        self.ctx = context()
+        self.blocks = [to_device(A) for A in blocks]
+        nb = len(self.blocks)
+        self.nblk = nb
+        self.m = np.array([A.shape[0] for A in self.blocks], dtype=np.int64)
+        self.n = np.array([A.shape[1] for A in self.blocks], dtype=np.int64)
+        self.lda = np.array([_ld(A) for A in self.blocks], dtype=np.int64)
+        a, b = opts.sketchfact_randn_samp
+        adap = opts.sketchfact_adap or opts.rank < 0
+        target = opts.nb if adap else opts.rank
+        cap = max(1, a * target + b)
+        while adap and a * 2 * target + b <= 72:   # the later stages' sketch orders
+            target *= 2
+            cap = a * target + b
+        cap = int(ldt) if ldt is not None else cap
+        self.ldt = np.array([max(1, min(int(n), cap)) for n in self.n], dtype=np.int64)
+        self.sk = [torch.empty(max(int(n), 1), dtype=torch.int64, device="cuda") for n in self.n]
+        self.rd = [torch.empty(max(int(n), 1), dtype=torch.int64, device="cuda") for n in self.n]
+        self.T = [torch.empty(int(l) * max(int(n), 1), dtype=torch.float64, device="cuda")
+                  for l, n in zip(self.ldt, self.n)]
+        arr = C.c_void_p * max(nb, 1)
+        self._A = arr(*[_ptr(A) for A in self.blocks])
+        self._sk = arr(*[_ptr(t) for t in self.sk])
+        self._rd = arr(*[_ptr(t) for t in self.rd])
+        self._T = arr(*[_ptr(t) for t in self.T])
+        self._copts = opts.to_c()
+        self.k = np.full(nb, -1, dtype=np.int64)
+        self.n_fallback = 0
+        self.h = C.c_void_p()
+        check(_lib.load().lra_idfact_batched_plan(
+            self.ctx.h, nb, self.m.ctypes.data, self.n.ctypes.data, C.cast(self._A, C.c_void_p),
+            self.lda.ctypes.data, C.byref(self._copts), C.cast(self._sk, C.c_void_p),
+            C.cast(self._rd, C.c_void_p), C.cast(self._T, C.c_void_p), self.ldt.ctypes.data,
+            C.byref(self.h)), "lra_idfact_batched_plan")
+
+    def capture(self) -> None:
+        """Capture the launch in a hipGraph (needs a non-default current stream)."""
+        check(_lib.load().lra_idfact_batched_capture(self.h), "lra_idfact_batched_capture")
+
+    def launch(self) -> None:
+        check(_lib.load().lra_idfact_batched_launch(self.h), "lra_idfact_batched_launch")
+
+    def finish(self) -> list:
+        i64v = C.c_int64(0)
+        st = _lib.load().lra_idfact_batched_finish(self.h, self.k.ctypes.data, C.byref(i64v))
+        over = []
+        if st != 0:
+            over = [b for b in range(self.nblk) if self.k[b] > self.ldt[b]]
+            if st != _lib.E_UNSUPPORTED or not over:
+                check(st, "lra_idfact_batched_finish")
+        self.n_fallback = int(i64v.value)
+        out = []
+        for b in range(self.nblk):
+            if b in over:  # rank beyond the reserved T rows: idfact with its own T
+                out.append(idfact(self.blocks[b], self.opts))
+                continue
+            k, n, ldt = int(self.k[b]), int(self.n[b]), int(self.ldt[b])
+            T = self.T[b][: ldt * max(n - k, 0)].reshape(max(n - k, 0), ldt).t()[:k]
+            out.append(IDPackedV(self.sk[b][:k], self.rd[b][: n - k], T))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            _lib.load().lra_idfact_batched_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def idfact_batched(blocks, opts: LRAOptions = None, **kw) -> list:
+    """idfact on every block of `blocks` (list of m_b x n_b matrices) through one persistent
+    fused launch (MI355X extension): the same IDPackedV as idfact(A_b, opts) for each."""
+    plan = BatchedID(blocks, opts, **kw)
+    try:
+        plan.launch()
+        return plan.finish()
+    finally:
+        plan.close()
 
 
 def _borrow_T(out, k: int, nr: int):

@@ -132,6 +132,13 @@ SIGNATURES = [
     ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
                                      C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
                                      P_I, P_I, C.POINTER(i64)]),
+    ("lra_idfact_batched_plan", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
+                                          C.POINTER(lra_opts), C.c_void_p, C.c_void_p,
+                                          C.c_void_p, P_I, C.POINTER(C.c_void_p)]),
+    ("lra_idfact_batched_capture", C.c_int, [C.c_void_p]),
+    ("lra_idfact_batched_launch", C.c_int, [C.c_void_p]),
+    ("lra_idfact_batched_finish", C.c_int, [C.c_void_p, P_I, C.POINTER(i64)]),
+    ("lra_idfact_batched_plan_destroy", C.c_int, [C.c_void_p]),
 ]
 
 
@@ -160,6 +167,9 @@ def load(path: str = LIB_PATH):
             f.argtypes = args
         _lib = L
         return L
+
+
+E_HIP, E_ALLOC, E_UNSUPPORTED, E_TIMEOUT, E_STATE = 1, 2, 3, 4, 5
 
 
 def check(status: int, what: str = "") -> None:

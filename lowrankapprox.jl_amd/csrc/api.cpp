@@ -61,6 +61,7 @@ static int chkopts(const lra_opts* o) {
   if (o->precision < LRA_PREC_F64 || o->precision > LRA_PREC_BF16X3) { lra_set_error("precision"); return -1; }
   return LRA_OK;
 }
+int lra_chkopts_public(const lra_opts* o) { return chkopts(o); }
 
 // ---------------------------------------------------------------------------------------
 // context
