@@ -326,12 +326,15 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
     double a[RPT];
     {
       const double* src = Bs + (i64)lc * LM + rbase;
+      // (row tests as i < ell - rbase: a per-lane bound against compile-time i, instead of
+      // RPT per-lane row numbers the compiler would keep live)
+      const int el = ell - rbase;
 #pragma unroll
       for (int i = 0; i < RPT; i += 2) {
         d2 v = {0.0, 0.0};
         if (have) v = *reinterpret_cast<const d2*>(src + i);
-        a[i] = (rbase + i < ell) ? v.x : 0.0;
-        a[i + 1] = (rbase + i + 1 < ell) ? v.y : 0.0;
+        a[i] = (i < el) ? v.x : 0.0;
+        a[i + 1] = (i + 1 < el) ? v.y : 0.0;
       }
     }
     double vn1, vn2;
@@ -437,7 +440,8 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
       // rows l and l + 64
       const int rk = s;
       double tau = 0.0;
-      {
+      // waves without an active column skip it (wave 0 always builds it: it writes R11)
+      if (wave == 0 || __any(active)) {
         const double* xc = cw >= 0 ? sCand[par][cw] : sPc[CL == 2 ? wave : 0];
         double xr[NR];
         double ss = 0.0;
@@ -515,10 +519,11 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
         // re-read v from LDS below rather than keeping its values live beside the column
         asm volatile("" ::: "memory");
         double xs = 0.0;
+        const int rl = rk - rbase;  // row rk within this lane's part
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
           a[i] -= tw * v[i];
-          xs = (rbase + i == rk) ? a[i] : xs;
+          xs = (i == rl) ? a[i] : xs;
         }
         xs = group_sum(xs, TPC);  // new R(rk, j), from the lane that holds row rk
         if (rk + 1 < lastrk && vn1 != 0.0) {
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
           // the block ends here; the flagged norm is recomputed from the updated rows
           double s3 = 0.0;
 #pragma unroll
-          for (int i = 0; i < RPT; ++i) s3 += (rbase + i > rk) ? a[i] * a[i] : 0.0;
+          for (int i = 0; i < RPT; ++i) s3 += (i > rl) ? a[i] * a[i] : 0.0;
           vn1 = vn2 = sqrt(group_sum(s3, TPC));
           myflag = 1;
         }
