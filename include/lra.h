@@ -342,6 +342,41 @@ int lra_idfact_batched_launch(lra_batch_plan* plan);
 int lra_idfact_batched_finish(lra_batch_plan* plan, int64_t* k_out, int64_t* n_fallback);
 int lra_idfact_batched_plan_destroy(lra_batch_plan* plan);
 
+/*
+ * Serving (MI355X extension; the reference has no server, each call factors one matrix).
+ * Requests on device-resident matrices are admitted in arrival order when a worker is free
+ * and their memory estimate fits the HBM budget next to the factors already held; workers
+ * (host threads with a context each) overlap their kernels on the device; factors stay in
+ * device memory until released.  Each result is what lra_idfact / lra_psvdfact returns for
+ * the request alone (the workers call them).  Times are seconds on the server's clock.
+ */
+enum { LRA_REQ_IDFACT = 0, LRA_REQ_PSVDFACT = 1 };
+enum { LRA_REQ_QUEUED = 0, LRA_REQ_RUNNING = 1, LRA_REQ_DONE = 2, LRA_REQ_FAILED = 3 };
+typedef struct lra_server lra_server;
+typedef struct lra_server_out {
+  int32_t kind, status;
+  int64_t m, n, k;
+  int64_t* sk; /* idfact: sk (k), rd (n-k), T (k x (n-k), ldt) */
+  int64_t* rd;
+  double* T;
+  int64_t ldt;
+  double* U; /* psvdfact: U (m x k, ldu), S (k), Vt (k x n, ldvt) */
+  int64_t ldu;
+  double* S;
+  double* Vt;
+  int64_t ldvt;
+  double t_submit, t_start, t_done;
+} lra_server_out;
+int lra_server_create(int device, size_t hbm_budget, int nworkers, lra_server** srv);
+int lra_server_submit(lra_server* srv, int kind, int64_t m, int64_t n, const double* A,
+                      int64_t lda, const lra_opts* opts, int64_t* handle);
+int lra_server_poll(lra_server* srv, int64_t handle, int32_t* status);
+int lra_server_wait(lra_server* srv, int64_t handle, double timeout_s, int32_t* status);
+int lra_server_result(lra_server* srv, int64_t handle, lra_server_out* out);
+int lra_server_release(lra_server* srv, int64_t handle);
+int lra_server_stats(lra_server* srv, int64_t* queued, int64_t* running, size_t* bytes_held);
+int lra_server_destroy(lra_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
            "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
-           "BatchedID", "idfact_batched"]
+           "BatchedID", "idfact_batched", "Server"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -949,3 +949,118 @@ def snormdiff(A, B, opts: LRAOptions = None, x0=None, **kw) -> float:
         check(L.lra_snormdiff(ctx.h, m, n, _ptr(A), _ld(A), _ptr(Bd), _ld(Bd), herm,
                               C.byref(copts), _ptr(x), C.byref(res)), "lra_snormdiff")
     return float(res.value)
+
+
+# ----------------------------------------------------------------------------------------
+# serving
+
+class Server:
+    """Serving loop for idfact / psvdfact requests on device-resident matrices (lra_server_*):
+    requests are admitted in arrival order when a worker is free and their memory estimate
+    fits `hbm_budget` next to the factors held; `workers` host threads, each with its own
+    context and stream, overlap their requests' kernels; factors stay on the device until
+    `release`.  Each result equals idfact / psvdfact of the request alone."""
+
+    KINDS = {"idfact": 0, "psvdfact": 1}
+    STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+    def __init__(self, hbm_budget: int = 64 << 30, workers: int = 4,
+                 device: Optional[int] = None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise LraError("no GPU visible: the MI355X path has no CPU fallback")
+        self.device = torch.cuda.current_device() if device is None else device
+        self.h = C.c_void_p()
+        check(_lib.load().lra_server_create(self.device, int(hbm_budget), int(workers),
+                                            C.byref(self.h)), "lra_server_create")
+        self._keep = {}  # handle -> (A, kind): A stays alive while the request may read it
+
+    def submit(self, A, kind: str = "idfact", opts: LRAOptions = None, **kw) -> int:
+        torch = _torch()
+        opts = (opts or LRAOptions()).copy(**kw)
+        if kind == "idfact":
+            opts.pqrfact_retval = "t"
+        A = to_device(A)
+        # the workers' streams do not wait for torch's: A must be complete now
+        torch.cuda.current_stream().synchronize()
+        copts = opts.to_c()
+        h = _lib.i64(0)
+        check(_lib.load().lra_server_submit(self.h, self.KINDS[kind], A.shape[0], A.shape[1],
+                                            _ptr(A), _ld(A), C.byref(copts), C.byref(h)),
+              "lra_server_submit")
+        self._keep[int(h.value)] = (A, kind)
+        return int(h.value)
+
+    def poll(self, h: int) -> str:
+        st = C.c_int32(0)
+        check(_lib.load().lra_server_poll(self.h, h, C.byref(st)), "lra_server_poll")
+        return self.STATUS[st.value]
+
+    def wait(self, h: int, timeout: float = -1.0) -> str:
+        st = C.c_int32(0)
+        check(_lib.load().lra_server_wait(self.h, h, float(timeout), C.byref(st)),
+              "lra_server_wait")
+        return self.STATUS[st.value]
+
+    def _out(self, h: int):
+        out = _lib.lra_server_out()
+        check(_lib.load().lra_server_result(self.h, h, C.byref(out)), "lra_server_result")
+        return out
+
+    def times(self, h: int):
+        """(submitted, started, done) in seconds on the server's clock."""
+        out = self._out(h)
+        return out.t_submit, out.t_start, out.t_done
+
+    def result(self, h: int):
+        """Copy of the factors: IDPackedV (idfact) or PartialSVD (psvdfact)."""
+        torch = _torch()
+        out = self._out(h)
+        m, n, k = int(out.m), int(out.n), int(out.k)
+        hip = _hip()
+        st = torch.cuda.current_stream().cuda_stream
+        if out.kind == 0:
+            sk = torch.empty(max(k, 1), dtype=torch.int64, device="cuda")
+            rd = torch.empty(max(n - k, 1), dtype=torch.int64, device="cuda")
+            if k > 0:
+                hip.hipMemcpyAsync(C.c_void_p(sk.data_ptr()), C.c_void_p(out.sk),
+                                   C.c_size_t(8 * k), C.c_int(3), C.c_void_p(st))
+            if n - k > 0:
+                hip.hipMemcpyAsync(C.c_void_p(rd.data_ptr()), C.c_void_p(out.rd),
+                                   C.c_size_t(8 * (n - k)), C.c_int(3), C.c_void_p(st))
+            T = empty_colmajor(k, n - k)
+            if k > 0 and n - k > 0:
+                _copy_device(T.data_ptr(), int(out.T), k, n - k, int(out.ldt), k)
+            return IDPackedV(sk[:k], rd[: n - k], T)
+        U = empty_colmajor(m, k)
+        Vt = empty_colmajor(k, n)
+        S = torch.empty(k, dtype=torch.float64, device="cuda")
+        if k > 0:
+            _copy_device(U.data_ptr(), int(out.U), m, k, int(out.ldu), m)
+            _copy_device(Vt.data_ptr(), int(out.Vt), k, n, int(out.ldvt), k)
+            _copy_device(S.data_ptr(), int(out.S), k, 1, k, k)
+        return PartialSVD(U, S, Vt)
+
+    def release(self, h: int) -> None:
+        torch = _torch()
+        torch.cuda.current_stream().synchronize()  # copies made by result() read the factors
+        check(_lib.load().lra_server_release(self.h, h), "lra_server_release")
+        self._keep.pop(h, None)
+
+    def stats(self):
+        q, r, b = _lib.i64(0), _lib.i64(0), C.c_size_t(0)
+        check(_lib.load().lra_server_stats(self.h, C.byref(q), C.byref(r), C.byref(b)),
+              "lra_server_stats")
+        return {"queued": int(q.value), "running": int(r.value), "bytes_held": int(b.value)}
+
+    def close(self) -> None:
+        if self.h:
+            _lib.load().lra_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -67,6 +67,13 @@ class lra_pheig_out(C.Structure):
                 ("kp", i64)]
 
 
+class lra_server_out(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("status", C.c_int32), ("m", i64), ("n", i64), ("k", i64),
+                ("sk", P_I), ("rd", P_I), ("T", P_D), ("ldt", i64), ("U", P_D), ("ldu", i64),
+                ("S", P_D), ("Vt", P_D), ("ldvt", i64), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
 APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, i64, C.c_void_p, i64, C.c_void_p, i64,
                        C.c_void_p)
 
@@ -139,6 +146,16 @@ SIGNATURES = [
     ("lra_idfact_batched_launch", C.c_int, [C.c_void_p]),
     ("lra_idfact_batched_finish", C.c_int, [C.c_void_p, P_I, C.POINTER(i64)]),
     ("lra_idfact_batched_plan_destroy", C.c_int, [C.c_void_p]),
+    ("lra_server_create", C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    ("lra_server_submit", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64,
+                                    C.POINTER(lra_opts), C.POINTER(i64)]),
+    ("lra_server_poll", C.c_int, [C.c_void_p, i64, C.POINTER(C.c_int32)]),
+    ("lra_server_wait", C.c_int, [C.c_void_p, i64, C.c_double, C.POINTER(C.c_int32)]),
+    ("lra_server_result", C.c_int, [C.c_void_p, i64, C.POINTER(lra_server_out)]),
+    ("lra_server_release", C.c_int, [C.c_void_p, i64]),
+    ("lra_server_stats", C.c_int, [C.c_void_p, C.POINTER(i64), C.POINTER(i64),
+                                   C.POINTER(C.c_size_t)]),
+    ("lra_server_destroy", C.c_int, [C.c_void_p]),
 ]
 
 

@@ -12,6 +12,8 @@
 
 #include "lra_internal.h"
 
+#include <mutex>
+
 #include <functional>
 
 static thread_local char g_err[1024] = "";
@@ -64,6 +66,50 @@ static int chkopts(const lra_opts* o) {
 int lra_chkopts_public(const lra_opts* o) { return chkopts(o); }
 
 // ---------------------------------------------------------------------------------------
+// device-wide order of the grid-synchronising kernels
+
+namespace {
+struct SpinGate {
+  std::mutex mu;
+  hipEvent_t last = nullptr;
+};
+SpinGate g_spin[64];
+}  // namespace
+
+SpinSection::SpinSection(lra_ctx* ctx) : c(ctx) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (cs != hipStreamCaptureStatusNone || c->device < 0 || c->device >= 64) return;
+  SpinGate& g = g_spin[c->device];
+  g.mu.lock();
+  held = true;
+  if (g.last) (void)hipStreamWaitEvent(c->stream, g.last, 0);
+}
+
+SpinSection::~SpinSection() {
+  if (!held) return;
+  SpinGate& g = g_spin[c->device];
+  if (!c->spin_ev && hipEventCreateWithFlags(&c->spin_ev, hipEventDisableTiming) != hipSuccess) {
+    c->spin_ev = nullptr;
+    (void)hipGetLastError();
+  }
+  if (c->spin_ev && hipEventRecord(c->spin_ev, c->stream) == hipSuccess) g.last = c->spin_ev;
+  g.mu.unlock();
+}
+
+void lra_spin_forget(lra_ctx* c) {
+  if (c->device < 0 || c->device >= 64 || !c->spin_ev) return;
+  SpinGate& g = g_spin[c->device];
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.last == c->spin_ev) g.last = nullptr;
+  (void)hipEventDestroy(c->spin_ev);
+  c->spin_ev = nullptr;
+}
+
+// ---------------------------------------------------------------------------------------
 // context
 
 extern "C" int lra_ctx_create(int device, size_t hbm_budget, lra_ctx** out) {
@@ -92,6 +138,7 @@ extern "C" int lra_ctx_destroy(lra_ctx* c) {
   if (!c) return LRA_OK;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  lra_spin_forget(c);
   for (int i = 0; i < lra_ctx::kSlots; ++i)
     if (c->ws[i]) hipFree(c->ws[i]);
   if (c->coop) hipFree(c->coop);  // the uncached area (coop_uc) stays allocated

@@ -702,7 +702,13 @@ static int enqueue(lra_batch_plan* P) {
       s.gen = P->d_counts + 4 * NSTAGE;
       s.err = c->d_err;
       const int grid = kind == 0 ? P->grid : 2 * P->npairs;
-      const hipError_t e = launch_variant(P->variant[r], kind == 1, s, grid, c->stream);
+      hipError_t e;
+      if (kind == 1) {
+        SpinSection gate(c);
+        e = launch_variant(P->variant[r], true, s, grid, c->stream);
+      } else {
+        e = launch_variant(P->variant[r], false, s, grid, c->stream);
+      }
       if (e != hipSuccess) {
         lra_set_error("batched launch: %s", hipGetErrorString(e));
         return LRA_E_HIP;

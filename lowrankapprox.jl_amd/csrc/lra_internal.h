@@ -58,7 +58,22 @@ struct lra_ctx {
   bool coop_uc_tried = false;
   uint32_t qrcp_gen = 0;      // launch generation of the cooperative QRCP (chunk tags)
   int* d_err = nullptr;  // device error word (timeout etc.)
+  hipEvent_t spin_ev = nullptr;  // recorded after this context's grid-synchronising kernels
 };
+
+// Kernels whose workgroups wait for each other (the cooperative QRCP, the geqrf panel, the
+// batched workgroup pairs) need all their workgroups resident at once.  Two of them from
+// different contexts running together could each hold part of the device while waiting for
+// their remaining workgroups, so on one device they run one at a time, in launch order: a
+// SpinSection around such a launch makes the context's stream wait for the previous one.
+// Other kernels overlap freely.  (Skipped while the stream is being captured.)
+struct SpinSection {
+  explicit SpinSection(lra_ctx* c);
+  ~SpinSection();
+  lra_ctx* c;
+  bool held = false;
+};
+void lra_spin_forget(lra_ctx* c);  // at context destruction
 
 // named workspace slots (each call path uses fixed slots so they can overlap safely)
 enum WsSlot {

@@ -312,7 +312,11 @@ int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* 
   d.uncached = uc ? 1 : 0;
   d.err = c->d_err;
   LRA_CHECK(lra_launch_zero_u32(c->stream, (unsigned*)area, 256 / 4));
-  hipError_t e = rpr == 1 ? launch_panel<1>(d, c->stream) : launch_panel<2>(d, c->stream);
+  hipError_t e;
+  {
+    SpinSection gate(c);
+    e = rpr == 1 ? launch_panel<1>(d, c->stream) : launch_panel<2>(d, c->stream);
+  }
   if (e != hipSuccess) {
     lra_set_error("geqrf panel launch (m=%lld G=%lld): %s", (long long)m, (long long)G,
                   hipGetErrorString(e));

@@ -732,7 +732,13 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   // chunk tags carry the launch generation instead
   LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + slot_bytes) / 4));
   d.gen = ++ctx->qrcp_gen;
-  hipError_t e = launch_any(sh.rpt, sh.block, d, ctx->stream);
+  hipError_t e;
+  if (sh.G > 1) {
+    SpinSection gate(ctx);
+    e = launch_any(sh.rpt, sh.block, d, ctx->stream);
+  } else {
+    e = launch_any(sh.rpt, sh.block, d, ctx->stream);
+  }
   if (e != hipSuccess) {
     lra_set_error("qrcp launch (m=%d n=%d tpc=%d rpt=%d G=%d): %s", q.m, q.n, sh.tpc, sh.rpt,
                   sh.G, hipGetErrorString(e));

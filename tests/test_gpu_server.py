@@ -1,0 +1,105 @@
+"""Serving loop: requests of different sizes and kinds admitted together overlap on the device
+and each result equals the request run alone (idfact / psvdfact on the same matrix and
+options); admission by memory; release returns the memory."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def decaying(m, n, rate, seed):
+    """A = U diag(exp(-rate (i-1))) V' with U, V the Q factors of seeded Gaussian matrices."""
+    rng = np.random.default_rng(seed)
+    r = min(m, n, int(40 / rate) + 1)
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = np.exp(-rate * np.arange(r))
+    return np.asfortranarray((U * s) @ V.T)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def id_err(A, V):
+    sk, rd, T = host(V.sk), host(V.rd), host(V.T)
+    return np.linalg.norm(A[:, rd - 1] - A[:, sk - 1] @ T, 2) if len(rd) else 0.0
+
+
+REQUESTS = [
+    ("idfact", 900, 700, 0.05, 1e-10), ("psvdfact", 600, 600, 0.1, 1e-6),
+    ("idfact", 1500, 300, 0.2, 1e-6), ("psvdfact", 1200, 800, 0.08, 1e-10),
+    ("idfact", 400, 400, 0.02, 1e-10), ("idfact", 2000, 1000, 0.12, 1e-10),
+    ("psvdfact", 300, 256, 0.15, 1e-6), ("idfact", 700, 700, 0.01, 1e-6),
+    ("psvdfact", 1000, 500, 0.03, 1e-10), ("idfact", 256, 900, 0.06, 1e-6),
+]
+
+
+def test_server_mix_equals_alone(gpu):
+    mats = [gpu.to_device(decaying(m, n, r, 100 + i))
+            for i, (_, m, n, r, _) in enumerate(REQUESTS)]
+    srv = gpu.Server(hbm_budget=32 << 30, workers=4)
+    try:
+        hs = [srv.submit(A, kind, rtol=rtol, seed=3)
+              for A, (kind, _, _, _, rtol) in zip(mats, REQUESTS)]
+        print(f"submitted {len(hs)} requests", flush=True)
+        for h in hs:
+            assert srv.wait(h, 120.0) == "done"
+        # requests overlapped: some started before others had finished
+        t = [srv.times(h) for h in hs]
+        assert any(t[j][1] < t[i][2] for i in range(len(t)) for j in range(len(t)) if i != j)
+        for h, A, (kind, m, n, rate, rtol) in zip(hs, mats, REQUESTS):
+            Ah = host(A)
+            got = srv.result(h)
+            if kind == "idfact":
+                alone = gpu.idfact(A, rtol=rtol, seed=3)
+                assert got.k == alone.k, (m, n)
+                e, ea = id_err(Ah, got), id_err(Ah, alone)
+                assert e <= 10 * max(ea, 1e-14 * np.linalg.norm(Ah, 2))
+                if np.array_equal(host(got.sk), host(alone.sk)) and got.T.numel():
+                    Tg, Ta = host(got.T), host(alone.T)
+                    assert np.abs(Tg - Ta).max() <= 1e-6 * max(1.0, np.abs(Ta).max())
+            else:
+                alone = gpu.psvdfact(A, rtol=rtol, seed=3)
+                assert got.k == alone.k, (m, n)
+                Sg, Sa = host(got.S), host(alone.S)
+                assert np.abs(Sg - Sa).max() <= 1e-9 * Sa[0]
+                R = host(got.U) @ np.diag(Sg) @ host(got.Vt)
+                Ra = host(alone.U) @ np.diag(Sa) @ host(alone.Vt)
+                ea = np.linalg.norm(Ah - Ra, 2)
+                assert np.linalg.norm(Ah - R, 2) <= 10 * max(ea, 1e-14 * Sa[0])
+            srv.release(h)
+        assert srv.stats() == {"queued": 0, "running": 0, "bytes_held": 0}
+    finally:
+        srv.close()
+
+
+def test_server_admission_by_memory(gpu):
+    """A budget that holds one request's estimate (about 15.6 MB for idfact of 800 x 600) but
+    not two: with three workers the requests still run one after another."""
+    A = gpu.to_device(decaying(800, 600, 0.1, 7))
+    srv = gpu.Server(hbm_budget=20 << 20, workers=3)
+    try:
+        hs = [srv.submit(A, "idfact", rtol=1e-8) for _ in range(3)]
+        for h in hs:
+            assert srv.wait(h, 60.0) == "done"
+        t = sorted(srv.times(h) for h in hs)
+        for a, b in zip(t, t[1:]):
+            assert b[1] >= a[2]  # started after the previous one was done
+        ks = {srv.result(h).k for h in hs}
+        assert len(ks) == 1
+        for h in hs:
+            srv.release(h)
+        assert srv.stats()["bytes_held"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_refuses_oversized(gpu):
+    A = gpu.to_device(decaying(500, 500, 0.1, 9))
+    srv = gpu.Server(hbm_budget=1 << 20, workers=1)
+    try:
+        with pytest.raises(gpu.LraError):
+            srv.submit(A, "psvdfact")
+    finally:
+        srv.close()
