@@ -7,6 +7,8 @@ One step = one complete idfact of A (sketch rounds + pivoted QR + T), A resident
 Reported: the plain FP64 drop-in (`value`), the mixed-precision form beside it, p50/p99
 latency per ID, the dominant kernel against its MI355X roofline (HIP events on the stream the
 kernel runs on), and the CPU restatement of the reference timed on this host's cores.
+Beside them: the batched form on 4096 uneven kernel blocks (rtol 1e-8, one captured launch)
+and the serving loop under Poisson arrivals (p50/p99 latency and completed requests/s).
 """
 from __future__ import annotations
 
@@ -132,6 +134,153 @@ def kernel_roofline(lra, torch, A, reps=5):
              "flops_per_launch": gflops})
 
 
+def kernel_blocks(torch, nblk: int = 4096, seed: int = 5):
+    """Config-4 blocks on the device: m, n = round(64 16^U), K[i,j] = 1/|x_i - y_j| with x_i
+    uniform in [0,1]^3 and y_j in [3,4] x [0,1]^2 (input generation, not timed)."""
+    rng = np.random.default_rng(seed)
+    mn = np.rint(64 * 16.0 ** rng.random((nblk, 2))).astype(int)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    blocks = []
+    for m, n in mn:
+        x = torch.rand(int(m), 3, device="cuda", dtype=torch.float64, generator=g)
+        y = torch.rand(int(n), 3, device="cuda", dtype=torch.float64, generator=g)
+        y[:, 0] += 3.0
+        blocks.append((1.0 / torch.cdist(x, y)).t().contiguous().t())
+    return blocks, mn
+
+
+def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
+    """Config 4: batched idfact at rtol=1e-8 of 4096 uneven kernel blocks, the whole batch as
+    one captured launch (counter reset + one persistent kernel per adaptive round), replayed
+    `reps` times; beside it the plain FP64 drop-in (idfact per block) on a 64-block sample."""
+    blocks, mn = kernel_blocks(torch, nblk)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan = lra.BatchedID(blocks, rtol=1e-8)
+        plan.launch()
+        plan.finish()
+        ks = plan.k.copy()
+        nfb = plan.n_fallback
+        plan.capture()
+        plan.launch()
+        s.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            plan.launch()
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        plan.finish()
+        plan.close()
+    # the reference's rounds per block: order 40 accepts k < 32, order 72 accepts k < 64
+    rounds = np.where(ks < 32, 1, 2)
+    mnp = (mn[:, 0] * mn[:, 1]).astype(float)
+    bytes_alg = float(8.0 * (mnp * rounds).sum())
+    flops = float((2.0 * 40 * mnp).sum() + (2.0 * 72 * mnp * (rounds > 1)).sum())
+    sample = list(range(0, nblk, nblk // 64))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in sample:
+        lra.idfact(blocks[b], rtol=1e-8)
+    torch.cuda.synchronize()
+    drop = len(sample) / (time.perf_counter() - t0)
+    ids_s = nblk / (ms * 1e-3)
+    gbs = bytes_alg / (ms * 1e-3) / 1e9
+    return {"workload": f"{nblk} kernel blocks K=1/|x-y|, m,n=round(64*16^U) (sum m*n = "
+                        f"{int(mnp.sum())}), rtol=1e-8, FP64",
+            "value": round(ids_s, 1), "unit": "IDs/s", "ms_per_batch": round(ms, 3),
+            "ranks": {"min": int(ks.min()), "max": int(ks.max()),
+                      "second_round": int((ks >= 32).sum())},
+            "blocks_left_to_idfact": int(nfb),
+            "fp64_dropin_ids_per_s": round(drop, 1),
+            "speedup_vs_fp64_dropin": round(ids_s / drop, 1),
+            "roofline": {"kernel": "id_batched_kernel (all rounds)", "bound": "hbm",
+                         "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                         "bytes_per_launch": bytes_alg, "mfma_flops_per_launch": flops,
+                         "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 2)},
+            "precision_note": "FP64 sketch: at rtol=1e-8 an FP32 sketch cannot certify the "
+                              "rank (noise floor 1e-6), idfact would recompute it in FP64"}
+
+
+def serving_pool(torch, count: int, seed: int = 6):
+    """Config-5 matrices: m log-uniform in [256, 8192], square or tall (n log-uniform in
+    [256, m]) with probability 1/2, A = U diag(exp(-r (i-1))) V', r ~ U[0.01, 0.2], U and V
+    Q factors of Gaussian matrices (singular values below exp(-40) sigma_1 dropped)."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    pool = []
+    for _ in range(count):
+        m = int(np.rint(256 * 32.0 ** rng.random()))
+        n = m if rng.random() < 0.5 else int(np.rint(256 * (m / 256.0) ** rng.random()))
+        rate = float(rng.uniform(0.01, 0.2))
+        r = min(m, n, int(40.0 / rate) + 1)
+        U, _ = torch.linalg.qr(torch.randn(m, r, dtype=torch.float64, device="cuda",
+                                           generator=g))
+        V, _ = torch.linalg.qr(torch.randn(n, r, dtype=torch.float64, device="cuda",
+                                           generator=g))
+        sig = torch.exp(-rate * torch.arange(r, dtype=torch.float64, device="cuda"))
+        pool.append(((U * sig) @ V.T).t().contiguous().t())
+        del U, V
+    torch.cuda.synchronize()
+    return pool
+
+
+def bench_serving(lra, torch, rates=(200, 1000, 5000), nreq: int = 32, pool_size: int = 12,
+                  workers: int = 4, budget_gb: int = 96, seed: int = 6):
+    """Config 5: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
+    1e-10 50/50) on resident matrices; per rate: requests/s completed and p50/p99 latency
+    from submission to result (server clock)."""
+    pool = serving_pool(torch, pool_size, seed)
+    rng = np.random.default_rng(seed + 1)
+    srv = lra.Server(hbm_budget=budget_gb << 30, workers=workers)
+    out = []
+    try:
+        # warm-up: every matrix once per kind (workspaces grow, GEMM shapes tune)
+        print(f"# serving pool ready: {[tuple(A.shape) for A in pool]}", file=sys.stderr,
+              flush=True)
+        hs = [srv.submit(A, kind, rtol=1e-10) for A in pool for kind in ("idfact", "psvdfact")]
+        for i, h in enumerate(hs):
+            srv.wait(h)
+            t = srv.times(h)
+            print(f"# warm-up request {i}: {1e3 * (t[2] - t[1]):.1f} ms", file=sys.stderr,
+                  flush=True)
+            srv.release(h)
+        for lam in rates:
+            gaps = rng.exponential(1.0 / lam, nreq)
+            reqs = [(int(rng.integers(len(pool))), "idfact" if rng.random() < 0.5 else "psvdfact",
+                     1e-6 if rng.random() < 0.5 else 1e-10) for _ in range(nreq)]
+            hs = []
+            t_next = time.perf_counter()
+            for gap, (i, kind, rtol) in zip(gaps, reqs):
+                t_next += gap
+                while time.perf_counter() < t_next:
+                    pass
+                hs.append(srv.submit(pool[i], kind, rtol=rtol, seed=len(hs)))
+            for h in hs:
+                srv.wait(h)
+            tm = np.array([srv.times(h) for h in hs])
+            for h in hs:
+                srv.release(h)
+            lat = tm[:, 2] - tm[:, 0]
+            span = tm[:, 2].max() - tm[:, 0].min()
+            out.append({"arrival_rate": lam, "requests": nreq,
+                        "completed_per_s": round(nreq / span, 2),
+                        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+                        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2)})
+            print(f"# serving rate {lam}/s: {out[-1]}", file=sys.stderr, flush=True)
+    finally:
+        srv.close()
+    sizes = [tuple(A.shape) for A in pool]
+    return {"workload": f"{pool_size} resident matrices {sizes}, idfact/psvdfact 50/50, "
+                        "rtol 1e-6/1e-10 50/50, sigma_i = exp(-r(i-1)), r ~ U[0.01,0.2]",
+            "workers": workers, "hbm_budget_gb": budget_gb, "by_rate": out}
+
+
 def cpu_baseline(A_host, seed_rounds, threads, lra_omega):
     """The oracle (CPU restatement of the reference: numpy/OpenBLAS dgemm + OpenBLAS dlaqps
     through geqp3_adap_main!'s loop + dtrsm) on one ID of the same matrix with the same
@@ -160,6 +309,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--mixed", default="bf16x3", help="precision of the mixed form")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1"}))
@@ -182,6 +333,14 @@ def main():
     err64 = id_error(lra, A, V64)
     errmx = id_error(lra, A, Vmx)
     roof, roof_gemm = kernel_roofline(lra, torch, A)
+
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(lra, torch)
+        print(f"# batched: {batched['value']} IDs/s", file=sys.stderr, flush=True)
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(lra, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -232,6 +391,8 @@ def main():
         "roofline": roof,
         "roofline_sketch_gemm": roof_gemm,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
 
