@@ -159,6 +159,200 @@ __global__ __launch_bounds__(SRFT_THREADS) void srft_kernel(
   }
 }
 
+// ---- SRFT for power-of-two l <= 256: two register passes of length <= 16 ----------------
+// e^{-2 pi i k / 16}, k = 0..15 (cos, sin)
+__device__ __constant__ double kW16c[16] = {
+    1.0, 0.92387953251128674, 0.70710678118654757, 0.38268343236508978, 0.0,
+    -0.38268343236508978, -0.70710678118654757, -0.92387953251128674, -1.0,
+    -0.92387953251128674, -0.70710678118654757, -0.38268343236508978, 0.0,
+    0.38268343236508978, 0.70710678118654757, 0.92387953251128674};
+__device__ __constant__ double kW16s[16] = {
+    0.0, -0.38268343236508978, -0.70710678118654757, -0.92387953251128674, -1.0,
+    -0.92387953251128674, -0.70710678118654757, -0.38268343236508978, 0.0,
+    0.38268343236508978, 0.70710678118654757, 0.92387953251128674, 1.0,
+    0.92387953251128674, 0.70710678118654757, 0.38268343236508978};
+
+// in-place DFT of length N <= 16 (radix-2, natural order in and out): x[k] = sum_p x[p] w_N^kp
+template <int N>
+__device__ __forceinline__ void dft_reg(cplx* x) {
+  if constexpr (N > 1) {
+    cplx e[N / 2], o[N / 2];
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) { e[i] = x[2 * i]; o[i] = x[2 * i + 1]; }
+    dft_reg<N / 2>(e);
+    dft_reg<N / 2>(o);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      const int j = k * (16 / N);
+      const cplx w = {kW16c[j], kW16s[j]};
+      const cplx t = (k == 0) ? o[k] : cmul(w, o[k]);
+      x[k] = {e[k].re + t.re, e[k].im + t.im};
+      x[k + N / 2] = {e[k].re - t.re, e[k].im - t.im};
+    }
+  }
+}
+
+// One workgroup per column of A.  The column is the l x mm matrix X[p, q] = (d.*x)[p mm + q];
+// slabs of 2 QC columns q are packed as QC complex sequences.  With l = L1 L2 and p = p1 L2 +
+// p2, k = k1 + L1 k2: pass 1 (a thread per (sequence, p2)) loads its L1 inputs straight from
+// HBM, transforms them in registers and applies w_l^(k1 p2); pass 2 (a thread per (sequence,
+// k1)) transforms the L2 values from LDS; every output slot then accumulates its frequency as
+// in srft_kernel.  The next slab's inputs are loaded while the current slab is transformed.
+template <int L1, int L2>
+__global__ __launch_bounds__(256) void srft_fft_kernel(
+    i64 m, const double* __restrict__ A, i64 lda, i64 mm, const double* __restrict__ d,
+    const i64* __restrict__ idx, int ell, double* __restrict__ B, i64 ldb) {
+  constexpr int L = L1 * L2;
+  constexpr int QC = 256 / (L1 > L2 ? L1 : L2);  // sequences per slab
+  extern __shared__ double lds[];
+  cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2][c]
+  cplx* Z = Y + (size_t)L * QC;           // [k][c]
+  cplx* tw = Z + (size_t)L * QC;          // w_l^j, j < l
+  const int tid = threadIdx.x;
+  const i64 col = blockIdx.x;
+  const double* x = A + col * lda;
+  for (int j = tid; j < L; j += 256) {
+    double sn, cs;
+    sincospi(-2.0 * (double)j / (double)L, &sn, &cs);
+    tw[j] = {cs, sn};
+  }
+  const int nslots = (ell + 1) / 2;
+  constexpr int MAXS = 4;
+  cplx acc[MAXS], wstep[MAXS];
+  int fcol[MAXS], fconj[MAXS];
+  i64 ff[MAXS];
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) {
+    const int s = tid + u * 256;
+    acc[u] = {0.0, 0.0};
+    fcol[u] = fconj[u] = 0;
+    ff[u] = 0;
+    wstep[u] = {1.0, 0.0};
+    if (s < nslots) {
+      const i64 f = idx[2 * s] - 1;
+      ff[u] = f;
+      fcol[u] = (int)(f % L);
+      fconj[u] = (int)((L - f % L) % L);
+      double sn, cs;
+      sincospi(-2.0 * (double)f / (double)m, &sn, &cs);
+      wstep[u] = {cs, sn};
+    }
+  }
+  // pass-1 role: sequence c1, column p2 of the reshape
+  const bool p1act = tid < QC * L2;
+  const int c1 = tid % QC, p2 = tid / QC;
+  // pass-2 role: sequence c2, frequency k1
+  const bool p2act = tid < QC * L1;
+  const int c2 = tid % QC, k1 = tid / QC;
+  const i64 nq = mm;
+  cplx in[L1];
+  // (every load is unconditional, from a clamped address, with the mask applied afterwards:
+  // loads under branches would each wait for the previous one)
+  auto load = [&](i64 q0) {
+    const i64 q = q0 + 2 * c1;
+    const i64 qa = min(q, nq - 1), qb = min(q + 1, nq - 1);
+    const bool ma = p1act && q < nq, mb = p1act && q + 1 < nq;
+    const int pr = p1act ? p2 : 0;
+#pragma unroll
+    for (int a = 0; a < L1; ++a) {
+      const i64 base = (i64)(a * L2 + pr) * mm;
+      const double x0 = d[base + qa] * x[base + qa];
+      const double x1 = d[base + qb] * x[base + qb];
+      in[a] = {ma ? x0 : 0.0, mb ? x1 : 0.0};
+    }
+  };
+  load(0);
+  for (i64 q0 = 0; q0 < nq; q0 += 2 * QC) {
+    // pass 1
+    if (p1act) {
+      cplx v[L1];
+#pragma unroll
+      for (int a = 0; a < L1; ++a) v[a] = in[a];
+      dft_reg<L1>(v);
+#pragma unroll
+      for (int a = 0; a < L1; ++a) {
+        const cplx t = (a * p2 == 0) ? v[a] : cmul(tw[(a * p2) % L], v[a]);
+        Y[((size_t)a * L2 + p2) * QC + c1] = t;
+      }
+    }
+    if (q0 + 2 * QC < nq) load(q0 + 2 * QC);  // next slab, in flight during this one
+    __syncthreads();
+    // pass 2
+    if (p2act) {
+      cplx v[L2];
+#pragma unroll
+      for (int b = 0; b < L2; ++b) v[b] = Y[((size_t)k1 * L2 + b) * QC + c2];
+      dft_reg<L2>(v);
+#pragma unroll
+      for (int b = 0; b < L2; ++b) Z[((size_t)k1 + L1 * b) * QC + c2] = v[b];
+    }
+    __syncthreads();
+    // accumulate every output frequency (unpacking the two real spectra of a sequence)
+#pragma unroll
+    for (int u = 0; u < MAXS; ++u) {
+      const int s = tid + u * 256;
+      if (s >= nslots) continue;
+      const i64 e0 = (ff[u] * (q0 % m)) % m;
+      double sn, cs;
+      sincospi(-2.0 * (double)e0 / (double)m, &sn, &cs);
+      cplx wq = {cs, sn};
+      cplx a = acc[u];
+      for (int c = 0; c < QC; ++c) {
+        const i64 q = q0 + 2 * c;
+        if (q >= nq) break;
+        const cplx zc = Z[(size_t)fcol[u] * QC + c];
+        const cplx zn = Z[(size_t)fconj[u] * QC + c];
+        const cplx x1 = {0.5 * (zc.re + zn.re), 0.5 * (zc.im - zn.im)};
+        const cplx x2 = {0.5 * (zc.im + zn.im), -0.5 * (zc.re - zn.re)};
+        cplx t = cmul(wq, x1);
+        a.re += t.re; a.im += t.im;
+        wq = cmul(wq, wstep[u]);
+        if (q + 1 < nq) {
+          t = cmul(wq, x2);
+          a.re += t.re; a.im += t.im;
+        }
+        wq = cmul(wq, wstep[u]);
+      }
+      acc[u] = a;
+    }
+    // (the next slab's pass 2 overwrites Z only after its first barrier, which every thread
+    // reaches after finishing this accumulation)
+  }
+#pragma unroll
+  for (int u = 0; u < MAXS; ++u) {
+    const int s = tid + u * 256;
+    if (s >= nslots) continue;
+    double* y = B + col * ldb;
+    y[2 * s] = acc[u].re;
+    if (2 * s + 1 < ell) y[2 * s + 1] = acc[u].im;
+  }
+}
+
+template <int L1, int L2>
+int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
+                    const double* d, const i64* idx, int ell, double* B, i64 ldb) {
+  constexpr int L = L1 * L2;
+  constexpr int QC = 256 / (L1 > L2 ? L1 : L2);
+  const size_t bytes = (size_t)2 * L * QC * 16 + (size_t)L * 16;
+  if (bytes > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)srft_fft_kernel<L1, L2>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)bytes);
+    if (e != hipSuccess) {
+      lra_set_error("srft: LDS attribute: %s", hipGetErrorString(e));
+      return LRA_E_HIP;
+    }
+  }
+  hipLaunchKernelGGL((srft_fft_kernel<L1, L2>), dim3((unsigned)Q), dim3(256), bytes, st, P, X,
+                     ldx, mm, d, idx, ell, B, ldb);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    lra_set_error("srft launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
 // B[i, j] = A[r[i]-1, j]
 __global__ void subset_kernel(i64 n, const double* A, i64 lda, int ell, const i64* r, double* B,
                               i64 ldb) {
@@ -238,6 +432,17 @@ int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
   while (P % l) --l;
   const i64 mm = P / l;
   const bool pow2 = (l & (l - 1)) == 0;
+  if (pow2 && ell <= 2048) {
+    // two register passes for power-of-two l <= 256
+    switch (l) {
+      case 256: return launch_srft_fft<16, 16>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      case 128: return launch_srft_fft<16, 8>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      case 64: return launch_srft_fft<8, 8>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      case 32: return launch_srft_fft<8, 4>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      case 16: return launch_srft_fft<4, 4>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      default: break;
+    }
+  }
   int logl = 0;
   while ((1ll << logl) < l) ++logl;
   // packed complex columns per slab: LDS = l*qc*16 (+ l*qc*16 for the DFT path) + twiddles

@@ -27,7 +27,8 @@ def srft_call(lra, A, ell, d, idx, trans="n"):
 
 
 @pytest.mark.parametrize("m,n,ell", [(4096, 20, 40), (1000, 30, 40), (97, 5, 6), (12, 3, 5),
-                                     (2048, 9, 310), (1536, 4, 73)])
+                                     (2048, 9, 310), (1536, 4, 73), (2048, 5, 17),
+                                     (896, 7, 130), (4100, 3, 40)])
 def test_srft_injected_vs_oracle(gpu, orc, m, n, ell):
     rng = np.random.default_rng(m + ell)
     A = np.asfortranarray(rng.standard_normal((m, n)))
