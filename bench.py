@@ -29,6 +29,13 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (`bench.py
+# --profile-kernels`), calibrated on a 1 GiB copy (fetch x2.000, write x1.000); the commands
+# and raw counters are in profiles/hbm_traffic_counters_qrcp_and_batched.md.
+TRAFFIC_QRCP_FINAL = 0.698e9      # qrcp_kernel<36, 512, true>, 520 x 8192 sketch, k = 436
+TRAFFIC_BATCHED = 12.48e9         # the three id_batched_kernel launches of one batch
+TRAFFIC_SOURCE = "profiles/hbm_traffic_counters_qrcp_and_batched.md"
+
 
 def load_pkg():
     name = "lowrankapprox_jl_amd"
@@ -124,7 +131,8 @@ def kernel_roofline(lra, torch, A, reps=5):
     g = gflops / (ms_g * 1e-3) / 1e12
     return ({"kernel": f"qrcp_kernel (pivoted QR of the {ell}x{n} sketch, {k} steps)",
              "bound": "mfma", "achieved": round(q, 3), "peak": PEAK_FP64_TFLOPS,
-             "unit": "TFLOP/s", "frac": round(q / PEAK_FP64_TFLOPS, 5), "traffic": None,
+             "unit": "TFLOP/s", "frac": round(q / PEAK_FP64_TFLOPS, 5),
+             "traffic": TRAFFIC_QRCP_FINAL, "traffic_source": TRAFFIC_SOURCE,
              "ms_per_launch": round(ms_q, 4), "flops_per_launch": qflops,
              "note": "latency-bound: one grid-wide argmax + reflector exchange per pivot; "
                      f"{1e3 * ms_q / max(k, 1):.2f} us per step"},
@@ -199,7 +207,8 @@ def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
             "speedup_vs_fp64_dropin": round(ids_s / drop, 1),
             "roofline": {"kernel": "id_batched_kernel (all rounds)", "bound": "hbm",
                          "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": TRAFFIC_BATCHED,
+                         "traffic_source": TRAFFIC_SOURCE,
                          "bytes_per_launch": bytes_alg, "mfma_flops_per_launch": flops,
                          "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 2)},
             "precision_note": "FP64 sketch: at rtol=1e-8 an FP32 sketch cannot certify the "
@@ -336,6 +345,32 @@ def bench_srft(lra, torch, reps: int = 5, m: int = 65536, n: int = 4096, r: int 
     return out
 
 
+def profile_kernels(lra, torch, n: int):
+    """Launch sequence for `rocprofv3 --pmc` runs: a plain copy of 1 GiB (its counters
+    calibrate bytes), then one final-round qrcp_kernel (520 x n sketch, rtol 1e-10), then one
+    batched launch (4096 blocks, rtol 1e-8)."""
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    A, _ = make_matrix(torch, n)
+    B = lra.sketch("left", "n", A, 520, seed=5, stream=4)
+    torch.cuda.synchronize()
+    k = lra.geqp3_adap(B, -1, 0.0, 1e-10, 32)[2]
+    torch.cuda.synchronize()
+    del A, B
+    blocks, _ = kernel_blocks(torch)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan = lra.BatchedID(blocks, rtol=1e-8)
+        plan.launch()
+        plan.finish()
+        plan.close()
+    torch.cuda.synchronize()
+    print(f"# profiled: copy 1 GiB, qrcp 520x{n} (k={k}), batched 4096 blocks", file=sys.stderr)
+
+
 def cpu_baseline(A_host, seed_rounds, threads, lra_omega):
     """The oracle (CPU restatement of the reference: numpy/OpenBLAS dgemm + OpenBLAS dlaqps
     through geqp3_adap_main!'s loop + dtrsm) on one ID of the same matrix with the same
@@ -367,6 +402,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-srft", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch, for rocprofv3 counter runs: a 1 GiB device copy "
+                         "(calibration), the final-round pivoted QR of the 8192^2 workload, "
+                         "and one batched launch of the 4096-block workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1"}))
@@ -374,6 +413,9 @@ def main():
     import torch
     lra = load_pkg()
     torch.cuda.set_device(0)
+    if args.profile_kernels:
+        profile_kernels(lra, torch, args.n)
+        return
     A, sig = make_matrix(torch, args.n)
     torch.cuda.synchronize()
     print(f"# matrix ready {args.n}x{args.n}", file=sys.stderr, flush=True)
