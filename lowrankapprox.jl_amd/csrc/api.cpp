@@ -251,12 +251,27 @@ int lra_check_device_error(lra_ctx* c) {
 }
 
 // synchronous read of device scalars through the pinned staging area
+// (the device error word comes back in the same synchronisation)
 template <class T>
 static int read_scalars(lra_ctx* c, const T* dev, T* host, int count) {
-  LRA_HIP(hipMemcpyAsync(c->host_pinned, dev, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+  char* hp = static_cast<char*>(c->host_pinned);
+  if (sizeof(T) * count > 2048) {
+    lra_set_error("read_scalars: %d values exceed the staging area", count);
+    return LRA_E_UNSUPPORTED;
+  }
+  LRA_HIP(hipMemcpyAsync(hp, dev, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+  LRA_HIP(hipMemcpyAsync(hp + 2048, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   LRA_HIP(hipStreamSynchronize(c->stream));
-  std::memcpy(host, c->host_pinned, sizeof(T) * count);
-  return lra_check_device_error(c);
+  std::memcpy(host, hp, sizeof(T) * count);
+  int h = 0;
+  std::memcpy(&h, hp + 2048, sizeof(int));
+  if (h != 0) {
+    LRA_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    LRA_HIP(hipStreamSynchronize(c->stream));
+    lra_set_error("device error %d (a bounded wait expired in a cooperative kernel)", h);
+    return h;
+  }
+  return LRA_OK;
 }
 
 // ---------------------------------------------------------------------------------------
