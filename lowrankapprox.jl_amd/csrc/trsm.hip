@@ -68,30 +68,60 @@ __global__ __launch_bounds__(256) void id_T_mfma_kernel(i64 k, i64 nr, const dou
     }
     __syncthreads();
     if (wave == 0 && lane < NC) {
+      // the column's 16 rows in registers (compile-time indices)
       double* x = slab + r0 + lane * ld;
+      double xv[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) xv[i] = x[i];
+#pragma unroll
       for (int i = 15; i >= 0; --i) {
-        const double xi = x[i] / s_rbb[i][i];
-        x[i] = xi;
-        for (int r = 0; r < i; ++r) x[r] -= s_rbb[r][i] * xi;
+        const double xi = xv[i] / s_rbb[i][i];
+        xv[i] = xi;
+#pragma unroll
+        for (int r = 0; r < i; ++r) xv[r] -= s_rbb[r][i] * xi;
       }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = xv[i];
     }
     __syncthreads();
-    // S(0:r0, :) -= R(0:r0, r0:r0+16) * X(r0:r0+16, :)
+    // S(0:r0, :) -= R(0:r0, r0:r0+16) * X(r0:r0+16, :).  Wave w takes row tiles w, w+4, ...:
+    // it loads all its R operands first (one memory latency per block instead of one per
+    // tile), then runs the MFMAs for every column tile.
     const int ntiles = r0 / 16;
-    for (int tt = wave; tt < ntiles * (NC / 16); tt += 4) {
-      const int t = tt / (NC / 16), ct = tt % (NC / 16);
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
+    constexpr int MAXT = 8;
+    for (int tb = wave; tb < ntiles; tb += 4 * MAXT) {
+      double av[MAXT][4];
 #pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const i64 gi = (i64)t * 16 + (lane & 15), gj = r0 + kk + (lane >> 4);
-        const double a = gj < k ? R[gi + gj * ldr] : 0.0;  // gi < r0 <= gj: never diagonal
-        const double b = slab[(r0 + kk + (lane >> 4)) + (ct * 16 + (lane & 15)) * ld];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      for (int j = 0; j < MAXT; ++j) {
+        const int t = tb + 4 * j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const i64 gi = (i64)t * 16 + (lane & 15), gj = r0 + 4 * q + (lane >> 4);
+          // gi < r0 <= gj: never diagonal.  Unconditional load from a clamped address (a
+          // load under a branch would wait for the one before it), masked afterwards.
+          const bool ok = t < ntiles && gj < k;
+          const double v = R[(ok ? gi : 0) + (ok ? gj : 0) * ldr];
+          av[j][q] = ok ? v : 0.0;
+        }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double* d = slab + (t * 16 + (lane >> 4) + 4 * r) + (ct * 16 + (lane & 15)) * ld;
-        *d -= acc[r];
+      for (int j = 0; j < MAXT; ++j) {
+        const int t = tb + 4 * j;
+        if (t >= ntiles) break;
+#pragma unroll
+        for (int ct = 0; ct < NC / 16; ++ct) {
+          d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double b = slab[(r0 + 4 * q + (lane >> 4)) + (ct * 16 + (lane & 15)) * ld];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j][q], b, acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double* d = slab + (t * 16 + (lane >> 4) + 4 * r) + (ct * 16 + (lane & 15)) * ld;
+            *d -= acc[r];
+          }
+        }
       }
     }
     __syncthreads();
