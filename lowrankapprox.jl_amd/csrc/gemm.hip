@@ -406,10 +406,11 @@ struct Cfg { int id, BM, BN; };
 static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 256}, {4, 128, 128},
                             {5, 16, 256}, {6, 32, 256}, {7, 48, 256}, {8, 64, 256}, {9, 128, 128},
                             {10, 64, 128}, {11, 128, 256}, {12, 128, 128}, {13, 64, 256},
-                            {14, 256, 128}};
-constexpr int kNumCfg = 15;
-// ids 5..14: the k-contiguous FP64 kernel (gemm_kc_f64_kernel; 11, 13, 14 with 512 threads,
-// 12 with 32-wide k-tiles)
+                            {14, 256, 128}, {15, 80, 256}, {16, 96, 256}, {17, 160, 256}};
+constexpr int kNumCfg = 18;
+// ids 5..17: the k-contiguous FP64 kernel (gemm_kc_f64_kernel; 11, 13, 14, 16, 17 with 512
+// threads, 12 with 32-wide k-tiles).  15-17 hold a short sketch (ell = 72, 136) in one M-tile,
+// so each k-tile of A is read from L2 once.
 bool kc_cfg(int id) { return id >= 5; }
 bool kc_ok(const GemmArgs& g) {
   return g.prec == LRA_PREC_F64 && g.transA == 1 && g.transB == 0 && g.bgather == nullptr &&
@@ -435,6 +436,9 @@ hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
     case 12: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 32>), dim3(blocks), dim3(256), 0, st, d); break;
     case 13: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
     case 14: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<256, 128, 4, 2>), dim3(blocks), dim3(512), 0, st, d); break;
+    case 15: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<80, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
+    case 16: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<96, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
+    case 17: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<160, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
