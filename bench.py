@@ -136,7 +136,8 @@ def kernel_roofline(lra, torch, A, reps=5):
              "ms_per_launch": round(ms_q, 4), "flops_per_launch": qflops,
              "note": "latency-bound: one grid-wide argmax + reflector exchange per pivot; "
                      f"{1e3 * ms_q / max(k, 1):.2f} us per step"},
-            {"kernel": f"gemm_kernel<F64> (sketch {ell}x{n}x{n})", "bound": "mfma",
+            {"kernel": f"gemm_kc_f64_kernel, autotuned tile (sketch {ell}x{n}x{n})",
+             "bound": "mfma",
              "achieved": round(g, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
              "frac": round(g / PEAK_FP64_TFLOPS, 4), "ms_per_launch": round(ms_g, 4),
              "flops_per_launch": gflops})
