@@ -81,7 +81,7 @@ enum WsSlot {
   WS_T, WS_TMP0, WS_TMP1, WS_TMP2, WS_TMP3, WS_LP0, WS_LP1, WS_BATCH0, WS_BATCH1,
   WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
   WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
-  WS_PI0, WS_PI1, WS_PI2, WS_COUNT
+  WS_PI0, WS_PI1, WS_PI2, WS_QG, WS_COUNT
 };
 static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
 
@@ -263,6 +263,9 @@ struct QrcpCall {
   int* early_out;            // device scalar: 1 when stop_after fired
 };
 int lra_qrcp(lra_ctx* ctx, const QrcpCall& q);
+// qrcp_global.hip: the same (mode 0) with the sketch in HBM, for sketches beyond the
+// register-resident kernel
+int lra_qrcp_global(lra_ctx* ctx, const QrcpCall& q);
 
 // trsm.hip: T = R11 \ R12; in-place op(U) X = B with U the upper k x k triangle of R
 int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt);

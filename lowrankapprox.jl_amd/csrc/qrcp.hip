@@ -697,10 +697,11 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
     return LRA_E_UNSUPPORTED;
   }
   QrcpShape sh;
-  if (!choose_shape(ctx, q.m, q.n, &sh)) {
-    lra_set_error("qrcp: panel %d x %d does not fit the register-resident engine", q.m, q.n);
-    return LRA_E_UNSUPPORTED;
-  }
+  // LRA_QRCP_GLOBAL=1 sends every mode-0 call to the global-memory path (used by the tests)
+  const char* fg = getenv("LRA_QRCP_GLOBAL");
+  const bool force_global = fg && fg[0] == '1';
+  if ((force_global && q.mode == 0 && q.offset == 0) || !choose_shape(ctx, q.m, q.n, &sh))
+    return lra_qrcp_global(ctx, q);
   QrcpDev d;
   d.A = q.A; d.lda = q.lda; d.m = q.m; d.n = q.n; d.offset = q.offset; d.kmax = q.kmax;
   d.nb = q.nb; d.mode = q.mode; d.stop_after = q.stop_after;

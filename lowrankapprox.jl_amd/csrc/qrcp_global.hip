@@ -1,0 +1,317 @@
+// qrcp_global.hip -- the rank-adaptive pivoted QR (geqp3_adap!, src/pqr.jl:348-418, with the
+// dlaqps steps of src/lapack.jl:117-139) for sketches too large for the register-resident
+// kernel of qrcp.hip (e.g. order 1032 or 2056 on thousands of columns).
+//
+// The sketch stays in HBM (column-major, in place, LAPACK layout) and each pivot step is three
+// launches on the context's stream, with every decision kept on the device:
+//   qg_pick   one workgroup: dlaqps block bookkeeping and the stop rule for the step just
+//             finished, then the first position of the largest partial norm (idamax);
+//   qg_refl   one workgroup: LAPACK's column swap (data, jpvt, vn1, vn2), dlarfg on the pivot
+//             column (R's diagonal in place, v below it, a contiguous copy of v);
+//   qg_apply  one wave per trailing column: H applied, LAWN 176 downdate; a column whose
+//             downdate fails the tol3z test gets its norm recomputed and ends the block.
+// Once the stop rule fires, the remaining launches return at once.  The arithmetic and the
+// control follow qrcp_kernel, so both give the same pivots up to rounding.
+#include "lra_internal.h"
+#include "lra_wave.h"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+namespace {
+
+struct GState {
+  int s, j0, jb, done, k, early, pvt, flag;
+  double ptol;
+};
+
+constexpr int GT = 1024;  // threads of the single-workgroup kernels
+
+// column norms (one wave per column), jpvt = 1..n
+__global__ __launch_bounds__(256) void qg_norms(int m, int n, const double* __restrict__ B,
+                                                i64 ldb, double* vn1, double* vn2, i64* jpvt) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double* c = B + (i64)j * ldb;
+  double ss = 0.0;
+  for (int i = lane; i < m; i += 64) ss += c[i] * c[i];
+  ss = group_sum(ss, 64);
+  if (lane == 0) {
+    vn1[j] = vn2[j] = sqrt(ss);
+    jpvt[j] = j + 1;
+  }
+}
+
+// ptol = max(atol, rtol * max_j vn1[j]); state reset
+__global__ __launch_bounds__(GT) void qg_init(int n, const double* vn1, double atol, double rtol,
+                                              int kmax, int nb, GState* st) {
+  __shared__ double sm[GT / 64];
+  double mx = 0.0;
+  for (int j = threadIdx.x; j < n; j += GT) mx = fmax(mx, vn1[j]);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m2 = sm[0];
+    for (int w = 1; w < GT / 64; ++w) m2 = fmax(m2, sm[w]);
+    st->ptol = fmax(atol, rtol * m2);
+    st->s = 0;
+    st->j0 = 0;
+    st->jb = min(nb, kmax);
+    st->done = kmax <= 0 ? 1 : 0;
+    st->k = kmax > 0 ? kmax : 0;
+    st->early = 0;
+    st->pvt = 0;
+    st->flag = 0;
+  }
+}
+
+// step s: bookkeeping of step s-1, then the pivot of step s
+__global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, int stop_after,
+                                              const double* vn1, const double* diag,
+                                              GState* st) {
+  __shared__ int s_done;
+  __shared__ double sv[GT / 64];
+  __shared__ int sp[GT / 64];
+  if (threadIdx.x == 0) {
+    int done = st->done;
+    if (!done && s > 0) {
+      const int last = s - 1;
+      const bool blk_end = (last - st->j0 + 1 == st->jb) || (st->flag != 0) || s == kmax;
+      if (blk_end) {
+        const double ptol = st->ptol;
+        bool stop = false;
+        if (fabs(diag[last]) <= ptol) {
+          int i = st->j0;
+          for (; i <= last; ++i)
+            if (fabs(diag[i]) <= ptol) break;
+          if (i <= last) { st->k = i; stop = true; }
+        }
+        if (!stop && s == kmax) { st->k = kmax; stop = true; }
+        if (!stop && stop_after > 0 && s >= stop_after) {
+          st->k = s;
+          st->early = 1;
+          stop = true;
+        }
+        if (stop) {
+          st->done = done = 1;
+        } else {
+          st->j0 = s;
+          st->jb = min(nb, kmax - s);
+        }
+      }
+    }
+    if (!done && s == kmax) { st->k = kmax; st->done = done = 1; }
+    st->flag = 0;
+    s_done = done;
+  }
+  __syncthreads();
+  if (s_done) return;
+  // idamax over positions s..n-1: largest vn1, then lowest position
+  double bv = -1.0;
+  int bp = INT_MAX;
+  for (int j = s + threadIdx.x; j < n; j += GT) {
+    const double v = vn1[j];
+    if (v > bv) { bv = v; bp = j; }  // strided order: the first max of this thread's share
+  }
+  double wv;
+  const int wl = wave_argmax(bv, bp, 1, &wv);
+  const int wp = __builtin_amdgcn_readlane(bp, wl < 0 ? 0 : wl);
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = wv;
+    sp[threadIdx.x >> 6] = wl < 0 ? INT_MAX : wp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double cv = sv[0];
+    int cp = sp[0];
+    for (int w = 1; w < GT / 64; ++w)
+      if (sv[w] > cv || (sv[w] == cv && sp[w] < cp)) { cv = sv[w]; cp = sp[w]; }
+    st->pvt = cp;
+    st->s = s;
+  }
+}
+
+// step s: swap columns s and pvt, build H(s) (dlarfg) in place, copy v
+__global__ __launch_bounds__(GT) void qg_refl(int s, int m, double* __restrict__ B, i64 ldb,
+                                              i64* jpvt, double* vn1, double* vn2, double* tau,
+                                              double* diag, double* v, const GState* st) {
+  __shared__ double sw[GT / 64];
+  __shared__ double s_alpha;
+  if (st->done) return;
+  const int pvt = st->pvt;
+  double* cs = B + (i64)s * ldb;
+  if (pvt != s) {
+    double* cp = B + (i64)pvt * ldb;
+    for (int i = threadIdx.x; i < m; i += GT) {
+      const double t = cs[i];
+      cs[i] = cp[i];
+      cp[i] = t;
+    }
+    if (threadIdx.x == 0) {
+      const i64 t = jpvt[pvt];
+      jpvt[pvt] = jpvt[s];
+      jpvt[s] = t;
+      vn1[pvt] = vn1[s];
+      vn2[pvt] = vn2[s];
+    }
+  }
+  __syncthreads();
+  // dlarfg on cs[s..m-1]
+  double ss = 0.0;
+  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) ss += cs[i] * cs[i];
+  ss = group_sum(ss, 64);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = ss;
+  if (threadIdx.x == 0) s_alpha = cs[s];
+  __syncthreads();
+  double tot = 0.0;
+  for (int w = 0; w < GT / 64; ++w) tot += sw[w];
+  double alpha = s_alpha, xnorm = sqrt(tot), beta = alpha, t = 0.0, scal = 1.0, fac = 1.0;
+  if (m - s > 1 && xnorm != 0.0) {
+    beta = -copysign(lapy2(alpha, xnorm), alpha);
+    const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+    int knt = 0;
+    if (fabs(beta) < safmin) {
+      const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+      do {
+        ++knt;
+        fac *= rsafmn;
+        beta *= rsafmn;
+        alpha *= rsafmn;
+      } while (fabs(beta) < safmin && knt < 20);
+      __syncthreads();  // every thread has read sw
+      double s2 = 0.0;
+      for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
+        const double x = cs[i] * fac;
+        s2 += x * x;
+      }
+      s2 = group_sum(s2, 64);
+      if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = s2;
+      __syncthreads();
+      double t2 = 0.0;
+      for (int w = 0; w < GT / 64; ++w) t2 += sw[w];
+      xnorm = sqrt(t2);
+      beta = -copysign(lapy2(alpha, xnorm), alpha);
+    }
+    t = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+    for (int j = 0; j < knt; ++j) beta *= safmin;
+  }
+  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
+    const double x = (cs[i] * fac) * scal;
+    cs[i] = x;
+    v[i] = x;
+  }
+  if (threadIdx.x == 0) {
+    cs[s] = beta;
+    v[s] = 1.0;
+    tau[s] = t;
+    diag[s] = beta;
+  }
+}
+
+// step s: columns s+1..n-1 (one wave each): apply H(s), downdate the partial norms
+__global__ __launch_bounds__(256) void qg_apply(int s, int m, int n, int lastrk,
+                                                double* __restrict__ B, i64 ldb,
+                                                const double* __restrict__ v,
+                                                const double* __restrict__ tau, double* vn1,
+                                                double* vn2, GState* st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const int j = s + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
+  double* c = B + (i64)j * ldb;
+  double w = 0.0;
+  for (int i = s + lane; i < m; i += 64) w += v[i] * c[i];
+  w = group_sum(w, 64);
+  const double tw = tau[s] * w;
+  // update; each lane also sums the squares of its updated rows below s (the recompute norm)
+  // and lane 0 keeps the new R(s, j), so nothing is read back from memory
+  double ss = 0.0, x = 0.0;
+  for (int i = s + lane; i < m; i += 64) {
+    const double y = c[i] - tw * v[i];
+    c[i] = y;
+    if (i == s) x = y;
+    else ss += y * y;
+  }
+  x = __shfl(x, 0, 64);
+  double a = vn1[j];
+  if (s + 1 < lastrk && a != 0.0) {
+    double temp = fabs(x) / a;
+    temp = (1.0 + temp) * (1.0 - temp);
+    temp = temp < 0.0 ? 0.0 : temp;
+    const double q = a / vn2[j];
+    const double temp2 = temp * q * q;
+    if (temp2 <= tol3z) {
+      ss = group_sum(ss, 64);
+      if (lane == 0) {
+        vn1[j] = vn2[j] = sqrt(ss);
+        __hip_atomic_store(&st->flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else if (lane == 0) {
+      vn1[j] = a * sqrt(temp);
+    }
+  }
+}
+
+__global__ void qg_finish(const GState* st, i64* k_out, int* early_out) {
+  *k_out = st->k;
+  if (early_out) *early_out = st->early;
+}
+
+}  // namespace
+
+// geqp3_adap (mode 0 of QrcpCall) on a sketch of any size, in HBM.
+int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
+  if (q.mode != 0) {
+    lra_set_error("qrcp: laqps panel %d x %d does not fit the register-resident engine", q.m,
+                  q.n);
+    return LRA_E_UNSUPPORTED;
+  }
+  if (q.offset != 0) {
+    lra_set_error("qrcp: offset %d on the global-memory path", q.offset);
+    return LRA_E_UNSUPPORTED;
+  }
+  const int m = q.m, n = q.n;
+  double* w = nullptr;
+  // vn1, vn2, diag (kmax), v (m), state
+  const size_t nd = 2 * (size_t)n + (size_t)std::max(q.kmax, 1) + (size_t)m + 16;
+  LRA_CHECK(lra_wsT(c, WS_QG, nd, &w));
+  double* vn1 = w;
+  double* vn2 = vn1 + n;
+  double* diag = vn2 + n;
+  double* v = diag + std::max(q.kmax, 1);
+  GState* st = reinterpret_cast<GState*>(v + m);
+  const int lastrk = std::min(m, n);
+  hipStream_t s0 = c->stream;
+  hipLaunchKernelGGL(qg_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, q.A, q.lda,
+                     vn1, vn2, q.jpvt);
+  hipLaunchKernelGGL(qg_init, dim3(1), dim3(GT), 0, s0, n, vn1, q.atol, q.rtol, q.kmax, q.nb,
+                     st);
+  LRA_HIP(hipGetLastError());
+  for (int s = 0; s <= q.kmax; ++s) {
+    hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, q.kmax, q.nb, q.stop_after, vn1,
+                       diag, st);
+    if (s == q.kmax) break;
+    hipLaunchKernelGGL(qg_refl, dim3(1), dim3(GT), 0, s0, s, m, q.A, q.lda, q.jpvt, vn1, vn2,
+                       q.tau, diag, v, st);
+    if (s + 1 < n)
+      hipLaunchKernelGGL(qg_apply, dim3((unsigned)((n - s - 1 + 3) / 4)), dim3(256), 0, s0, s,
+                         m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+    LRA_HIP(hipGetLastError());
+    // (every 64 steps, stop enqueueing once the device has stopped)
+    if ((s & 63) == 63) {
+      int done = 0;
+      LRA_HIP(hipMemcpyAsync(c->host_pinned, &st->done, sizeof(int), hipMemcpyDeviceToHost, s0));
+      LRA_HIP(hipStreamSynchronize(s0));
+      std::memcpy(&done, c->host_pinned, sizeof(int));
+      if (done) break;
+    }
+  }
+  hipLaunchKernelGGL(qg_finish, dim3(1), dim3(1), 0, s0, st, q.k_out, q.early_out);
+  LRA_HIP(hipGetLastError());
+  return LRA_OK;
+}

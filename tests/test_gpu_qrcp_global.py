@@ -1,0 +1,73 @@
+"""The global-memory pivoted QR (sketches beyond the register-resident kernel): forced on the
+oracle cases, against the register-resident kernel on the same sketch, and end to end on a
+wide matrix whose adaptive rounds outgrow the register-resident kernel."""
+import numpy as np
+import pytest
+
+from test_gpu_kernels import first_mismatch_ok, graded, host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("m,n,rtol", [(40, 300, 1e-12), (72, 3000, 1e-10), (264, 2500, 1e-13),
+                                      (12, 5, 0.0), (33, 700, 1e-6), (700, 400, 1e-12)])
+def test_global_qrcp_vs_oracle(gpu, orc, monkeypatch, m, n, rtol):
+    import torch
+    monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
+    B = graded(m, n, m * 7 + n, decay=0.85 if m < 100 else 0.95)
+    print(f"global geqp3_adap m={m} n={n} rtol={rtol}", flush=True)
+    Bf, p_o, tau_o, k_o, gap = orc.geqp3_adap(B, -1, 0.0, rtol, 32, want_gap=True)
+    Bd = gpu.to_device(B)
+    p_g, tau_g, k_g = gpu.geqp3_adap(Bd, -1, 0.0, rtol, 32)
+    torch.cuda.synchronize()
+    p_g, tau_g, Bg = host(p_g), host(tau_g), host(Bd)
+    ok, i = first_mismatch_ok(p_g, p_o, gap)
+    assert ok, f"pivot mismatch at step {i} with gap {gap[i]}"
+    if i == len(p_g):
+        assert k_g == k_o
+        kk = min(k_o, min(m, n))
+        R_g, R_o = np.triu(Bg[:kk, :]), np.triu(Bf[:kk, :])
+        assert np.abs(R_g - R_o).max() <= 1e-10 * np.abs(R_o).max()
+
+
+def test_global_matches_register_kernel(gpu, monkeypatch):
+    """The same 520 x 8192 sketch through both engines: same rank and pivots."""
+    import torch
+    B = graded(520, 8192, 11, decay=0.97)
+    B1, B2 = gpu.to_device(B), gpu.to_device(B)
+    p1, _, k1 = gpu.geqp3_adap(B1, -1, 0.0, 1e-10, 32)
+    monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
+    p2, _, k2 = gpu.geqp3_adap(B2, -1, 0.0, 1e-10, 32)
+    torch.cuda.synchronize()
+    assert k1 == k2
+    assert np.array_equal(host(p1)[:k1], host(p2)[:k2])
+    R1, R2 = np.triu(host(B1)[:k1]), np.triu(host(B2)[:k2])
+    assert np.abs(R1 - R2).max() <= 1e-10 * np.abs(R1).max()
+
+
+def test_wide_matrix_beyond_register_kernel(gpu, orc):
+    """400 x 16384 with rank ~380 at rtol 1e-12: the order-264 and order-520 rounds no longer
+    fit the register-resident kernel (16384 columns), so they run on the global path; checked
+    against the oracle with the same Omegas."""
+    rng = np.random.default_rng(3)
+    m, n, r = 400, 16384, 390
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = 10.0 ** (-14 * np.arange(r) / (r - 1))
+    A = np.asfortranarray((U * s) @ V.T)
+
+    class DeviceOmega:
+        def gaussian(self, rnd, rows, cols):
+            return np.asfortranarray(gpu.gaussian(rows, cols, 4, rnd).cpu().numpy())
+
+    V_g = gpu.idfact(A, rtol=1e-12, seed=4)
+    V_o = orc.idfact("n", A, orc.LRAOptions().copy(rtol=1e-12), rand=DeviceOmega())
+    print(f"wide: k={V_g.k} oracle k={V_o.k} rounds={V_g.rounds}", flush=True)
+    assert V_g.rounds == V_o.rounds and V_g.rounds >= 4
+    assert V_g.k == V_o.k
+    first_bad = next((i for i, (a, b) in enumerate(zip(host(V_g.sk), V_o.sk)) if a != b), None)
+    if first_bad is not None:
+        assert V_o.gap[first_bad] < 1e-10
+    e_g = np.linalg.norm(A[:, host(V_g.rd) - 1] - A[:, host(V_g.sk) - 1] @ host(V_g.T), 2)
+    e_o = np.linalg.norm(A[:, V_o.rd - 1] - A[:, V_o.sk - 1] @ V_o.T, 2)
+    assert e_g <= 10 * max(e_o, 1e-15)
