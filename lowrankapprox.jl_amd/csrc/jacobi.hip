@@ -4,18 +4,30 @@
 // X (m x n, m >= n) is orthogonalised column pair by column pair: for a pair (p, q) with
 // a = |x_p|^2, b = |x_q|^2, g = x_p'x_q, the rotation that diagonalises [a g; g b] is applied
 // to the two columns of X and of V.  A sweep visits every pair once in the round-robin
-// ("circle") order: n/2 disjoint pairs per round, one workgroup per pair, n-1 rounds per
-// sweep; sweeps repeat until a sweep rotates nothing (|g| <= tol sqrt(a b)).  At the end the
-// column norms of X are the singular values, X's normalised columns the left vectors, V the
-// right ones (X_in = U diag(s) V'), sorted by decreasing s.  One-sided Jacobi computes the
-// small singular values to high relative accuracy.
+// ("circle") order; sweeps repeat until a sweep rotates nothing (|g| <= tol sqrt(a b)).  At
+// the end the column norms of X are the singular values, X's normalised columns the left
+// vectors, V the right ones (X_in = U diag(s) V'), sorted by decreasing s.  One-sided Jacobi
+// computes the small singular values to high relative accuracy.
+//
+// Block-cyclic order (jacobi_block_kernel): the columns form NB blocks of W; one launch is a
+// round of the circle order over blocks, a workgroup per block pair.  The workgroup keeps its
+// 2W columns of X in LDS and visits every cross pair of its two blocks (W steps of W disjoint
+// pairs; the first round of a sweep also visits each block's inner pairs), each rotation from
+// dot products of the current columns, exactly as the pairwise method.  The rotations are
+// accumulated in a 2W x 2W matrix Q that updates V's 2W columns once per round.  A sweep
+// (every pair once) is NB - 1 launches instead of n - 1.  Matrices taller than the LDS holds
+// use one launch per round of single pairs (jacobi_round_kernel).
 #include "lra_internal.h"
+#include "lra_wave.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 // pair (p, q) of slot i in round r for N (even) players: player 0 fixed, the others rotate
 LRA_DEV void rr_pair(int N, int r, int i, int* p, int* q) {
@@ -70,6 +82,256 @@ __global__ __launch_bounds__(256) void jacobi_round_kernel(int m, int n, int N, 
   if (threadIdx.x == 0) __hip_atomic_fetch_add(rotated, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sum over the L consecutive lanes of a group (L = 16, 32 or 64); every lane of the group ends
+// with the bit-identical sum (each step adds the same two operands in both partner lanes).
+template <int L>
+LRA_DEV double group_sum_u(double v) {
+  v += dpp_f64<DPP_XOR1>(v);
+  v += dpp_f64<DPP_XOR2>(v);
+  v += dpp_f64<DPP_HMIRROR>(v);
+  v += dpp_f64<DPP_MIRROR>(v);
+  if constexpr (L == 32) v += __shfl_xor(v, 16, 64);
+  if constexpr (L == 64)
+    v = (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+  return v;
+}
+
+// 1/sqrt(x) for x in [1, 2]: the hardware estimate and two Newton steps
+LRA_DEV double rsqrt_12(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  r = r * (1.5 - hx * r * r);
+  r = r * (1.5 - hx * r * r);
+  return r;
+}
+
+// One round r of the circle order over NB blocks of W columns (a sweep's first round, first
+// = 1, also visits the pairs inside each block).  A pair is handled by L lanes (L = 256 / W
+// for W >= 4, so the W pairs of a step fill one wave per SIMD), each holding MR rows in
+// registers; Xs columns are zero-padded to mp = L * MR >= m rows (stride mp + 16).  Dynamic
+// LDS: Xs (2W columns), Qs (2W x 2W, row-major with a padded stride, so a column of Q is read
+// bank-conflict free and a row contiguously).
+template <int W, int MR>
+__global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB, int r, int first,
+                                                           double* __restrict__ X, i64 ldx,
+                                                           double* __restrict__ V, i64 ldv,
+                                                           double tol, int* __restrict__ rotated) {
+  constexpr int C = 2 * W;
+  constexpr int QS = C + 1;                 // row stride of Qs
+  constexpr int NW = 8;                     // waves
+  constexpr int L = W >= 16 ? 16 : W == 8 ? 32 : 64;  // lanes per pair
+  constexpr int G = 64 / L;                 // pairs per wave
+  constexpr int mp = L * MR;                // padded column length in LDS
+  constexpr int mps = mp + 16;              // column stride (128 bytes of skew between columns)
+  extern __shared__ double lds[];
+  double* Xs = lds;
+  double* Qs = lds + (size_t)C * mps;  // Qs[t * QS + j] = Q(t, j)
+  __shared__ int any;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int P, Q;
+  rr_pair(NB, r, blockIdx.x, &P, &Q);
+  if (P > Q) { const int t = P; P = Q; Q = t; }
+  // local slot c -> global column: slots [0, W) are block P, [W, 2W) block Q (P < Q, so the
+  // local order is the global order)
+  auto gcol = [&](int c) { return c < W ? P * W + c : Q * W + (c - W); };
+  // Xs(i, c) = X(i, gcol(c)), zero for padding rows and columns; every load of a thread is
+  // issued before the first store
+  for (int i0 = 0; i0 < mp; i0 += 512) {
+    const int i = i0 + tid;
+    double tmp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int g = gcol(c);
+      const bool ok = i < m && g < n;
+      const double x = X[ok ? (i64)g * ldx + i : 0];
+      tmp[c] = ok ? x : 0.0;
+    }
+    if (i < mp) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) Xs[c * mps + i] = tmp[c];
+    }
+  }
+  for (int e = tid; e < C * QS; e += 512) Qs[e] = (e / QS) == (e % QS) ? 1.0 : 0.0;
+  if (tid == 0) any = 0;
+  __syncthreads();
+  const double tol2 = tol * tol;
+  int nrot = 0;
+  const int grp = lane / L, gl = lane % L;
+  const int pi = wave * G + grp;  // pair index in the step
+  const bool act = pi < W;
+  // steps: W - 1 inner steps (first round only), then W cross steps; every step holds W
+  // disjoint pairs
+  const int ninner = first && W > 1 ? W - 1 : 0;
+  const bool busy = wave * G < W;  // waves that hold pairs (wave-uniform)
+  // rotate the pair (x1 = u, x2 = w) when |g| > tol sqrt(a b); returns whether it did
+  auto pair_step = [&](double (&u)[MR], double (&w)[MR], int c1, int c2, bool store1) -> bool {
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0, g0 = 0.0, g1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < MR; q += 2) {
+      a0 += u[q] * u[q];
+      b0 += w[q] * w[q];
+      g0 += u[q] * w[q];
+      if (q + 1 < MR) {
+        a1 += u[q + 1] * u[q + 1];
+        b1 += w[q + 1] * w[q + 1];
+        g1 += u[q + 1] * w[q + 1];
+      }
+    }
+    const double a = group_sum_u<L>(a0 + a1);
+    const double b = group_sum_u<L>(b0 + b1);
+    const double g = group_sum_u<L>(g0 + g1);
+    // |g| > tol sqrt(a b), without the square roots
+    if (!(act && g != 0.0 && g * g > tol2 * (a * b))) return false;
+    // t = tan(theta), the smaller root of t^2 + 2 zeta t - 1 = 0 with zeta = d / e:
+    // t = sign(d) e / (|d| + sqrt(d^2 + e^2))
+    const double d = b - a, e = 2.0 * g;
+    const double h = sqrt(d * d + e * e);
+    const double t = copysign(1.0, d) * (e / (fabs(d) + h));
+    const double cs = rsqrt_12(1.0 + t * t), sn = cs * t;
+    double* x1 = Xs + c1 * mps + gl;
+    double* x2 = Xs + c2 * mps + gl;
+#pragma unroll
+    for (int q = 0; q < MR; ++q) {
+      const double y = u[q], z = w[q];
+      u[q] = cs * y - sn * z;
+      w[q] = sn * y + cs * z;
+      if (store1) x1[q * L] = u[q];
+      x2[q * L] = w[q];
+    }
+    for (int t2 = gl; t2 < C; t2 += L) {
+      double* q1 = Qs + t2 * QS + c1;
+      double* q2 = Qs + t2 * QS + c2;
+      const double y = *q1, z = *q2;
+      *q1 = cs * y - sn * z;
+      *q2 = sn * y + cs * z;
+    }
+    return true;
+  };
+  double u[MR], w[MR];
+  // inner steps (a sweep's first round): W/2 pairs inside each block, circle order over W
+  // players; both columns go back to LDS
+  for (int step = 0; step < ninner; ++step) {
+    if (busy) {
+      int c1 = 0, c2 = 1;
+      if (act) {
+        constexpr int half = W > 1 ? W / 2 : 1;
+        const int blk = pi / half, slot = pi % half;
+        int a, b;
+        rr_pair(W, step, slot, &a, &b);
+        c1 = blk * W + min(a, b);
+        c2 = blk * W + max(a, b);
+      }
+      const double* x1 = Xs + c1 * mps + gl;
+      const double* x2 = Xs + c2 * mps + gl;
+#pragma unroll
+      for (int q = 0; q < MR; ++q) {
+        u[q] = x1[q * L];
+        w[q] = x2[q * L];
+      }
+      if (pair_step(u, w, c1, c2, true)) ++nrot;
+    }
+    __syncthreads();
+  }
+  // cross steps: pair pi keeps top column pi in registers for the whole round and meets
+  // bottom column W + (pi + s) mod W at step s; only the bottom columns move through LDS
+  const int c1 = act ? pi : 0;
+  if (busy) {
+    const double* x1 = Xs + c1 * mps + gl;
+#pragma unroll
+    for (int q = 0; q < MR; ++q) u[q] = x1[q * L];
+  }
+  bool moved = false;
+  for (int step = 0; step < W; ++step) {
+    if (busy) {
+      const int c2 = act ? W + ((pi + step) & (W - 1)) : 1;
+      const double* x2 = Xs + c2 * mps + gl;
+#pragma unroll
+      for (int q = 0; q < MR; ++q) w[q] = x2[q * L];
+      if (pair_step(u, w, c1, c2, false)) {
+        ++nrot;
+        moved = true;
+      }
+    }
+    __syncthreads();
+  }
+  if (busy && moved) {
+    double* x1 = Xs + c1 * mps + gl;
+#pragma unroll
+    for (int q = 0; q < MR; ++q) x1[q * L] = u[q];
+  }
+  __syncthreads();
+  if (nrot) any = 1;  // benign race: every writer stores 1
+  __syncthreads();
+  if (!any) return;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int g = gcol(c);
+    if (g >= n) continue;
+    double* x = X + (i64)g * ldx;
+    for (int i = tid; i < m; i += 512) x[i] = Xs[c * mps + i];
+  }
+  // V(:, block cols) <- V(:, block cols) * Q on MFMA, as V' <- Q' V': the block of V (n x 2W)
+  // is staged in LDS over Xs; lane l of a 16 x 16 output tile then holds 16 consecutive rows of
+  // one column of V, so the stores are coalesced.
+  __syncthreads();  // Xs has been written out
+  double* Vs = Xs;  // Vs[t * mps + i] = V(i, gcol(t))
+  for (int i0 = 0; i0 < n; i0 += 512) {
+    const int i = i0 + tid;
+    double tmp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int g = gcol(c);
+      const bool ok = i < n && g < n;
+      const double x = V[ok ? (i64)g * ldv + i : 0];
+      tmp[c] = ok ? x : 0.0;
+    }
+    if (i < n) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) Vs[c * mps + i] = tmp[c];
+    }
+  }
+  __syncthreads();
+  {
+    constexpr int KS = (C + 3) / 4;        // k-steps of 4
+    constexpr int CT = (C + 15) / 16;      // 16-column tiles of Q
+    constexpr int WPT = NW / CT;           // waves per column tile
+    const int ct = wave / WPT, rt0 = wave % WPT;
+    const int rtiles = (n + 15) / 16;
+    const int cq = ct * 16 + (lane & 15);  // A = Q' (row cq, k): Q(k, cq)
+    double qa[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * 4 + (lane >> 4);
+      qa[ks] = (k < C && cq < C) ? Qs[k * QS + cq] : 0.0;
+    }
+    for (int rt = rt0; rt < rtiles; rt += WPT) {
+      const int i = rt * 16 + (lane & 15);  // B = V' (k, i): V(i, gcol(k))
+      double vb[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = ks * 4 + (lane >> 4);
+        const bool ok = k < C && i < n;
+        const double x = Vs[ok ? k * mps + i : 0];
+        vb[ks] = ok ? x : 0.0;
+      }
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[ks], vb[ks], acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = ct * 16 + (lane >> 4) + 4 * q;  // output (c, i) = V_new(i, gcol(c))
+        if (c < C && i < n) {
+          const int g = gcol(c);
+          if (g < n) V[i + (i64)g * ldv] = acc[q];
+        }
+      }
+    }
+  }
+  if (wave == 0 && lane == 0)
+    __hip_atomic_fetch_add(rotated, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ void jacobi_eye_kernel(int n, double* V, i64 ldv, int* rotated) {
   const i64 total = (i64)n * n;
   for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
@@ -110,6 +372,86 @@ __global__ void jacobi_finish_kernel(int m, int n, const int* __restrict__ perm,
   if (threadIdx.x == 0) so[j] = sj;
 }
 
+constexpr size_t kJacobiLds = 150 * 1024;
+
+struct BlockCfg {
+  int W = 0, MR = 0;
+  size_t lds = 0;
+};
+
+constexpr int lanes_per_pair(int W) { return W >= 16 ? 16 : W == 8 ? 32 : 64; }
+
+// Widest block (8, 4, 2, 1; 16 on request) whose 2W zero-padded columns of X fit in LDS with
+// at most 32 rows per lane in registers, narrower when the matrix has few columns (at least 4
+// blocks); W = 0 (one launch per round of single pairs) when none does.  At k = 426 a sweep
+// takes 0.95 ms at W = 8 and 1.06 ms at W = 16: the step cost grows with the pairs per wave
+// faster than the round count falls.  LRA_JACOBI_W=<w> caps the width (tests and tuning).
+BlockCfg block_cfg(i64 m, i64 n) {
+  BlockCfg cfg;
+  int want = 8;
+  if (const char* e = getenv("LRA_JACOBI_W")) want = std::max(0, atoi(e));
+  for (int W : {16, 8, 4, 2, 1}) {
+    if (W > want) continue;
+    if (W > 1 && (i64)4 * W > n) continue;
+    const i64 L = lanes_per_pair(W);
+    int MR = 0;
+    for (int b : {4, 8, 16, 24, 28, 32})
+      if (L * b >= m && !((W <= 2) && (b == 24 || b == 28))) { MR = b; break; }
+    if (!MR) continue;
+    const size_t lds = ((size_t)2 * W * (L * MR + 16) + (size_t)2 * W * (2 * W + 1)) * sizeof(double);
+    if (lds > kJacobiLds) continue;
+    cfg.W = W;
+    cfg.MR = MR;
+    cfg.lds = lds;
+    return cfg;
+  }
+  return cfg;
+}
+
+template <int W, int MR>
+void launch_block_wm(int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
+                     double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)jacobi_block_kernel<W, MR>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kJacobiLds);
+    (void)hipGetLastError();
+    attr = true;
+  }
+  hipLaunchKernelGGL((jacobi_block_kernel<W, MR>), dim3((unsigned)(NB / 2)), dim3(512), lds, st,
+                     m, n, NB, r, first, X, ldx, V, ldv, tol, rot);
+}
+
+template <int W>
+void launch_block_w(int MR, int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
+                    double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+  switch (MR) {
+    case 4: launch_block_wm<W, 4>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 8: launch_block_wm<W, 8>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 16: launch_block_wm<W, 16>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 24:
+      if constexpr (W > 2) launch_block_wm<W, 24>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot);
+      break;
+    case 28:
+      if constexpr (W > 2) launch_block_wm<W, 28>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot);
+      break;
+    default: launch_block_wm<W, 32>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+  }
+}
+
+void launch_block(const BlockCfg& cfg, int NB, hipStream_t st, int m, int n, int r, int first,
+                  double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+  const int MR = cfg.MR;
+  const size_t lds = cfg.lds;
+  switch (cfg.W) {
+    case 16: launch_block_w<16>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 8: launch_block_w<8>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 4: launch_block_w<4>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 2: launch_block_w<2>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    default: launch_block_w<1>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+  }
+}
+
 }  // namespace
 
 // X (m x n, m >= n) = U diag(s) V': X is overwritten, U (m x n), s (n, decreasing), Vt-less
@@ -131,12 +473,20 @@ int lra_gesvj_impl(lra_ctx* c, i64 m, i64 n, double* X, i64 ldx, double* U, i64 
                      dim3(256), 0, c->stream, (int)n, Vw, n, rot);
   LRA_HIP(hipGetLastError());
   const double tol = std::sqrt((double)m) * 1.1102230246251565e-16;
+  const BlockCfg cfg = block_cfg(m, n);
+  const int W = cfg.W;
+  const int NB = W ? (int)((n + W - 1) / W + (((n + W - 1) / W) & 1)) : 0;
   int sw_done = 0;
   for (int sweep = 0; sweep < 60; ++sweep) {
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, c->stream, rot);
-    for (int r = 0; r < N - 1; ++r)
-      hipLaunchKernelGGL(jacobi_round_kernel, dim3(N / 2), dim3(256), 0, c->stream, (int)m,
-                         (int)n, N, r, X, ldx, Vw, n, tol, rot);
+    if (W) {
+      for (int r = 0; r < NB - 1; ++r)
+        launch_block(cfg, NB, c->stream, (int)m, (int)n, r, r == 0, X, ldx, Vw, n, tol, rot);
+    } else {
+      for (int r = 0; r < N - 1; ++r)
+        hipLaunchKernelGGL(jacobi_round_kernel, dim3(N / 2), dim3(256), 0, c->stream, (int)m,
+                           (int)n, N, r, X, ldx, Vw, n, tol, rot);
+    }
     LRA_HIP(hipGetLastError());
     int h = 0;
     LRA_HIP(hipMemcpyAsync(&h, rot, sizeof(int), hipMemcpyDeviceToHost, c->stream));

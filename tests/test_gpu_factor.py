@@ -141,20 +141,51 @@ def test_pheig_reference_bound(gpu, orc):
     assert np.linalg.norm(E) < 100 * 5 * EPS * nrm * 10
 
 
-def test_gesvj_vs_numpy(gpu):
+def _gesvj(gpu, X):
     import torch
     L = gpu._lib.load()
     ctx = gpu.context()
+    m, k = X.shape
+    Xd = gpu.to_device(np.asfortranarray(X))
+    U = gpu.empty_colmajor(m, k)
+    V = gpu.empty_colmajor(k, k)
+    s = torch.empty(k, dtype=torch.float64, device="cuda")
+    gpu.check(L.lra_gesvj_f64(ctx.h, m, k, Xd.data_ptr(), m, U.data_ptr(), m, s.data_ptr(),
+                              V.data_ptr(), k), "gesvj")
+    return host(U), host(s), host(V)
+
+
+def _check_svd(X, U, s, V, rtol=1e-11):
+    m, k = X.shape
+    sr = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(s, sr, rtol=rtol, atol=1e-14 * sr[0])
+    assert np.all(np.diff(s) <= 0)
+    Xr = (U * s) @ V.T
+    assert np.abs(Xr - X).max() <= 1e-13 * np.abs(X).max() * k
+    assert np.abs(V.T @ V - np.eye(k)).max() <= 1e-13 * k
+
+
+def test_gesvj_vs_numpy(gpu):
     rng = np.random.default_rng(4)
-    for k in (1, 7, 64, 201):
-        X = np.asfortranarray(rng.standard_normal((k, k)) @ np.diag(10.0 ** -np.linspace(0, 10, k)))
-        Xd = gpu.to_device(X)
-        U = gpu.empty_colmajor(k, k)
-        V = gpu.empty_colmajor(k, k)
-        s = torch.empty(k, dtype=torch.float64, device="cuda")
-        gpu.check(L.lra_gesvj_f64(ctx.h, k, k, Xd.data_ptr(), k, U.data_ptr(), k, s.data_ptr(),
-                                  V.data_ptr(), k), "gesvj")
-        sr = np.linalg.svd(X, compute_uv=False)
-        assert np.allclose(host(s), sr, rtol=1e-11, atol=0)
-        Xr = (host(U) * host(s)) @ host(V).T
-        assert np.abs(Xr - X).max() <= 1e-13 * np.abs(X).max() * k
+    for k in (1, 7, 64, 201, 436):
+        X = rng.standard_normal((k, k)) @ np.diag(10.0 ** -np.linspace(0, 10, k))
+        print(f"gesvj k={k}", flush=True)
+        _check_svd(X, *_gesvj(gpu, X))
+
+
+@pytest.mark.parametrize("w", ["0", "1", "2", "4", "8", "16"])
+def test_gesvj_block_widths(gpu, w, monkeypatch):
+    """Every block width of the block-cyclic order (0: one launch per round of single pairs)
+    on a tall matrix with graded columns, an odd column count and two zero columns."""
+    monkeypatch.setenv("LRA_JACOBI_W", w)
+    rng = np.random.default_rng(11)
+    m, k = 150, 93
+    X = rng.standard_normal((m, k)) @ np.diag(10.0 ** -np.linspace(0, 9, k))
+    X[:, 10] = 0.0
+    X[:, 50] = 0.0
+    print(f"gesvj W={w}", flush=True)
+    U, s, V = _gesvj(gpu, X)
+    _check_svd(X, U, s, V)
+    assert s[-2:].max() == 0.0
+    nz = s > 0
+    assert np.abs(U[:, nz].T @ U[:, nz] - np.eye(nz.sum())).max() <= 1e-12 * k
