@@ -46,6 +46,7 @@ struct GemmDev {
   double* partial;  // split-K partial sums [split][N][M]... laid out as split*(M*N) col-major M x N
   int splits, kchunk;
   int MT, NT, NG;  // M tiles, N tiles, groups (= NT * splits)
+  int ct;          // store D(row, col) to C(col, row) (the transposed product, splits == 1)
 };
 
 __device__ __forceinline__ void split3(double x, __bf16* h, __bf16* m, __bf16* l) {
@@ -229,8 +230,24 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
 
   // ---- epilogue.  Output lane layout of a 16x16 tile (column l16 in every case):
   //   f32/bf16 MFMA: register r holds row 4*lq + r;  f64 MFMA: register r holds row lq + 4*r.
+  // With beta != 0 the C values of one row of tiles are loaded together (unconditional loads
+  // from clamped addresses), so the epilogue waits for memory once per tile row.
+  const bool rd = p.splits == 1 && p.beta != 0.0;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TM; ++i) {
+    double cv[TN][4];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + bn0 + j * 16 + l16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + am0 + i * 16 + (PREC == LRA_PREC_F64 ? lq + 4 * r : lq * 4 + r);
+        const bool ok = rd && row < p.M && col < p.N;
+        const i64 ci = p.ct ? col + (i64)row * p.ldc : row + (i64)col * p.ldc;
+        const double x = p.C[ok ? ci : 0];
+        cv[j][r] = ok ? x : 0.0;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const int col = n0 + bn0 + j * 16 + l16;
@@ -242,12 +259,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
           if (p.splits > 1) {
             p.partial[(i64)split * p.M * p.N + row + (i64)col * p.M] = v;
           } else {
-            double* c = p.C + row + (i64)col * p.ldc;
-            *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+            const i64 ci = p.ct ? col + (i64)row * p.ldc : row + (i64)col * p.ldc;
+            p.C[ci] = rd ? p.alpha * v + p.beta * cv[j][r] : p.alpha * v;
           }
         }
       }
     }
+  }
 }
 
 // FP64 GEMM for the case where both operands are k-contiguous in memory (op(A) = A' with A
@@ -400,6 +418,82 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, const double* par
   }
 }
 
+// C (M x N) = alpha A' B + beta C for a short product over a long K (M <= 32): the Gram-type
+// products of the blocked QR (V'V and V'X, K = the panel's rows).  A workgroup owns a 32 x 32
+// output tile and one K-range; 32-deep k-tiles of both operands are loaded coalesced (32
+// consecutive k of one column per 32 lanes) into double-buffered LDS and consumed by
+// v_mfma_f64_16x16x4f64, a 16 x 16 quarter per wave.  With several K-ranges the partials go
+// to `part` ([split][M x N]) and splitk_reduce_kernel adds them in a fixed order.
+
+__global__ __launch_bounds__(256) void gemm_tn_skinny_kernel(int M, int N, int K, int kchunk,
+                                                             int ntiles, const double* __restrict__ A,
+                                                             i64 lda, const double* __restrict__ B,
+                                                             i64 ldb, double* __restrict__ part,
+                                                             double alpha, double beta,
+                                                             double* __restrict__ C, i64 ldc) {
+  __shared__ double sA[2][32][33];  // [i][k]
+  __shared__ double sB[2][32][33];  // [j][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nt = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
+  const int n0 = nt * 32;
+  const int kbeg = split * kchunk, kend = min(K, kbeg + kchunk);
+  const int kk = tid & 31, c0 = tid >> 5;  // loader: k offset, first column (+8 u)
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
+    const int k = k0 + kk;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + 8 * u;
+      // unconditional loads from clamped addresses, masked afterwards
+      const bool oa = k < kend && c < M, ob = k < kend && n0 + c < N;
+      const double xa = A[oa ? k + (i64)c * lda : 0];
+      const double xb = B[ob ? k + (i64)(n0 + c) * ldb : 0];
+      ra[u] = oa ? xa : 0.0;
+      rb[u] = ob ? xb : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sA[buf][c0 + 8 * u][kk] = ra[u];
+      sB[buf][c0 + 8 * u][kk] = rb[u];
+    }
+  };
+  const int wm = wave & 1, wn = wave >> 1;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  int buf = 0;
+  if (kbeg < kend) {
+    load(kbeg);
+    store(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    const bool more = k0 + 32 < kend;
+    if (more) load(k0 + 32);
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const double a = sA[buf][wm * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+      const double b = sB[buf][wn * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // lane holds D(row = (lane >> 4) + 4 r, col = lane & 15) of its quarter
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = wm * 16 + (lane >> 4) + 4 * r, col = n0 + wn * 16 + (lane & 15);
+    if (row >= M || col >= N) continue;
+    if (part) {
+      part[(i64)split * M * N + row + (i64)col * M] = acc[r];
+    } else {
+      double* c = C + row + (i64)col * ldc;
+      *c = (beta == 0.0) ? alpha * acc[r] : alpha * acc[r] + beta * *c;
+    }
+  }
+}
+
 // ---- configurations -------------------------------------------------------------------
 struct Cfg { int id, BM, BN; };
 // id -> (BM, BN, WAVES_M, WAVES_N)
@@ -498,6 +592,7 @@ int prepare(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits, GemmDev* d, in
   d->NT = (int)((g.N + c.BN - 1) / c.BN);
   d->NG = d->NT * splits;
   d->partial = nullptr;
+  d->ct = 0;
   if (splits > 1) {
     double* part = nullptr;
     LRA_CHECK(lra_wsT(ctx, WS_SPLITK, (size_t)splits * g.M * g.N, &part));
@@ -529,6 +624,40 @@ int run(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits) {
       lra_set_error("splitk reduce launch: %s", hipGetErrorString(e));
       return LRA_E_HIP;
     }
+  }
+  return LRA_OK;
+}
+
+// A' B with M <= 32 and a long K on gemm_tn_skinny_kernel: 32 x 32 tiles, K split so that
+// about one workgroup per CU runs (ranges of at least 256, multiples of 32).
+bool tn_skinny_ok(const GemmArgs& g) {
+  return g.prec == LRA_PREC_F64 && g.transA == 1 && g.transB == 0 && g.bgather == nullptr &&
+         g.M <= 32 && g.K >= 256;
+}
+
+int run_tn_skinny(lra_ctx* ctx, const GemmArgs& g) {
+  const i64 ntiles = (g.N + 31) / 32;
+  i64 splits = std::max<i64>(1, (ctx->num_cu + ntiles - 1) / ntiles);
+  splits = std::min<i64>(splits, std::max<i64>(1, g.K / 256));
+  i64 kchunk = (g.K + splits - 1) / splits;
+  kchunk = (kchunk + 31) / 32 * 32;
+  splits = (g.K + kchunk - 1) / kchunk;
+  double* part = nullptr;
+  if (splits > 1) LRA_CHECK(lra_wsT(ctx, WS_SPLITK, (size_t)(splits * g.M * g.N), &part));
+  hipLaunchKernelGGL(gemm_tn_skinny_kernel, dim3((unsigned)(ntiles * splits)), dim3(256), 0,
+                     ctx->stream, (int)g.M, (int)g.N, (int)g.K, (int)kchunk, (int)ntiles, g.A,
+                     g.lda, g.B, g.ldb, part, g.alpha, g.beta, g.C, g.ldc);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && splits > 1) {
+    const i64 total = g.M * g.N;
+    const int rb = (int)std::min<i64>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb), dim3(256), 0, ctx->stream, (int)g.M,
+                       (int)g.N, (int)splits, part, g.alpha, g.beta, g.C, g.ldc);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    lra_set_error("skinny gemm launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
   }
   return LRA_OK;
 }
@@ -565,6 +694,33 @@ TuneVal heuristic(const GemmArgs& g, int num_cu) {
   return {cfg, s};
 }
 
+// A short-K product with a tall C (the blocked QR's X -= V W and its like) as the transposed
+// product C' = op(B)' op(A)' stored transposed: the MFMA output tile then puts 16 consecutive
+// rows of one column of C in 16 consecutive lanes, so the read-modify-write of C is
+// coalesced.  One K-range, the general kernel's configuration for the swapped shape.
+bool ct_ok(const GemmArgs& g) {
+  return g.bgather == nullptr && g.K <= 64 && g.M >= 256 && g.M >= 2 * g.N;
+}
+
+int run_ct(lra_ctx* ctx, const GemmArgs& g) {
+  GemmArgs s = g;
+  s.M = g.N; s.N = g.M;
+  s.A = g.B; s.lda = g.ldb; s.transA = !g.transB;
+  s.B = g.A; s.ldb = g.lda; s.transB = !g.transA;
+  const int cfg = heuristic(s, ctx->num_cu).cfg;
+  GemmDev d;
+  int blocks = 0;
+  LRA_CHECK(prepare(ctx, s, cfg, 1, &d, &blocks));
+  d.C = g.C; d.ldc = g.ldc;
+  d.ct = 1;
+  const hipError_t e = launch_prec(g.prec, cfg, d, blocks, ctx->stream);
+  if (e != hipSuccess) {
+    lra_set_error("gemm launch: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
 }  // namespace
 
 int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
@@ -584,6 +740,8 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
         (!kc_cfg(id) || kc_ok(g)))
       return run(ctx, g, id, sp);
   }
+  if (tn_skinny_ok(g)) return run_tn_skinny(ctx, g);
+  if (ct_ok(g)) return run_ct(ctx, g);
   TuneKey key{g.prec, g.transA, g.transB, bucket(g.M), bucket(g.N), bucket(g.K)};
   TuneVal choice;
   bool have = false;

@@ -5,9 +5,10 @@
 // diagonal signs are dgeqrf's:
 //   panel (nb <= 32 columns, all remaining rows): a cooperative grid keeps the panel in
 //     registers, one row per thread (rows strided over the block for coalescing); every
-//     column needs two grid-wide all-reductions (the column norm with the pivot entry, then
-//     the nb-c-1 dot products v'A), each done with tagged 16-byte chunks in uncached memory
-//     and summed in block order, so every block holds bit-identical reflectors;
+//     column needs one grid-wide all-reduction (the column norm, the pivot entry, and for
+//     each column to its right the dot product with this column below the pivot and its
+//     pivot-row entry, from which v'A follows), done with tagged 16-byte chunks in uncached
+//     memory and summed in block order, so every block holds bit-identical reflectors;
 //   trailing update: compact WY (I - V T V'), T from V'V (dlarft), with FP64 MFMA GEMMs:
 //     W = V' X2, W = T' W, X2 -= V W.
 // orgqr applies the blocks in reverse to [I; 0].
@@ -19,6 +20,7 @@ namespace {
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 constexpr int QR_NB = 32;
+constexpr int QR_NV = 2 * QR_NB;  // values of one column's exchange (2 + 2 (nb - 1) <= 64)
 
 struct PanelDev {
   double* A;
@@ -26,7 +28,7 @@ struct PanelDev {
   int m, j0, nb;  // panel rows j0..m-1, columns j0..j0+nb-1
   int G;          // blocks; block g holds rows j0 + g*256*RPR ...
   double* tau;    // tau[j0 + c]
-  u64x2* xchg;    // [2][G][QR_NB + 2] chunks
+  u64x2* xchg;    // [2][G][QR_NV + 2] chunks
   uint32_t gen;
   int uncached;
   int* err;
@@ -76,11 +78,12 @@ LRA_DEV void qr_put(const PanelDev& p, u64x2* dst, double val, uint64_t tag) {
 template <int RPR>
 __global__ __launch_bounds__(256) void geqr2_coop_kernel(PanelDev p) {
   constexpr int NB = QR_NB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NV = QR_NV;
+  const int tid = threadIdx.x;
   const int g = blockIdx.x;
   const int nb = p.nb;
-  __shared__ double s_w[NB][257];  // per-value block partials, transposed
-  __shared__ double s_red[NB];
+  __shared__ double s_w[NV][257];  // per-value thread partials, transposed
+  __shared__ double s_red[NV];
   __shared__ int s_ok;
 
   // rows held: r(i) = j0 + g*256*RPR + i*256 + tid
@@ -96,54 +99,49 @@ __global__ __launch_bounds__(256) void geqr2_coop_kernel(PanelDev p) {
   }
 
   int seq = 0;
-  // Grid all-reduction of vals[0..nv) (nv <= 32): the block sums each value over its 256
-  // threads (8 threads per value, 32 rows each, then a fixed butterfly), publishes the
-  // block partials as tagged chunks, and every block adds the G partials of each value in a
-  // fixed order (8 interleaved strands per value, combined by the same butterfly).  The
-  // result, bit-identical in every block, is left in s_red.
-  auto allreduce = [&](const double* vals, int nv) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < nv) s_w[j][tid] = vals[j];
+  // Grid all-reduction of the nv (<= NV) values each thread left in s_w[0..nv)[tid]: the
+  // block sums each value over its 256 threads (4 threads per value, 64 partials each, then a
+  // fixed butterfly), publishes the block partials as tagged chunks, and every block adds the
+  // G partials of each value in a fixed order (4 interleaved strands per value, combined by
+  // the same butterfly).  The result, bit-identical in every block, is left in s_red.
+  auto allreduce = [&](int nv) {
     __syncthreads();
-    const int j = tid >> 3, q = tid & 7;
+    const int j = tid >> 2, q = tid & 3;
     const uint64_t tag = ((uint64_t)p.gen << 32) | (uint64_t)(seq + 1);
-    u64x2* region = p.xchg + (size_t)(seq & 1) * p.G * (NB + 2);
+    u64x2* region = p.xchg + (size_t)(seq & 1) * p.G * (NV + 2);
     double bp = 0.0;
     if (j < nv) {
 #pragma unroll 8
-      for (int r = 0; r < 32; ++r) bp += s_w[j][q * 32 + r];
+      for (int r = 0; r < 64; ++r) bp += s_w[j][q * 64 + r];
     }
     bp += __shfl_xor(bp, 1, 64);
     bp += __shfl_xor(bp, 2, 64);
-    bp += __shfl_xor(bp, 4, 64);
-    if (j < nv && q == 0) qr_put(p, region + (size_t)g * (NB + 2) + j, bp, tag);
+    if (j < nv && q == 0) qr_put(p, region + (size_t)g * (NV + 2) + j, bp, tag);
     if (!p.uncached) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     double sum = 0.0;
     bool lok = true;
     if (j < nv) {
-      // strand q: blocks q, q+8, q+16, ... in ascending order, loads issued in batches
-      for (int b0 = q; b0 < p.G; b0 += 64) {
+      // strand q: blocks q, q+4, q+8, ... in ascending order, loads issued in batches
+      for (int b0 = q; b0 < p.G; b0 += 32) {
         u64x2 t[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int b = b0 + 8 * u;
-          const u64x2* src = region + (size_t)b * (NB + 2) + j;
+          const int b = b0 + 4 * u;
+          const u64x2* src = region + (size_t)b * (NV + 2) + j;
           if (b < p.G) t[u] = p.uncached ? *src : *(const volatile u64x2*)src;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int b = b0 + 8 * u;
+          const int b = b0 + 4 * u;
           if (b >= p.G) continue;
           double val = __longlong_as_double((long long)t[u].x);
-          if (t[u].y != tag) val = qr_read(p, region + (size_t)b * (NB + 2) + j, tag, &lok);
+          if (t[u].y != tag) val = qr_read(p, region + (size_t)b * (NV + 2) + j, tag, &lok);
           sum += val;
         }
       }
     }
     sum += __shfl_xor(sum, 1, 64);
     sum += __shfl_xor(sum, 2, 64);
-    sum += __shfl_xor(sum, 4, 64);
     if (!lok) {
       qr_raise(p, LRA_E_TIMEOUT);
       s_ok = 0;
@@ -157,17 +155,41 @@ __global__ __launch_bounds__(256) void geqr2_coop_kernel(PanelDev p) {
 
   for (int c = 0; c < nb; ++c) {
     const int rc = p.j0 + c;
-    // ---- column norm below the pivot row and the pivot entry
-    double vals[2] = {0.0, 0.0};
+    const int nq = nb - c - 1;  // panel columns right of c
+    // ---- one exchange per column: the norm below the pivot row, the pivot entry, and for
+    // every column j right of c the dot product with column c below the pivot (d_j) and the
+    // pivot-row entry (p_j); v'A(:, j) = p_j + d_j / (alpha - beta) follows from them.
+    {
+      // column c below the pivot (zero elsewhere) and the pivot-row indicator, per row held
+      double xb[RPR], pr[RPR];
+      double ss = 0.0, al = 0.0;
 #pragma unroll
-    for (int i = 0; i < RPR; ++i) {
-      double xc = 0.0;
+      for (int i = 0; i < RPR; ++i) {
+        double xc = 0.0;
 #pragma unroll
-      for (int j = 0; j < NB; ++j) xc = (j == c) ? x[i][j] : xc;
-      if (row[i] > rc && row[i] < p.m) vals[0] += xc * xc;
-      if (row[i] == rc) vals[1] = xc;
+        for (int j = 0; j < NB; ++j) xc = (j == c) ? x[i][j] : xc;
+        const bool below = row[i] > rc && row[i] < p.m;
+        xb[i] = below ? xc : 0.0;
+        pr[i] = row[i] == rc ? 1.0 : 0.0;
+        ss += xb[i] * xb[i];
+        if (row[i] == rc) al = xc;
+      }
+      s_w[0][tid] = ss;
+      s_w[1][tid] = al;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j <= c || j >= nb) continue;
+        double d = 0.0, pj = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPR; ++i) {
+          d += xb[i] * x[i][j];
+          pj += pr[i] * x[i][j];
+        }
+        s_w[2 + 2 * (j - c - 1)][tid] = d;
+        s_w[3 + 2 * (j - c - 1)][tid] = pj;
+      }
     }
-    allreduce(vals, 2);
+    allreduce(2 + 2 * nq);
     if (!s_ok) return;
     const double ss = s_red[0], alpha0 = s_red[1];
     // ---- dlarfg (identical in every block)
@@ -209,22 +231,14 @@ __global__ __launch_bounds__(256) void geqr2_coop_kernel(PanelDev p) {
 #pragma unroll
       for (int j = 0; j < NB; ++j) x[i][j] = (j == c) ? nx : x[i][j];
     }
-    if (c + 1 >= nb) break;
-    // ---- w_j = v' A(:, j) for the panel columns right of c, then A(:, j) -= tau w_j v
-    double w[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int i = 0; i < RPR; ++i) s += v[i] * x[i][j];
-      w[j] = (j > c) ? s : 0.0;
-    }
-    allreduce(w, nb);  // (columns <= c reduce zeros)
-    if (!s_ok) return;
+    if (nq == 0) break;
+    // ---- A(:, j) -= tau (v'A(:, j)) v for the panel columns right of c
+    const double fs = fac * scal;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j > c && j < nb) {
-        const double tw = tau * s_red[j];
+        const double w = s_red[3 + 2 * (j - c - 1)] + fs * s_red[2 + 2 * (j - c - 1)];
+        const double tw = tau * w;
 #pragma unroll
         for (int i = 0; i < RPR; ++i) x[i][j] -= tw * v[i];
       }
@@ -252,20 +266,45 @@ __global__ void extract_v_kernel(int mp, int nb, const double* __restrict__ A, i
 }
 
 // dlarft (forward, columnwise): T (nb x nb, ld nb) upper triangular from S = V'V and tau:
-// T(c,c) = tau_c, T(0:c, c) = -tau_c T(0:c,0:c) S(0:c, c).  One thread per output row.
+// T(c,c) = tau_c, T(0:c, c) = -tau_c T(0:c,0:c) S(0:c, c).  S, tau and T live in LDS; one
+// thread per output row (256 threads, the first nb compute), the sum over l in ascending order.
 __global__ void larft_kernel(int nb, const double* __restrict__ S, const double* __restrict__ tau,
                              double* __restrict__ T) {
   __shared__ double sT[QR_NB][QR_NB + 1];
+  __shared__ double sS[QR_NB][QR_NB + 1];
+  __shared__ double st[QR_NB];
   const int i = threadIdx.x;
-  for (int e = i; e < QR_NB * QR_NB; e += blockDim.x) sT[e % QR_NB][e / QR_NB] = 0.0;
+  // 256 threads: S comes in with four loads per thread issued together
+  double sv[QR_NB * QR_NB / 256];
+#pragma unroll
+  for (int u = 0; u < QR_NB * QR_NB / 256; ++u) {
+    const int e = i + 256 * u;
+    const bool ok = e < nb * nb;
+    const double x = S[ok ? e : 0];
+    sv[u] = ok ? x : 0.0;
+  }
+  const double tv = i < nb ? tau[i] : 0.0;
+#pragma unroll
+  for (int u = 0; u < QR_NB * QR_NB / 256; ++u) {
+    const int e = i + 256 * u;
+    sT[e % QR_NB][e / QR_NB] = 0.0;
+    if (e < nb * nb) sS[e % nb][e / nb] = sv[u];
+  }
+  if (i < nb) st[i] = tv;
   __syncthreads();
   for (int c = 0; c < nb; ++c) {
     if (i < c) {
+      // every term read (unconditional LDS loads, issued together), the ones outside
+      // [i, c) multiplied by zero; sT(i, l) is zero for l < i, so only l >= c needs a mask
       double s = 0.0;
-      for (int l = i; l < c; ++l) s += sT[i][l] * S[l + c * nb];
-      sT[i][c] = -tau[c] * s;
+#pragma unroll
+      for (int l = 0; l < QR_NB; ++l) {
+        const double t = sT[i][l] * sS[l][c];
+        s += l < c ? t : 0.0;
+      }
+      sT[i][c] = -st[c] * s;
     }
-    if (i == c) sT[c][c] = tau[c];
+    if (i == c) sT[c][c] = st[c];
     __syncthreads();
   }
   for (int e = i; e < nb * nb; e += blockDim.x) T[e] = sT[e % nb][e / nb];
@@ -300,7 +339,7 @@ int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* 
     lra_set_error("geqrf: %lld rows exceed the panel kernel", (long long)m);
     return LRA_E_UNSUPPORTED;
   }
-  const size_t bytes = 256 + 2 * (size_t)G * (QR_NB + 2) * 16;
+  const size_t bytes = 256 + 2 * (size_t)G * (QR_NV + 2) * 16;
   void* area = nullptr;
   bool uc = false;
   LRA_CHECK(lra_coop_area(c, bytes, &area, &uc));
@@ -337,7 +376,7 @@ int panel_wy(lra_ctx* c, i64 m, i64 j0, int nb, const double* A, i64 lda, const 
   g.A = Vw; g.lda = mp; g.transA = 1; g.B = Vw; g.ldb = mp; g.transB = 0;
   g.C = Sw; g.ldc = nb; g.bgather = nullptr; g.prec = LRA_PREC_F64;
   LRA_CHECK(lra_gemm(c, g));
-  hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(64), 0, c->stream, nb, Sw, tau + j0, Tw);
+  hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(256), 0, c->stream, nb, Sw, tau + j0, Tw);
   LRA_HIP(hipGetLastError());
   return LRA_OK;
 }

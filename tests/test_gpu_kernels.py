@@ -178,3 +178,56 @@ def test_id_T_vs_scipy(gpu, k, n):
     assert res <= 1e-13 * np.abs(R).max() * max(1.0, np.abs(ref).max()) * k
     cond = np.linalg.cond(R[:, :k])
     assert np.abs(T - ref).max() <= 1e-14 * cond * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("M,N,K", [(32, 32, 8192), (32, 400, 8160), (17, 1000, 1031),
+                                   (1, 5, 256), (30, 33, 4099)])
+@pytest.mark.parametrize("beta", [0.0, -0.5])
+def test_dgemm_tn_skinny(gpu, M, N, K, beta):
+    """C = alpha A'B + beta C with M <= 32 and a long K (the blocked QR's V'V and V'X): the
+    split-K MFMA kernel against numpy, ragged M, N, K and odd leading dimensions."""
+    import torch
+    L = gpu._lib.load()
+    ctx = gpu.context()
+    rng = np.random.default_rng(M * 7 + N + K)
+    lda, ldb, ldc = K + 3, K + 1, M + 5
+    A = rng.standard_normal((lda, M))
+    B = rng.standard_normal((ldb, N))
+    C0 = rng.standard_normal((ldc, N))
+    Ad, Bd, Cd = gpu.to_device(A), gpu.to_device(B), gpu.to_device(C0)
+    print(f"dgemm tn M={M} N={N} K={K} beta={beta}", flush=True)
+    gpu.check(L.lra_dgemm(ctx.h, 1, 0, M, N, K, 1.5, Ad.data_ptr(), lda, Bd.data_ptr(), ldb,
+                          beta, Cd.data_ptr(), ldc), "dgemm")
+    got = host(Cd)
+    ref = C0.copy()
+    ref[:M, :] = 1.5 * A[:K, :].T @ B[:K, :] + beta * C0[:M, :]
+    assert np.abs(got[:M] - ref[:M]).max() <= 1e-13 * K
+    assert np.array_equal(got[M:], C0[M:])  # rows past M untouched
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("K,beta", [(32, -1.0), (7, 0.0), (64, 0.25)])
+def test_dgemm_short_k_tall_c(gpu, ta, tb, K, beta):
+    """C = alpha op(A) op(B) + beta C with a short K and a tall C (the blocked QR's trailing
+    update X -= V W): computed as the transposed product with a transposed store."""
+    L = gpu._lib.load()
+    ctx = gpu.context()
+    rng = np.random.default_rng(K * 4 + ta * 2 + tb)
+    M, N = 1000, 300
+    Am = rng.standard_normal((K, M) if ta else (M, K))
+    Bm = rng.standard_normal((N, K) if tb else (K, N))
+    lda, ldb, ldc = Am.shape[0] + 1, Bm.shape[0] + 2, M + 3
+    A = np.zeros((lda, Am.shape[1])); A[:Am.shape[0]] = Am
+    B = np.zeros((ldb, Bm.shape[1])); B[:Bm.shape[0]] = Bm
+    C0 = rng.standard_normal((ldc, N))
+    Ad, Bd, Cd = gpu.to_device(A), gpu.to_device(B), gpu.to_device(C0)
+    print(f"dgemm short-K ta={ta} tb={tb} K={K} beta={beta}", flush=True)
+    gpu.check(L.lra_dgemm(ctx.h, ta, tb, M, N, K, -0.75, Ad.data_ptr(), lda, Bd.data_ptr(), ldb,
+                          beta, Cd.data_ptr(), ldc), "dgemm")
+    got = host(Cd)
+    opA = Am.T if ta else Am
+    opB = Bm.T if tb else Bm
+    ref = C0.copy()
+    ref[:M] = -0.75 * opA @ opB + beta * C0[:M]
+    assert np.abs(got[:M] - ref[:M]).max() <= 1e-13 * max(K, 1) * 4
+    assert np.array_equal(got[M:], C0[M:])
