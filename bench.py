@@ -398,7 +398,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--mixed", default="bf16x3", help="precision of the mixed form")
+    ap.add_argument("--mixed", default="f32,bf16x3",
+                    help="precisions of the mixed form (comma list; the first is the headline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -425,12 +426,26 @@ def main():
                                               rtol=1e-10)
     print(f"# fp64 drop-in: {args.steps / tot64:.2f} IDs/s k={ks64[0]} rounds={rnd64[0]}",
           file=sys.stderr, flush=True)
-    totmx, latmx, ksmx, rndmx, Vmx = time_ids(lra, torch, A, args.steps, args.warmup,
-                                              rtol=1e-10, precision=args.mixed)
-    print(f"# mixed ({args.mixed}): {args.steps / totmx:.2f} IDs/s k={ksmx[0]}",
-          file=sys.stderr, flush=True)
+    mixed = []
+    for prec in [p for p in args.mixed.split(",") if p]:
+        totmx, latmx, ksmx, rndmx, Vmx = time_ids(lra, torch, A, args.steps, args.warmup,
+                                                  rtol=1e-10, precision=prec)
+        print(f"# mixed ({prec}): {args.steps / totmx:.2f} IDs/s k={ksmx[0]}",
+              file=sys.stderr, flush=True)
+        mixed.append({"precision": prec,
+                      "value": round(args.steps / totmx, 3), "unit": "IDs/s",
+                      "p50_ms": round(1000 * float(np.percentile(latmx, 50)), 3),
+                      "p99_ms": round(1000 * float(np.percentile(latmx, 99)), 3),
+                      "k": ksmx[0], "error_rel_spectral": id_error(lra, A, Vmx),
+                      "speedup_vs_f64": round(tot64 / totmx, 3)})
+        del Vmx
+    if mixed:
+        mixed[0]["note"] = ("rtol=1e-10 is below the low-precision noise floor (1e-6): rounds "
+                            "whose target the low-precision sketch certifies above the floor "
+                            "run in low precision, the later ones in FP64 (bit-for-bit the "
+                            "FP64 drop-in's result); the pivoted QR, precision-independent, "
+                            "is about two thirds of the time")
     err64 = id_error(lra, A, V64)
-    errmx = id_error(lra, A, Vmx)
     roof, roof_gemm = kernel_roofline(lra, torch, A)
 
     batched = None
@@ -484,12 +499,8 @@ def main():
         "p50_ms": round(1000 * float(np.percentile(lat64, 50)), 3),
         "p99_ms": round(1000 * float(np.percentile(lat64, 99)), 3),
         "error_rel_spectral": err64,
-        "mixed_precision": {"precision": args.mixed,
-                            "value": round(args.steps / totmx, 3), "unit": "IDs/s",
-                            "p50_ms": round(1000 * float(np.percentile(latmx, 50)), 3),
-                            "p99_ms": round(1000 * float(np.percentile(latmx, 99)), 3),
-                            "k": ksmx[0], "error_rel_spectral": errmx,
-                            "speedup_vs_f64": round(tot64 / totmx, 3)},
+        "mixed_precision": mixed[0] if mixed else None,
+        "mixed_precision_variants": mixed[1:],
         "roofline": roof,
         "roofline_sketch_gemm": roof_gemm,
         "cpu_baseline": cpu,

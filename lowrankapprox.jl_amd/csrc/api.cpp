@@ -495,13 +495,33 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
     q.vn2 = nullptr; q.stop_after = adap ? (int)std::min<i64>(target, 1 << 30) : 0;
     q.k_out = dk; q.early_out = dearly;
     LRA_CHECK(lra_qrcp(c, q));
-    i64 hk = 0;
-    LRA_CHECK(read_scalars(c, dk, &hk, 1));
+    // with rtol below the floor, a rejected low-precision round also reports |R(0,0)| (the
+    // largest column norm) and |R(t-1,t-1)|, t = target: R is written back on every exit
+    const bool look = adap && lowp && o->rtol < floor && target >= 1 && target <= l;
+    if (look) {
+      LRA_HIP(hipMemcpyAsync(dk + 2, B, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      LRA_HIP(hipMemcpyAsync(dk + 3, B + (target - 1) * (ell + 1), sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+    }
+    i64 hv[4] = {0, 0, 0, 0};
+    LRA_CHECK(read_scalars(c, dk, hv, look ? 4 : 1));
+    const i64 hk = hv[0];
     const bool accept = !adap || hk < target;
     if (accept && lowp && o->rtol < floor) {
       // the low-precision sketch cannot resolve this round: redo it in FP64 (same Omega)
       prec = LRA_PREC_F64;
       continue;
+    }
+    if (!accept && look) {
+      // the next round (twice the target) is run in FP64 from the start when this round's
+      // last pivot is already within kFloorAhead of the floor: a low-precision attempt would
+      // most likely stop on the floor and be recomputed.  Only the precision of a round
+      // changes; an FP64 round decides exactly as the FP64 drop-in does.
+      constexpr double kFloorAhead = 1e4;
+      double r0 = 0.0, rl = 0.0;
+      std::memcpy(&r0, &hv[2], sizeof(double));
+      std::memcpy(&rl, &hv[3], sizeof(double));
+      if (fabs(rl) < kFloorAhead * floor * fabs(r0)) prec = LRA_PREC_F64;
     }
     if (accept) {
       out->B = B; out->Bsave = Bsave; out->ell = ell; out->N = N; out->jpvt = jpvt;

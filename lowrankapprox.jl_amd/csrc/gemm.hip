@@ -274,16 +274,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDev p) {
 // the MFMAs consume the current one from a second LDS buffer, so each k-tile costs one
 // barrier.  LDS rows hold BK = 16 doubles padded to 17, which keeps the 16 lanes of an MFMA
 // operand read on distinct banks.
-template <int BM, int BN, int WAVES_M, int WAVES_N, int BK = 16>
+template <int BM, int BN, int WAVES_M, int WAVES_N, int BK = 16, bool F32 = false>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(GemmDev p) {
+  // F32: the same FP64 loads, rounded to FP32 on the way into LDS, v_mfma_f32_16x16x4f32
+  // with FP32 accumulation (the F32 sketch precision)
+  using E = typename std::conditional<F32, float, double>::type;
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  using ACC = typename std::conditional<F32, f4v, d4>::type;
   constexpr int NT = 64 * WAVES_M * WAVES_N;
   constexpr int LDK = BK + 1, KP = BK / 2;  // KP: 16-byte k-pairs per tile row
   constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int CA = BM * KP, CB = BN * KP;  // 16-byte chunks per tile
   constexpr int LA = (CA + NT - 1) / NT, LB = (CB + NT - 1) / NT;
-  __shared__ double sA[2][BM * LDK];
-  __shared__ double sB[2][BN * LDK];
+  __shared__ E sA[2][BM * LDK];
+  __shared__ E sB[2][BN * LDK];
 
   const int b = blockIdx.x;
   const int xcd = b & 7, idx = b >> 3;
@@ -334,8 +339,8 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(Gem
       const int e = tid + NT * t;
       if (e < CA) {
         const int o = (e / KP) * LDK + (e % KP) * 2;
-        sA[buf][o] = ra[t].x;
-        sA[buf][o + 1] = ra[t].y;
+        sA[buf][o] = (E)ra[t].x;
+        sA[buf][o + 1] = (E)ra[t].y;
       }
     }
 #pragma unroll
@@ -343,17 +348,17 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(Gem
       const int e = tid + NT * t;
       if (e < CB) {
         const int o = (e / KP) * LDK + (e % KP) * 2;
-        sB[buf][o] = rb[t].x;
-        sB[buf][o + 1] = rb[t].y;
+        sB[buf][o] = (E)rb[t].x;
+        sB[buf][o + 1] = (E)rb[t].y;
       }
     }
   };
 
-  d4 acc[TM][TN];
+  ACC acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = ACC{0, 0, 0, 0};
   const int l16 = lane & 15, lq = lane >> 4;
   const int am0 = wm * WTM, bn0 = wn * WTN;
 
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(Gem
       if (more) load_tiles(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        double af[TM], bfv[TN];
+        E af[TM], bfv[TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i) af[i] = sA[cur][(am0 + i * 16 + l16) * LDK + kk + lq];
 #pragma unroll
@@ -376,8 +381,12 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(Gem
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bfv[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (F32)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bfv[j], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bfv[j], acc[i][j], 0, 0, 0);
+          }
       }
       if (more) store_tiles(cur ^ 1);
       __syncthreads();
@@ -391,9 +400,10 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kc_f64_kernel(Gem
       const int col = n0 + bn0 + j * 16 + l16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + am0 + i * 16 + lq + 4 * r;
+        // f64 MFMA: register r holds row lq + 4 r; f32 MFMA: row 4 lq + r
+        const int row = m0 + am0 + i * 16 + (F32 ? 4 * lq + r : lq + 4 * r);
         if (row < p.M && col < p.N) {
-          const double v = acc[i][j][r];
+          const double v = (double)acc[i][j][r];
           if (p.splits > 1) {
             p.partial[(i64)split * p.M * p.N + row + (i64)col * p.M] = v;
           } else {
@@ -502,12 +512,13 @@ static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 25
                             {10, 64, 128}, {11, 128, 256}, {12, 128, 128}, {13, 64, 256},
                             {14, 256, 128}, {15, 80, 256}, {16, 96, 256}, {17, 160, 256}};
 constexpr int kNumCfg = 18;
-// ids 5..17: the k-contiguous FP64 kernel (gemm_kc_f64_kernel; 11, 13, 14, 16, 17 with 512
+// ids 5..17: the k-contiguous kernel (gemm_kc_f64_kernel, FP64 or F32 MFMA; 11, 13, 14, 16, 17 with 512
 // threads, 12 with 32-wide k-tiles).  15-17 hold a short sketch (ell = 72, 136) in one M-tile,
 // so each k-tile of A is read from L2 once.
 bool kc_cfg(int id) { return id >= 5; }
 bool kc_ok(const GemmArgs& g) {
-  return g.prec == LRA_PREC_F64 && g.transA == 1 && g.transB == 0 && g.bgather == nullptr &&
+  return (g.prec == LRA_PREC_F64 || g.prec == LRA_PREC_F32) && g.transA == 1 && g.transB == 0 &&
+         g.bgather == nullptr &&
          (g.lda % 2) == 0 && (g.ldb % 2) == 0 && ((uintptr_t)g.A % 16) == 0 &&
          ((uintptr_t)g.B % 16) == 0;
 }
@@ -520,26 +531,78 @@ hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
     case 2: hipLaunchKernelGGL((gemm_kernel<PREC, 48, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
     case 3: hipLaunchKernelGGL((gemm_kernel<PREC, 64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
     case 4: hipLaunchKernelGGL((gemm_kernel<PREC, 128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 5: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<16, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 6: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<32, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 7: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<48, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 8: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 9: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 10: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 128, 2, 2>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 11: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
-    case 12: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 32>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 13: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
-    case 14: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<256, 128, 4, 2>), dim3(blocks), dim3(512), 0, st, d); break;
-    case 15: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<80, 256, 1, 4>), dim3(blocks), dim3(256), 0, st, d); break;
-    case 16: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<96, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
-    case 17: if constexpr (PREC != LRA_PREC_F64) return hipErrorInvalidValue; else hipLaunchKernelGGL((gemm_kc_f64_kernel<160, 256, 2, 4>), dim3(blocks), dim3(512), 0, st, d); break;
+    case 5:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<16, 256, 1, 4, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<16, 256, 1, 4, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 6:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<32, 256, 1, 4, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<32, 256, 1, 4, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 7:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<48, 256, 1, 4, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<48, 256, 1, 4, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 8:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 1, 4, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 1, 4, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 9:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 10:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 128, 2, 2, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 128, 2, 2, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 11:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 256, 2, 4, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 256, 2, 4, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 12:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 32, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<128, 128, 2, 2, 32, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 13:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 2, 4, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<64, 256, 2, 4, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 14:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<256, 128, 4, 2, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<256, 128, 4, 2, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 15:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<80, 256, 1, 4, 16, false>), dim3(blocks), dim3(256), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<80, 256, 1, 4, 16, true>), dim3(blocks), dim3(256), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 16:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<96, 256, 2, 4, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<96, 256, 2, 4, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 17:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<160, 256, 2, 4, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<160, 256, 2, 4, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_prec(int prec, int id, const GemmDev& d, int blocks, hipStream_t st) {
-  if (kc_cfg(id) && prec != LRA_PREC_F64) return hipErrorInvalidValue;
+  if (kc_cfg(id) && prec != LRA_PREC_F64 && prec != LRA_PREC_F32) return hipErrorInvalidValue;
   switch (prec) {
     case LRA_PREC_F64: return launch_cfg<LRA_PREC_F64>(id, d, blocks, st);
     case LRA_PREC_F32: return launch_cfg<LRA_PREC_F32>(id, d, blocks, st);

@@ -231,3 +231,21 @@ def test_dgemm_short_k_tall_c(gpu, ta, tb, K, beta):
     ref[:M] = -0.75 * opA @ opB + beta * C0[:M]
     assert np.abs(got[:M] - ref[:M]).max() <= 1e-13 * max(K, 1) * 4
     assert np.array_equal(got[M:], C0[M:])
+
+
+@pytest.mark.parametrize("cfg", ["5:1", "6:2", "7:1", "8:1", "9:1", "10:4", "11:1", "12:2", "13:3",
+                                 "14:1", "15:2", "16:1", "17:4"])
+@pytest.mark.parametrize("M,N,K", [(40, 1000, 1031), (136, 513, 800)])
+def test_sketch_gemm_f32_kc_configs(gpu, monkeypatch, cfg, M, N, K):
+    """The k-contiguous kernel in its F32 form (FP64 loads rounded to FP32 into LDS,
+    v_mfma_f32_16x16x4f32), every configuration forced, against the FP64 product: the error
+    is that of FP32 rounding and accumulation."""
+    rng = np.random.default_rng(M * N + K + 1)
+    A = np.asfortranarray(rng.standard_normal((K, N)))
+    monkeypatch.setenv("LRA_GEMM_CFG", cfg)
+    print(f"forced f32 kc gemm cfg={cfg} M={M} N={N} K={K}", flush=True)
+    B = host(gpu.sketch("left", "n", A, M, seed=9, stream=3, precision="f32"))
+    Om = host(gpu.gaussian(M, K, 9, 3))
+    ref = Om @ A
+    err = np.abs(B - ref).max() / np.sqrt(K)
+    assert err < 2e-5, err
