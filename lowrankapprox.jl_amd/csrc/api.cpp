@@ -86,7 +86,8 @@ SpinSection::SpinSection(lra_ctx* ctx) : c(ctx) {
   SpinGate& g = g_spin[c->device];
   g.mu.lock();
   held = true;
-  if (g.last) (void)hipStreamWaitEvent(c->stream, g.last, 0);
+  // (this context's own last event is already ordered by its stream)
+  if (g.last && g.last != c->spin_ev) (void)hipStreamWaitEvent(c->stream, g.last, 0);
 }
 
 SpinSection::~SpinSection() {

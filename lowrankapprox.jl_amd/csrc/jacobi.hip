@@ -115,7 +115,11 @@ template <int W, int MR>
 __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB, int r, int first,
                                                            double* __restrict__ X, i64 ldx,
                                                            double* __restrict__ V, i64 ldv,
-                                                           double tol, int* __restrict__ rotated) {
+                                                           double tol, int* __restrict__ rotated,
+                                                           const int* __restrict__ prev) {
+  // a sweep enqueued before the host saw the previous one's count: nothing to do when the
+  // previous sweep rotated nothing (it completed before this launch began)
+  if (prev && *(const volatile int*)prev == 0) return;
   constexpr int C = 2 * W;
   constexpr int QS = C + 1;                 // row stride of Qs
   constexpr int NW = 8;                     // waves
@@ -332,15 +336,15 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
     __hip_atomic_fetch_add(rotated, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// V = I and the per-sweep rotation counters rotated[0..kMaxSweeps) = 0
+constexpr int kMaxSweeps = 60;
 __global__ void jacobi_eye_kernel(int n, double* V, i64 ldv, int* rotated) {
   const i64 total = (i64)n * n;
   for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
        e += (i64)gridDim.x * blockDim.x)
     V[(e % n) + (e / n) * ldv] = (e % n) == (e / n) ? 1.0 : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *rotated = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kMaxSweeps) rotated[threadIdx.x] = 0;
 }
-
-__global__ void zero_int_kernel(int* x) { *x = 0; }
 
 // column norms of X into s
 __global__ __launch_bounds__(256) void colnorm_kernel(int m, const double* __restrict__ X,
@@ -410,7 +414,8 @@ BlockCfg block_cfg(i64 m, i64 n) {
 
 template <int W, int MR>
 void launch_block_wm(int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
-                     double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+                     double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot,
+                     const int* prev) {
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)jacobi_block_kernel<W, MR>,
@@ -419,36 +424,38 @@ void launch_block_wm(int NB, size_t lds, hipStream_t st, int m, int n, int r, in
     attr = true;
   }
   hipLaunchKernelGGL((jacobi_block_kernel<W, MR>), dim3((unsigned)(NB / 2)), dim3(512), lds, st,
-                     m, n, NB, r, first, X, ldx, V, ldv, tol, rot);
+                     m, n, NB, r, first, X, ldx, V, ldv, tol, rot, prev);
 }
 
 template <int W>
 void launch_block_w(int MR, int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
-                    double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+                    double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot,
+                    const int* prev) {
   switch (MR) {
-    case 4: launch_block_wm<W, 4>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    case 8: launch_block_wm<W, 8>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    case 16: launch_block_wm<W, 16>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 4: launch_block_wm<W, 4>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 8: launch_block_wm<W, 8>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 16: launch_block_wm<W, 16>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
     case 24:
-      if constexpr (W > 2) launch_block_wm<W, 24>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot);
+      if constexpr (W > 2) launch_block_wm<W, 24>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
       break;
     case 28:
-      if constexpr (W > 2) launch_block_wm<W, 28>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot);
+      if constexpr (W > 2) launch_block_wm<W, 28>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
       break;
-    default: launch_block_wm<W, 32>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    default: launch_block_wm<W, 32>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
   }
 }
 
 void launch_block(const BlockCfg& cfg, int NB, hipStream_t st, int m, int n, int r, int first,
-                  double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot) {
+                  double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot,
+                  const int* prev) {
   const int MR = cfg.MR;
   const size_t lds = cfg.lds;
   switch (cfg.W) {
-    case 16: launch_block_w<16>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    case 8: launch_block_w<8>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    case 4: launch_block_w<4>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    case 2: launch_block_w<2>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
-    default: launch_block_w<1>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot); break;
+    case 16: launch_block_w<16>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 8: launch_block_w<8>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 4: launch_block_w<4>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 2: launch_block_w<2>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    default: launch_block_w<1>(MR, NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
   }
 }
 
@@ -477,22 +484,49 @@ int lra_gesvj_impl(lra_ctx* c, i64 m, i64 n, double* X, i64 ldx, double* U, i64 
   const int W = cfg.W;
   const int NB = W ? (int)((n + W - 1) / W + (((n + W - 1) / W) & 1)) : 0;
   int sw_done = 0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, c->stream, rot);
-    if (W) {
+  if (W) {
+    // Sweep s + 1 is enqueued before the host reads sweep s's count (its kernels return at
+    // once when sweep s rotated nothing), so the device does not idle while the host waits.
+    // The count of sweep s goes to pinned host memory (bytes 3072.. of the context's staging
+    // page) behind an event that covers sweep s only.
+    int* hcnt = (int*)((char*)c->host_pinned + 3072);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev) LRA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auto enqueue = [&](int sw) -> int {
       for (int r = 0; r < NB - 1; ++r)
-        launch_block(cfg, NB, c->stream, (int)m, (int)n, r, r == 0, X, ldx, Vw, n, tol, rot);
-    } else {
+        launch_block(cfg, NB, c->stream, (int)m, (int)n, r, r == 0, X, ldx, Vw, n, tol,
+                     rot + sw, sw ? rot + sw - 1 : nullptr);
+      LRA_HIP(hipGetLastError());
+      LRA_HIP(hipMemcpyAsync(hcnt + (sw & 1), rot + sw, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+      LRA_HIP(hipEventRecord(ev[sw & 1], c->stream));
+      return LRA_OK;
+    };
+    int rc = enqueue(0);
+    for (int sweep = 0; rc == LRA_OK && sweep < kMaxSweeps; ++sweep) {
+      if (sweep + 1 < kMaxSweeps && (rc = enqueue(sweep + 1)) != LRA_OK) break;
+      if (hipEventSynchronize(ev[sweep & 1]) != hipSuccess) {
+        lra_set_error("gesvj: %s", hipGetErrorString(hipGetLastError()));
+        rc = LRA_E_HIP;
+        break;
+      }
+      sw_done = sweep + 1;
+      if (hcnt[sweep & 1] == 0) break;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);  // released once their last record completes
+    LRA_CHECK(rc);
+  } else {
+    for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
       for (int r = 0; r < N - 1; ++r)
         hipLaunchKernelGGL(jacobi_round_kernel, dim3(N / 2), dim3(256), 0, c->stream, (int)m,
-                           (int)n, N, r, X, ldx, Vw, n, tol, rot);
+                           (int)n, N, r, X, ldx, Vw, n, tol, rot + sweep);
+      LRA_HIP(hipGetLastError());
+      int h = 0;
+      LRA_HIP(hipMemcpyAsync(&h, rot + sweep, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      LRA_HIP(hipStreamSynchronize(c->stream));
+      sw_done = sweep + 1;
+      if (h == 0) break;
     }
-    LRA_HIP(hipGetLastError());
-    int h = 0;
-    LRA_HIP(hipMemcpyAsync(&h, rot, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    LRA_HIP(hipStreamSynchronize(c->stream));
-    sw_done = sweep + 1;
-    if (h == 0) break;
   }
   if (sweeps) *sweeps = sw_done;
   hipLaunchKernelGGL(colnorm_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, (int)m, X, ldx,

@@ -265,49 +265,48 @@ __global__ void extract_v_kernel(int mp, int nb, const double* __restrict__ A, i
   }
 }
 
-// dlarft (forward, columnwise): T (nb x nb, ld nb) upper triangular from S = V'V and tau:
-// T(c,c) = tau_c, T(0:c, c) = -tau_c T(0:c,0:c) S(0:c, c).  S, tau and T live in LDS; one
-// thread per output row (256 threads, the first nb compute), the sum over l in ascending order.
-__global__ void larft_kernel(int nb, const double* __restrict__ S, const double* __restrict__ tau,
-                             double* __restrict__ T) {
-  __shared__ double sT[QR_NB][QR_NB + 1];
-  __shared__ double sS[QR_NB][QR_NB + 1];
+// dlarft (forward, columnwise): T (nb x nb, ld nb) upper triangular with
+// I - V T V' = H_1 ... H_nb, from S = V'V and tau.  dlarft's recurrence T(0:c, c) =
+// -tau_c T(0:c, 0:c) S(0:c, c) runs column after column; the same T is the inverse of
+// U = diag(1/tau) + triu(S, 1), so T(c, c) = tau_c and, for i < c,
+// T(i, c) = -tau_i sum_{l > i} S(i, l) T(l, c) (also with tau_i = 0: H_i = I, row i zero).
+// Lane c back-substitutes column c on its own: one wave, S in LDS (broadcast reads), no
+// barrier between columns.
+__global__ __launch_bounds__(64) void larft_kernel(int nb, const double* __restrict__ S,
+                                                   const double* __restrict__ tau,
+                                                   double* __restrict__ T) {
+  __shared__ double sS[QR_NB][QR_NB];  // S(i, l), zero outside nb x nb
   __shared__ double st[QR_NB];
-  const int i = threadIdx.x;
-  // 256 threads: S comes in with four loads per thread issued together
-  double sv[QR_NB * QR_NB / 256];
+  const int c = threadIdx.x;
+  double sv[QR_NB * QR_NB / 64];
 #pragma unroll
-  for (int u = 0; u < QR_NB * QR_NB / 256; ++u) {
-    const int e = i + 256 * u;
-    const bool ok = e < nb * nb;
-    const double x = S[ok ? e : 0];
+  for (int u = 0; u < QR_NB * QR_NB / 64; ++u) {
+    const int e = c + 64 * u, i = e % QR_NB, l = e / QR_NB;
+    const bool ok = i < nb && l < nb;
+    const double x = S[ok ? i + l * nb : 0];
     sv[u] = ok ? x : 0.0;
   }
-  const double tv = i < nb ? tau[i] : 0.0;
+  const double tv = c < nb ? tau[c] : 0.0;
 #pragma unroll
-  for (int u = 0; u < QR_NB * QR_NB / 256; ++u) {
-    const int e = i + 256 * u;
-    sT[e % QR_NB][e / QR_NB] = 0.0;
-    if (e < nb * nb) sS[e % nb][e / nb] = sv[u];
+  for (int u = 0; u < QR_NB * QR_NB / 64; ++u) {
+    const int e = c + 64 * u;
+    sS[e % QR_NB][e / QR_NB] = sv[u];
   }
-  if (i < nb) st[i] = tv;
+  if (c < QR_NB) st[c] = tv;
   __syncthreads();
-  for (int c = 0; c < nb; ++c) {
-    if (i < c) {
-      // every term read (unconditional LDS loads, issued together), the ones outside
-      // [i, c) multiplied by zero; sT(i, l) is zero for l < i, so only l >= c needs a mask
-      double s = 0.0;
+  double x[QR_NB];
 #pragma unroll
-      for (int l = 0; l < QR_NB; ++l) {
-        const double t = sT[i][l] * sS[l][c];
-        s += l < c ? t : 0.0;
-      }
-      sT[i][c] = -st[c] * s;
-    }
-    if (i == c) sT[c][c] = st[c];
-    __syncthreads();
+  for (int i = QR_NB - 1; i >= 0; --i) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = i + 1; l < QR_NB; ++l) s += sS[i][l] * x[l];
+    x[i] = i < c ? -st[i] * s : (i == c ? st[i] : 0.0);
   }
-  for (int e = i; e < nb * nb; e += blockDim.x) T[e] = sT[e % nb][e / nb];
+  if (c < nb) {
+#pragma unroll
+    for (int i = 0; i < QR_NB; ++i)
+      if (i < nb) T[i + c * nb] = x[i];
+  }
 }
 
 // C (m x n) = [I; 0] (the first n columns of the m x m identity)
@@ -330,7 +329,10 @@ hipError_t launch_panel(const PanelDev& d, hipStream_t st) {
 }
 
 // one panel: rows j0..m-1, columns j0..j0+nb-1 of A
-int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* tau) {
+// `first`: the first panel of a factorization clears the abort word (a raised abort ends the
+// factorization with an error, so later panels find it clear)
+int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* tau,
+                 bool first) {
   const i64 mp = m - j0;
   int rpr = 1;
   while (rpr < 2 && (mp + 256 * rpr - 1) / (256 * rpr) > 256) rpr *= 2;
@@ -350,7 +352,7 @@ int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* 
   d.gen = ++c->qrcp_gen;  // chunk tags of this launch (shared counter with the QRCP)
   d.uncached = uc ? 1 : 0;
   d.err = c->d_err;
-  LRA_CHECK(lra_launch_zero_u32(c->stream, (unsigned*)area, 256 / 4));
+  if (first) LRA_CHECK(lra_launch_zero_u32(c->stream, (unsigned*)area, 256 / 4));
   hipError_t e;
   {
     SpinSection gate(c);
@@ -376,7 +378,7 @@ int panel_wy(lra_ctx* c, i64 m, i64 j0, int nb, const double* A, i64 lda, const 
   g.A = Vw; g.lda = mp; g.transA = 1; g.B = Vw; g.ldb = mp; g.transB = 0;
   g.C = Sw; g.ldc = nb; g.bgather = nullptr; g.prec = LRA_PREC_F64;
   LRA_CHECK(lra_gemm(c, g));
-  hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(256), 0, c->stream, nb, Sw, tau + j0, Tw);
+  hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(64), 0, c->stream, nb, Sw, tau + j0, Tw);
   LRA_HIP(hipGetLastError());
   return LRA_OK;
 }
@@ -423,7 +425,7 @@ int lra_geqrf_impl(lra_ctx* c, i64 m, i64 n, double* A, i64 lda, double* tau) {
   W2 = W1 + (size_t)QR_NB * std::max<i64>(n, 1);
   for (i64 j0 = 0; j0 < n; j0 += QR_NB) {
     const int nb = (int)std::min<i64>(QR_NB, n - j0);
-    LRA_CHECK(factor_panel(c, m, j0, nb, A, lda, tau));
+    LRA_CHECK(factor_panel(c, m, j0, nb, A, lda, tau, j0 == 0));
     const i64 n2 = n - j0 - nb;
     if (n2 > 0) {
       LRA_CHECK(panel_wy(c, m, j0, nb, A, lda, tau, Vw, Sw, Tw));

@@ -27,12 +27,18 @@ def test_qr_vs_lapack(gpu, m, n):
     assert np.abs(Q @ R - A).max() <= 1e-13 * np.abs(A).max() * np.sqrt(m)
 
 
-def test_qr_rank_deficient_column(gpu):
+@pytest.mark.parametrize("n,zero", [(20, (7,)), (70, (7, 40))])
+def test_qr_rank_deficient_column(gpu, n, zero):
+    """Zero columns give tau = 0 (H = I); with n = 70 they sit in panels whose T (dlarft)
+    feeds a trailing update and the explicit Q."""
     rng = np.random.default_rng(1)
-    A = rng.standard_normal((500, 20))
-    A[:, 7] = 0.0
+    A = rng.standard_normal((500, n))
+    for j in zero:
+        A[:, j] = 0.0
     A = np.asfortranarray(A)
     Q, R = gpu.qr(A)
     Q, R = host(Q), host(R)
     assert np.abs(Q @ R - A).max() <= 1e-12
-    assert np.abs(Q.T @ Q - np.eye(20)).max() <= 1e-12
+    assert np.abs(Q.T @ Q - np.eye(n)).max() <= 1e-12
+    Rl = sla.qr(A, mode="r")[0][:n, :]
+    assert np.abs(R - Rl).max() <= 1e-11 * np.abs(Rl).max()
