@@ -372,6 +372,55 @@ def profile_kernels(lra, torch, n: int):
     print(f"# profiled: copy 1 GiB, qrcp 520x{n} (k={k}), batched 4096 blocks", file=sys.stderr)
 
 
+def profile_factor(lra, torch, n: int):
+    """Launch sequence for a `rocprofv3 --kernel-trace` run of the dense factorizations on the
+    n x n workload matrix: a warm-up of each (workspaces grow, GEMM shapes tune), then one
+    psvdfact and one pqrfact (rtol 1e-10), each preceded by a one-element torch add that
+    marks where it starts in the trace (`--summarize-trace` splits on these)."""
+    A, _ = make_matrix(torch, n)
+    lra.psvdfact(A, rtol=1e-10)
+    lra.pqrfact(A, rtol=1e-10)
+    mark = torch.ones(4, device="cuda")
+    torch.cuda.synchronize()
+    for name in ("psvdfact", "pqrfact"):
+        mark.add_(1.0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        F = getattr(lra, name)(A, rtol=1e-10, seed=5)
+        torch.cuda.synchronize()
+        print(f"# profiled {name}: k={F.k} {1e3 * (time.perf_counter() - t):.2f} ms",
+              file=sys.stderr, flush=True)
+
+
+def summarize_trace(path: str, names=("psvdfact", "pqrfact")):
+    """Per-kernel totals of the sections of a rocprofv3 kernel-trace CSV that follow the
+    marker kernels of `--profile-factor` (torch's elementwise add), as a markdown table."""
+    import csv
+    import collections
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if "vectorized_elementwise" in r["Kernel_Name"]]
+    marks = marks[-len(names):] + [len(rows)]
+
+    def short(k):
+        return k.replace("void ", "").replace("(anonymous namespace)::", "").split("(")[0]
+
+    for name, a, b in zip(names, marks, marks[1:]):
+        sel = rows[a + 1:b]
+        t0 = int(sel[0]["Start_Timestamp"])
+        t1 = max(int(r["End_Timestamp"]) for r in sel)
+        agg = collections.defaultdict(lambda: [0, 0])
+        for r in sel:
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            agg[short(r["Kernel_Name"])][0] += 1
+            agg[short(r["Kernel_Name"])][1] += d
+        busy = sum(v[1] for v in agg.values())
+        print(f"\n### {name}: span {(t1 - t0) / 1e3:.1f} us, kernels busy {busy / 1e3:.1f} us, "
+              f"{len(sel)} launches\n")
+        print("| kernel | calls | total us | avg us |\n|---|---|---|---|")
+        for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+            print(f"| `{k}` | {v[0]} | {v[1] / 1e3:.1f} | {v[1] / 1e3 / v[0]:.1f} |")
+
+
 def cpu_baseline(A_host, seed_rounds, threads, lra_omega):
     """The oracle (CPU restatement of the reference: numpy/OpenBLAS dgemm + OpenBLAS dlaqps
     through geqp3_adap_main!'s loop + dtrsm) on one ID of the same matrix with the same
@@ -408,7 +457,15 @@ def main():
                     help="only launch, for rocprofv3 counter runs: a 1 GiB device copy "
                          "(calibration), the final-round pivoted QR of the 8192^2 workload, "
                          "and one batched launch of the 4096-block workload")
+    ap.add_argument("--profile-factor", action="store_true",
+                    help="only launch, for a rocprofv3 kernel-trace run: psvdfact and pqrfact "
+                         "of the workload matrix after a warm-up, each behind a marker kernel")
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print per-kernel totals of a --profile-factor kernel trace and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1"}))
         sys.exit(2)
@@ -417,6 +474,9 @@ def main():
     torch.cuda.set_device(0)
     if args.profile_kernels:
         profile_kernels(lra, torch, args.n)
+        return
+    if args.profile_factor:
+        profile_factor(lra, torch, args.n)
         return
     A, sig = make_matrix(torch, args.n)
     torch.cuda.synchronize()
