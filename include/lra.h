@@ -40,7 +40,7 @@ enum { LRA_PREC_F64 = 0, LRA_PREC_F32 = 1, LRA_PREC_BF16 = 2, LRA_PREC_BF16X3 = 
 /* pqrfact_retval bits ("q", "r", "t"; src/pqr.jl:299-301) */
 enum { LRA_RET_Q = 1, LRA_RET_R = 2, LRA_RET_T = 4 };
 /* trans / side symbols (:n / :c, :left / :right) */
-enum { LRA_TRANS_N = 0, LRA_TRANS_C = 1 };
+enum { LRA_TRANS_N = 0, LRA_TRANS_C = 1, LRA_TRANS_B = 2 /* prange only */ };
 enum { LRA_SIDE_LEFT = 0, LRA_SIDE_RIGHT = 1 };
 
 /*
@@ -292,6 +292,16 @@ int lra_curfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda
  * QRCP of op(A) itself (pqrfact_none, :323-327). */
 int lra_pqrfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
                 const lra_opts* opts, lra_pqr_out* out);
+
+/* prange(trans, A, opts) (src/prange.jl:14-77): Q (P x k, orthonormal columns) spanning the
+ * numerical range of op(A) (trans n / c; P = rows of op(A)), or of A and A' together
+ * (trans LRA_TRANS_B, square A; an exactly symmetric A takes trans n).  sketch randn: the
+ * right-side adaptive sketchfact (src/sketch.jl:223-240) and its Householder Q; sub:
+ * prange_sub (:65-78); none: pqrfact(op(A))[:Q]; trans b with randn or none only; srft and
+ * sprn return LRA_E_UNSUPPORTED.  Only out->Q, ldq (NULL Q: library-owned, valid until the
+ * next call on the context), k and rounds are set. */
+int lra_prange(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
+               const lra_opts* opts, lra_pqr_out* out);
 
 /* psvdfact(A, opts) / psvdvals (src/psvd.jl:238-300): the ID, the thin QR of the skeleton,
  * the SVD of the k x n core through its QR and a one-sided Jacobi SVD, truncated by

@@ -27,7 +27,7 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
            "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
-           "BatchedID", "idfact_batched", "Server"]
+           "BatchedID", "idfact_batched", "Server", "prange"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -725,6 +725,26 @@ def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
     if "q" in rv and "r" in rv and "t" not in rv:
         return PartialQR(Q, R, p)
     return PQRFactors(Q, R, p, k, T)
+
+
+def prange(trans, A=None, opts: LRAOptions = None, **kw):
+    """prange(trans, A, opts) (src/prange.jl:14-77): Q with orthonormal columns spanning the
+    numerical range of op(A) (trans "n" / "c") or of A and A' together ("b", square A)."""
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    if trans not in _lib.PRANGE_TRANS:
+        raise ValueError("trans")
+    opts = (opts or LRAOptions()).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    P = n if trans == "c" else m
+    out = _lib.lra_pqr_out()
+    copts = opts.to_c()
+    check(_lib.load().lra_prange(ctx.h, _lib.PRANGE_TRANS[trans], m, n, _ptr(A), _ld(A),
+                                 C.byref(copts), C.byref(out)), "lra_prange")
+    k = int(out.k)
+    return _own(out.Q, P, k, int(out.ldq)) if k > 0 else empty_colmajor(P, 0)
 
 
 def pqr(trans, A=None, opts: LRAOptions = None, **kw):

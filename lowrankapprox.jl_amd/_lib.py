@@ -21,6 +21,7 @@ SKETCH = {"none": 0, "randn": 1, "srft": 2, "sub": 3, "sprn": 4}
 PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16x3": 3}
 RET_Q, RET_R, RET_T = 1, 2, 4
 TRANS = {"n": 0, "c": 1}
+PRANGE_TRANS = {"n": 0, "c": 1, "b": 2}
 SIDE = {"left": 0, "right": 1}
 
 
@@ -122,6 +123,8 @@ SIGNATURES = [
                               C.POINTER(i64)]),
     ("lra_pqrfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                               C.POINTER(lra_pqr_out)]),
+    ("lra_prange", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                             C.POINTER(lra_pqr_out)]),
     ("lra_psvdfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts),
                                C.POINTER(lra_psvd_out)]),
     ("lra_psvdvals", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_D,

@@ -530,6 +530,42 @@ def pqrr(R, T):
     return np.hstack([R, np.triu(R[:, :k]) @ T])
 
 
+def prange(trans: str, A, opts: LRAOptions = None, rand: Rand = None, rand2: Rand = None,
+           **kw) -> np.ndarray:
+    """prange (src/prange.jl:14-77) and prange_sub (:65-78).  trans "n"/"c": randn takes the
+    right-side sketchfact's Q (pqrfact_retval "q"), sub the QR of the left sketchfact's pivot
+    columns, none pqrfact(op(A))[:Q]; "b" (square A): both right sketchfacts (`rand` for A',
+    `rand2` for A), B = [Q_r R_r, Q_c R_c], then pqrfact_backend!(B)'s Q."""
+    opts = (opts or LRAOptions()).copy(**kw)
+    chkopts(opts)
+    rand = rand or Rand(0)
+    A = np.asarray(A, dtype=np.float64)
+    if trans == "b":
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("prange(:b) needs a square matrix")
+        if ishermitian(A):
+            return prange("n", A, opts, rand)
+        o = opts.copy(pqrfact_retval="qr")
+        if opts.sketch == "none":
+            Fr = pqrfact_backend(np.asfortranarray(A.T), o)
+            Fc = pqrfact_backend(np.asfortranarray(A), o)
+        else:
+            Fr = sketchfact("right", "c", A, o, rand)
+            Fc = sketchfact("right", "n", A, o, rand2 or Rand(1))
+        kr, kc = Fr.k, Fc.k
+        B = np.hstack([Fr.Q @ np.triu(Fr.R[:kr, :kr]), Fc.Q @ np.triu(Fc.R[:kc, :kc])])
+        return pqrfact_backend(np.asfortranarray(B), opts.copy(pqrfact_retval="q")).Q
+    o = opts.copy(pqrfact_retval="q")
+    if opts.sketch == "none":
+        return pqrfact_backend(np.asfortranarray(_op(A, trans)), o).Q
+    if opts.sketch == "sub":
+        F = sketchfact("left", trans, A, opts.copy(pqrfact_retval=""), rand)
+        cols = np.asarray(F.p[:F.k], dtype=np.int64) - 1
+        B = _op(A, trans)[:, cols]
+        return np.linalg.qr(B)[0] if F.k > 0 else np.zeros((B.shape[0], 0))
+    return sketchfact("right", trans, A, o, rand).Q
+
+
 def pqrfact(trans: str, A, opts: LRAOptions = None, rand: Rand = None, **kw):
     """pqrfact (src/pqr.jl:290-307)."""
     opts = (opts or LRAOptions()).copy(**kw)

@@ -243,3 +243,32 @@ def test_psvd_recovers_known_singular_values(orc):
     S = orc.psvdfact(A, rand=orc.Rand(3), rtol=1e-10)
     assert len(S.S) == r
     assert np.abs(S.S - sig).max() <= 1e-12
+
+
+@pytest.mark.parametrize("trans", ["n", "c"])
+@pytest.mark.parametrize("sketch", ["randn", "none", "sub"])
+def test_oracle_prange_exact_rank(trans, sketch):
+    """prange of an exactly rank-r matrix: k = r and Q spans the true range (the projector
+    equals the one of the r leading left singular vectors of op(A))."""
+    import oracle as orc
+    rng = np.random.default_rng(3)
+    m, n, r = 120, 90, 17
+    A = rng.standard_normal((m, r)) @ rng.standard_normal((r, n))
+    X = A if trans == "n" else A.T
+    Q = orc.prange(trans, A, rtol=1e-12, sketch=sketch)
+    assert Q.shape == (X.shape[0], r)
+    U = np.linalg.svd(X)[0][:, :r]
+    assert np.abs(Q @ Q.T - U @ U.T).max() <= 1e-10
+    assert np.abs(Q.T @ Q - np.eye(r)).max() <= 1e-13
+
+
+def test_oracle_prange_both_spans_both():
+    import oracle as orc
+    rng = np.random.default_rng(4)
+    n, r = 80, 9
+    A = rng.standard_normal((n, r)) @ rng.standard_normal((r, n))
+    Q = orc.prange("b", A, rtol=1e-12)
+    # range(A) + range(A') has dimension <= 2r; Q spans both
+    assert Q.shape[1] <= 2 * r
+    for X in (A, A.T):
+        assert np.linalg.norm(X - Q @ (Q.T @ X), 2) <= 1e-10 * np.linalg.norm(X, 2)
