@@ -240,7 +240,7 @@ def serving_pool(torch, count: int, seed: int = 6):
     return pool
 
 
-def bench_serving(lra, torch, rates=(200, 1000, 5000), nreq: int = 32, pool_size: int = 12,
+def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 32, pool_size: int = 12,
                   workers: int = 4, budget_gb: int = 96, seed: int = 6):
     """Config 5: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
     1e-10 50/50) on resident matrices; per rate: requests/s completed and p50/p99 latency
