@@ -160,7 +160,7 @@ def kernel_blocks(torch, nblk: int = 4096, seed: int = 5):
 
 
 def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
-    """Config 4: batched idfact at rtol=1e-8 of 4096 uneven kernel blocks, the whole batch as
+    """Batched idfact at rtol=1e-8 of 4096 uneven kernel blocks, the whole batch as
     one captured launch (counter reset + one persistent kernel per adaptive round), replayed
     `reps` times; beside it the plain FP64 drop-in (idfact per block) on a 64-block sample."""
     blocks, mn = kernel_blocks(torch, nblk)
@@ -217,7 +217,7 @@ def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
 
 
 def serving_pool(torch, count: int, seed: int = 6):
-    """Config-5 matrices: m log-uniform in [256, 8192], square or tall (n log-uniform in
+    """Serving pool matrices: m log-uniform in [256, 8192], square or tall (n log-uniform in
     [256, m]) with probability 1/2, A = U diag(exp(-r (i-1))) V', r ~ U[0.01, 0.2], U and V
     Q factors of Gaussian matrices (singular values below exp(-40) sigma_1 dropped)."""
     rng = np.random.default_rng(seed)
@@ -242,7 +242,7 @@ def serving_pool(torch, count: int, seed: int = 6):
 
 def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 32, pool_size: int = 12,
                   workers: int = 4, budget_gb: int = 96, seed: int = 6):
-    """Config 5: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
+    """Serving: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
     1e-10 50/50) on resident matrices; per rate: requests/s completed and p50/p99 latency
     from submission to result (server clock)."""
     pool = serving_pool(torch, pool_size, seed)
@@ -292,7 +292,7 @@ def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 32
 
 
 def bench_srft(lra, torch, reps: int = 5, m: int = 65536, n: int = 4096, r: int = 1400):
-    """Config 3: SRFT vs Gaussian sketch on a tall m x n matrix A = U diag(exp(-i/40)) V'
+    """SRFT vs Gaussian sketch on a tall m x n matrix A = U diag(exp(-i/40)) V'
     (U, V Q factors of Gaussians, seeds 3 and 4; exp(-r/40) < 1e-15 beyond r), idfact with
     rank 300, sketchfact_adap = false and samp n -> n + 10 (order 310).  The SRFT sketch is
     priced on HBM: 8mn + 8 ell n + 8m bytes per launch."""
@@ -342,7 +342,7 @@ def bench_srft(lra, torch, reps: int = 5, m: int = 65536, n: int = 4096, r: int 
                                  "achieved": round(tf, 2), "peak": PEAK_FP64_TFLOPS,
                                  "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4)}
         out[kind] = entry
-        print(f"# config 3 {kind}: {entry}", file=sys.stderr, flush=True)
+        print(f"# srft vs gaussian, {kind}: {entry}", file=sys.stderr, flush=True)
     return out
 
 
