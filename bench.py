@@ -50,7 +50,8 @@ def load_pkg():
 
 
 def make_matrix(torch, n: int, r: int = 512):
-    """Config-2 matrix on the device (input generation, not timed)."""
+    """The n x n workload matrix A = U diag(sigma) V' on the device (input generation, not
+    timed)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(1)
     U = torch.randn(n, r, dtype=torch.float64, device="cuda", generator=g)
@@ -144,7 +145,7 @@ def kernel_roofline(lra, torch, A, reps=5):
 
 
 def kernel_blocks(torch, nblk: int = 4096, seed: int = 5):
-    """Config-4 blocks on the device: m, n = round(64 16^U), K[i,j] = 1/|x_i - y_j| with x_i
+    """The batched workload's blocks on the device: m, n = round(64 16^U), K[i,j] = 1/|x_i - y_j| with x_i
     uniform in [0,1]^3 and y_j in [3,4] x [0,1]^2 (input generation, not timed)."""
     rng = np.random.default_rng(seed)
     mn = np.rint(64 * 16.0 ** rng.random((nblk, 2))).astype(int)
