@@ -287,6 +287,17 @@ int lra_sketchfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* 
 int lra_curfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
                 const lra_opts* opts, int64_t* rows, int64_t* cols, int64_t* k);
 
+/* CUR(A, CURPackedU) (src/cur.jl:85-93; HermCUR / SymCUR :102-116 invert the same
+ * intersection): from a row and column skeleton (rows, cols: device, k entries, 1-based),
+ * C = A[:, cols] (m x k), R = A[rows, :] (k x n), and U = pinv(A[rows, cols]) as
+ * PartialSVD(V, 1 ./ sigma, U') of the intersection's SVD W = U diag(sigma) V':
+ * V (k x k), sinv = 1 ./ sigma (k, 1/0 = Inf as in the reference), Ut = U' (k x k).
+ * C or R may be NULL (not formed). */
+int lra_cur_build(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                  const int64_t* rows, const int64_t* cols, int64_t k, double* C, int64_t ldc,
+                  double* R, int64_t ldr, double* V, int64_t ldv, double* sinv, double* Ut,
+                  int64_t ldut);
+
 /* pqrfact(trans, A, opts) (src/pqr.jl:285-320): the ID of op(A), then the thin QR of its
  * skeleton columns; pqrfact_retval bits choose Q, R, T.  sketch = none is the rank-adaptive
  * QRCP of op(A) itself (pqrfact_none, :323-327). */

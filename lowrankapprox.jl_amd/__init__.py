@@ -27,7 +27,7 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
            "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
-           "BatchedID", "idfact_batched", "Server", "prange"]
+           "BatchedID", "idfact_batched", "Server", "prange", "CUR"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -898,6 +898,57 @@ class CURPackedU:
         if key == "cols": return self.cols
         if key == "k": return int(self.rows.shape[0])
         raise KeyError(key)
+
+
+@dataclasses.dataclass
+class CUR:
+    """CUR (src/cur.jl:60-66): A ~ C U R with C = A[:, cols], R = A[rows, :] and U the
+    pseudo-inverse of the intersection A[rows, cols] as PartialSVD(V, 1 ./ sigma, U') (the
+    construction at src/cur.jl:85-93; for a symmetric skeleton rows = cols, the HermCUR /
+    SymCUR forms of :102-116 invert the same intersection)."""
+    rows: object
+    cols: object
+    C: object
+    U: PartialSVD
+    R: object
+
+    @classmethod
+    def from_packed(cls, A, U: "CURPackedU") -> "CUR":
+        """CUR(A, U::CURPackedU)."""
+        return cls.from_skeletons(A, U.rows, U.cols)
+
+    @classmethod
+    def from_skeletons(cls, A, rows, cols) -> "CUR":
+        """CUR(A, rows, cols) (1-based skeletons)."""
+        torch = _torch()
+        ctx = context()
+        A = to_device(A)
+        m, n = A.shape
+        def idx(v):
+            if not isinstance(v, torch.Tensor):
+                v = torch.as_tensor(np.asarray(v, dtype=np.int64))
+            return v.to(device="cuda", dtype=torch.int64).contiguous()
+        rows, cols = idx(rows), idx(cols)
+        k = int(rows.shape[0])
+        if int(cols.shape[0]) != k:
+            raise ValueError("rows and cols differ in length")
+        Cm, Rm = empty_colmajor(m, k), empty_colmajor(k, n)
+        V, Ut = empty_colmajor(k, k), empty_colmajor(k, k)
+        sinv = torch.empty(k, dtype=torch.float64, device="cuda")
+        check(_lib.load().lra_cur_build(ctx.h, m, n, _ptr(A), _ld(A), _ptr(rows), _ptr(cols), k,
+                                        _ptr(Cm), max(1, m), _ptr(Rm), max(1, k), _ptr(V),
+                                        max(1, k), _ptr(sinv), _ptr(Ut), max(1, k)),
+              "lra_cur_build")
+        return cls(rows, cols, Cm, PartialSVD(V, sinv, Ut), Rm)
+
+    def apply(self, X, trans: str = "n"):
+        """C (U (R X)) (trans "n") or R' (U' (C' X)) ("c") (mul!, src/cur.jl:209-523)."""
+        X = to_device(X)
+        if X.dim() == 1:
+            X = X.reshape(-1, 1)
+        if trans == "n":
+            return dgemm(self.C, self.U.apply(dgemm(self.R, X)))
+        return dgemm(self.R, self.U.apply(dgemm(self.C, X, transa="c"), "c"), transa="c")
 
 
 def curfact(A, opts: "LRAOptions" = None, **kw) -> CURPackedU:

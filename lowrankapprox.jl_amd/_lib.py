@@ -125,6 +125,8 @@ SIGNATURES = [
                               C.POINTER(lra_pqr_out)]),
     ("lra_prange", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                              C.POINTER(lra_pqr_out)]),
+    ("lra_cur_build", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.c_void_p, C.c_void_p, i64,
+                                P_D, i64, P_D, i64, P_D, i64, P_D, P_D, i64]),
     ("lra_psvdfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts),
                                C.POINTER(lra_psvd_out)]),
     ("lra_psvdvals", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_D,

@@ -416,13 +416,8 @@ template <int W, int MR>
 void launch_block_wm(int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
                      double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot,
                      const int* prev) {
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)jacobi_block_kernel<W, MR>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kJacobiLds);
-    (void)hipGetLastError();
-    attr = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  lra_lds_limit_once(lds_set, (const void*)jacobi_block_kernel<W, MR>, (int)kJacobiLds);
   hipLaunchKernelGGL((jacobi_block_kernel<W, MR>), dim3((unsigned)(NB / 2)), dim3(512), lds, st,
                      m, n, NB, r, first, X, ldx, V, ldv, tol, rot, prev);
 }

@@ -6,6 +6,8 @@
 
 #include "../../include/lra.h"
 
+#include <atomic>
+
 typedef int64_t i64;
 
 // ---------------------------------------------------------------------------------------
@@ -82,10 +84,22 @@ enum WsSlot {
   WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
   WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
   WS_PI0, WS_PI1, WS_PI2, WS_QG, WS_PR0, WS_PR1, WS_PR2, WS_PR3, WS_PR4, WS_PR5, WS_PR6,
-  WS_PR7, WS_PR8, WS_PR9, WS_PR10, WS_PR11, WS_PR12, WS_COUNT
+  WS_PR7, WS_PR8, WS_PR9, WS_PR10, WS_PR11, WS_PR12, WS_CU0, WS_COUNT
 };
 static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
 
+// Raise the dynamic-LDS limit of kernel `fn` to `bytes` once per device.  `done` holds one
+// bit per device; a bit is set only after the attribute call returned, so a caller that sees
+// it set may launch, and concurrent first callers at worst set the attribute twice.
+inline void lra_lds_limit_once(std::atomic<uint64_t>& done, const void* fn, int bytes) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const uint64_t bit = 1ull << (dev & 63);
+  if (done.load(std::memory_order_acquire) & bit) return;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipGetLastError();
+  done.fetch_or(bit, std::memory_order_release);
+}
 int lra_chkopts_public(const lra_opts* o);  // chkopts! (0 or -(bad field))
 int lra_ws(lra_ctx* ctx, int slot, size_t bytes, void** out);
 template <class T>

@@ -145,15 +145,9 @@ int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, doub
       const size_t bytes = (size_t)(kp + 1) * nc * sizeof(double);
       if (bytes > cap) continue;
       const i64 blocks = (nr + nc - 1) / nc;
-      static bool attr_set = false;
-      if (!attr_set) {  // dynamic LDS beyond 64 KiB
-        (void)hipFuncSetAttribute((const void*)id_T_mfma_kernel<32>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
-        (void)hipFuncSetAttribute((const void*)id_T_mfma_kernel<16>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
-        (void)hipGetLastError();
-        attr_set = true;
-      }
+      static std::atomic<uint64_t> lds32{0}, lds16{0};  // dynamic LDS beyond 64 KiB
+      lra_lds_limit_once(lds32, (const void*)id_T_mfma_kernel<32>, (int)cap);
+      lra_lds_limit_once(lds16, (const void*)id_T_mfma_kernel<16>, (int)cap);
       if (nc == 32)
         hipLaunchKernelGGL(id_T_mfma_kernel<32>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
                            nr, R, ldr, T, ldt);

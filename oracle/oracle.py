@@ -773,6 +773,21 @@ def curfact(A, opts: LRAOptions = None, rand: Rand = None, rand2: Rand = None, *
     return rows[:k], cols[:k]
 
 
+def cur_build(A, rows, cols):
+    """CUR(A, CURPackedU) (src/cur.jl:85-93): C = A[:, cols], R = A[rows, :], and U =
+    PartialSVD(V, 1 ./ sigma, U') from the SVD of W = A[rows, cols] (1-based skeletons).
+    Returns (C, (V, sinv, Ut), R)."""
+    A = np.asarray(A, dtype=np.float64)
+    r = np.asarray(rows, dtype=np.int64) - 1
+    c = np.asarray(cols, dtype=np.int64) - 1
+    C = A[:, c]
+    R = A[r, :]
+    Uw, s, Vt = np.linalg.svd(A[np.ix_(r, c)])
+    with np.errstate(divide="ignore"):
+        sinv = 1.0 / s
+    return C, (Vt.T, sinv, Uw.T), R
+
+
 def curfact_none(A, opts: LRAOptions):
     """curfact_none (src/cur.jl:574-606)."""
     if ishermitian(A):

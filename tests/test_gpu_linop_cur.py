@@ -1,5 +1,6 @@
-"""Matrix-free idfact/snorm (LinearOperator, src/linop.jl) and curfact (src/cur.jl:532-564)
-on the GPU against the oracle with the same injected sketches."""
+"""Matrix-free idfact/snorm (LinearOperator, src/linop.jl), curfact (src/cur.jl:532-564) and
+the CUR factors (src/cur.jl:85-93) on the GPU against the oracle with the same injected
+sketches."""
 import numpy as np
 import pytest
 
@@ -7,6 +8,10 @@ from test_gpu_idfact import DeviceOmega, pivots_match, id_err
 
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(float).eps
+
+
+def host(t):
+    return t.detach().cpu().numpy()
 
 
 class DeviceOmegaBase(DeviceOmega):
@@ -101,3 +106,41 @@ def test_sketchfact_matches_idfact_pivots(gpu):
     V = gpu.idfact(A, seed=2, rtol=1e-10)
     assert k == V.k == 12
     assert np.array_equal(p[:k].cpu().numpy(), V.sk.cpu().numpy())
+
+
+@pytest.mark.parametrize("m,n", [(400, 300), (250, 600)])
+def test_cur_build_vs_oracle(gpu, orc, m, n):
+    """CUR(A, curfact(A)) (src/cur.jl:85-93): the gathered C and R are A's entries exactly,
+    U = pinv(A[rows, cols]) matches the oracle's product V diag(1/sigma) U', and C U R
+    approximates A to the ID's accuracy (exactly for the exact-rank part)."""
+    rng = np.random.default_rng(m + n)
+    k0 = 40
+    A = rng.standard_normal((m, k0)) @ np.diag(np.exp(-0.3 * np.arange(k0))) @ \
+        rng.standard_normal((k0, n))
+    A = np.asfortranarray(A)
+    print(f"cur build {m}x{n}", flush=True)
+    U = gpu.curfact(A, rtol=1e-9, seed=2)
+    F = gpu.CUR.from_packed(A, U)
+    rows, cols = host(F.rows), host(F.cols)
+    assert np.array_equal(host(F.C), A[:, cols - 1])
+    assert np.array_equal(host(F.R), A[rows - 1, :])
+    Co, (Vo, so, Uto), Ro = orc.cur_build(A, rows, cols)
+    Wp = host(F.U.U) @ np.diag(host(F.U.S)) @ host(F.U.Vt)
+    Wpo = Vo @ np.diag(so) @ Uto
+    assert np.abs(Wp - Wpo).max() <= 1e-8 * np.abs(Wpo).max()
+    X = rng.standard_normal((n, 3))
+    Y = host(F.apply(X))
+    assert np.abs(Y - Co @ Wpo @ (Ro @ X)).max() <= 1e-8 * np.abs(Y).max()
+    Yc = host(F.apply(rng.standard_normal((m, 2)), "c"))
+    assert Yc.shape == (n, 2)
+    err = np.linalg.norm(A - host(F.C) @ Wp @ host(F.R), 2)
+    assert err <= 1e-6 * np.linalg.norm(A, 2)
+
+
+def test_cur_build_from_skeleton_lists(gpu):
+    A = np.asfortranarray(np.random.default_rng(1).standard_normal((50, 40)))
+    F = gpu.CUR.from_skeletons(A, [3, 7, 11], [1, 2, 30])
+    assert host(F.C).shape == (50, 3) and host(F.R).shape == (3, 40)
+    W = A[np.ix_([2, 6, 10], [0, 1, 29])]
+    Wp = host(F.U.U) @ np.diag(host(F.U.S)) @ host(F.U.Vt)
+    assert np.allclose(Wp, np.linalg.inv(W), rtol=1e-10, atol=1e-12)

@@ -122,3 +122,12 @@ def test_prange_errors(gpu):
         gpu.prange("n", A, sketch="srft")  # no right-side SRFT
     with pytest.raises(ValueError):
         gpu.prange("x", A)
+
+
+@pytest.mark.parametrize("shape,trans", [((0, 5), "n"), ((7, 0), "n"), ((0, 5), "c"),
+                                         ((6, 0), "c")])
+def test_prange_empty(gpu, shape, trans):
+    A = np.zeros(shape, order="F")
+    Q = gpu.prange(trans, A)
+    P = shape[0] if trans == "n" else shape[1]
+    assert tuple(Q.shape) == (P, 0)

@@ -272,3 +272,19 @@ def test_oracle_prange_both_spans_both():
     assert Q.shape[1] <= 2 * r
     for X in (A, A.T):
         assert np.linalg.norm(X - Q @ (Q.T @ X), 2) <= 1e-10 * np.linalg.norm(X, 2)
+
+
+def test_oracle_cur_build_exact_rank():
+    """A = C W^+ R when rank(W) = rank(A) = k: the CUR factors of an exactly rank-k matrix
+    from its skeleton reproduce it, and U is the pseudo-inverse of the intersection."""
+    import oracle as orc
+    rng = np.random.default_rng(5)
+    m, n, k = 90, 70, 12
+    A = rng.standard_normal((m, k)) @ rng.standard_normal((k, n))
+    rows, cols = orc.curfact(A, rtol=1e-12)
+    assert len(rows) == k and len(cols) == k
+    C, (V, sinv, Ut), R = orc.cur_build(A, rows, cols)
+    Wp = V @ np.diag(sinv) @ Ut
+    W = A[np.ix_(np.asarray(rows) - 1, np.asarray(cols) - 1)]
+    assert np.allclose(Wp, np.linalg.pinv(W), rtol=1e-8, atol=1e-10 * np.abs(Wp).max())
+    assert np.abs(C @ Wp @ R - A).max() <= 1e-9 * np.abs(A).max()
