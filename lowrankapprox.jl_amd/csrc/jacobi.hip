@@ -267,6 +267,18 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
   if (nrot) any = 1;  // benign race: every writer stores 1
   __syncthreads();
   if (!any) return;
+  // V(:, block cols) <- V(:, block cols) * Q on MFMA, as V' <- Q' V': the block of V (n x 2W)
+  // is staged in LDS over Xs; lane l of a 16 x 16 output tile then holds 16 consecutive rows of
+  // one column of V, so the stores are coalesced.  The first 512 rows of the block are loaded
+  // before X is written back, so the write-back covers their latency.
+  double vt[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int g = gcol(c);
+    const bool ok = tid < n && g < n;
+    const double x = V[ok ? (i64)g * ldv + tid : 0];
+    vt[c] = ok ? x : 0.0;
+  }
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int g = gcol(c);
@@ -274,12 +286,13 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
     double* x = X + (i64)g * ldx;
     for (int i = tid; i < m; i += 512) x[i] = Xs[c * mps + i];
   }
-  // V(:, block cols) <- V(:, block cols) * Q on MFMA, as V' <- Q' V': the block of V (n x 2W)
-  // is staged in LDS over Xs; lane l of a 16 x 16 output tile then holds 16 consecutive rows of
-  // one column of V, so the stores are coalesced.
   __syncthreads();  // Xs has been written out
   double* Vs = Xs;  // Vs[t * mps + i] = V(i, gcol(t))
-  for (int i0 = 0; i0 < n; i0 += 512) {
+  if (tid < n) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) Vs[c * mps + tid] = vt[c];
+  }
+  for (int i0 = 512; i0 < n; i0 += 512) {
     const int i = i0 + tid;
     double tmp[C];
 #pragma unroll
@@ -299,6 +312,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
     constexpr int KS = (C + 3) / 4;        // k-steps of 4
     constexpr int CT = (C + 15) / 16;      // 16-column tiles of Q
     constexpr int WPT = NW / CT;           // waves per column tile
+    constexpr int TB = 2;                  // row tiles per wave whose operands load together
     const int ct = wave / WPT, rt0 = wave % WPT;
     const int rtiles = (n + 15) / 16;
     const int cq = ct * 16 + (lane & 15);  // A = Q' (row cq, k): Q(k, cq)
@@ -308,26 +322,35 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
       const int k = ks * 4 + (lane >> 4);
       qa[ks] = (k < C && cq < C) ? Qs[k * QS + cq] : 0.0;
     }
-    for (int rt = rt0; rt < rtiles; rt += WPT) {
-      const int i = rt * 16 + (lane & 15);  // B = V' (k, i): V(i, gcol(k))
-      double vb[KS];
+    for (int rtb = rt0; rtb < rtiles; rtb += TB * WPT) {
+      double vb[TB][KS];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int k = ks * 4 + (lane >> 4);
-        const bool ok = k < C && i < n;
-        const double x = Vs[ok ? k * mps + i : 0];
-        vb[ks] = ok ? x : 0.0;
+      for (int u = 0; u < TB; ++u) {
+        const int i = (rtb + u * WPT) * 16 + (lane & 15);  // B = V' (k, i): V(i, gcol(k))
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int k = ks * 4 + (lane >> 4);
+          const bool ok = k < C && i < n;
+          const double x = Vs[ok ? k * mps + i : 0];
+          vb[u][ks] = ok ? x : 0.0;
+        }
       }
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[ks], vb[ks], acc, 0, 0, 0);
+      for (int u = 0; u < TB; ++u) {
+        const int rt = rtb + u * WPT;
+        if (rt >= rtiles) break;
+        const int i = rt * 16 + (lane & 15);
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int c = ct * 16 + (lane >> 4) + 4 * q;  // output (c, i) = V_new(i, gcol(c))
-        if (c < C && i < n) {
-          const int g = gcol(c);
-          if (g < n) V[i + (i64)g * ldv] = acc[q];
+        for (int ks = 0; ks < KS; ++ks)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[ks], vb[u][ks], acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = ct * 16 + (lane >> 4) + 4 * q;  // output (c, i) = V_new(i, gcol(c))
+          if (c < C && i < n) {
+            const int g = gcol(c);
+            if (g < n) V[i + (i64)g * ldv] = acc[q];
+          }
         }
       }
     }
