@@ -371,6 +371,22 @@ def dgemm(A, B, transa: str = "n", transb: str = "n", alpha: float = 1.0, C=None
     return C
 
 
+def _gesvj(X):
+    """One-sided Jacobi SVD X = U diag(s) V' of an m x n (m >= n) matrix (X is copied)."""
+    torch = _torch()
+    ctx = context()
+    X0 = to_device(X)
+    m, n = X0.shape
+    X = empty_colmajor(m, n)
+    X.copy_(X0)
+    U, V = empty_colmajor(m, n), empty_colmajor(n, n)
+    s = torch.empty(n, dtype=torch.float64, device="cuda")
+    if n:
+        check(_lib.load().lra_gesvj_f64(ctx.h, m, n, _ptr(X), _ld(X), _ptr(U), max(1, m),
+                                        _ptr(s), _ptr(V), max(1, n)), "lra_gesvj_f64")
+    return U, s, V
+
+
 def _scale_rows(d, X):
     ctx = context()
     check(_lib.load().lra_scale_rows(ctx.h, X.shape[0], X.shape[1], _ptr(d), _ptr(X), _ld(X)),
@@ -681,11 +697,54 @@ class PartialQR:
     def k(self) -> int:
         return int(self.R.shape[0])
 
-    def apply(self, X):
-        """op(A) X ~ Q R P' X = Q (R X[p, :]) (mul!, src/pqr.jl:86-131)."""
+    def apply(self, X, trans: str = "n"):
+        """op(A) X ~ Q R P' X = Q (R X[p, :]) (trans "n"), or op(A)' X = P R' (Q' X) ("c")
+        (mul!, src/pqr.jl:86-131)."""
+        torch = _torch()
         X = to_device(X)
-        Xp = X[self.p - 1, :].t().contiguous().t()
-        return dgemm(self.Q, dgemm(self.R, Xp))
+        if X.dim() == 1:
+            X = X.reshape(-1, 1)
+        if trans == "n":
+            Xp = X[self.p - 1, :].t().contiguous().t()
+            return dgemm(self.Q, dgemm(self.R, Xp))
+        Y = dgemm(self.R, dgemm(self.Q, X, transa="c"), transa="c")  # R' (Q' X), rows in p order
+        out = torch.empty_like(Y)
+        out[self.p - 1, :] = Y
+        return out.t().contiguous().t()
+
+    def rapply(self, X, trans: str = "n"):
+        """X op(A) = ((X Q) R) P' (trans "n") or X op(A)' = ((X P) R') Q' ("c")
+        (right multiplication, src/pqr.jl:133-158)."""
+        torch = _torch()
+        X = to_device(X)
+        if trans == "n":
+            Y = dgemm(dgemm(X, self.Q), self.R)  # columns in p order
+            out = torch.empty_like(Y)
+            out[:, self.p - 1] = Y
+            return out.t().contiguous().t()
+        Xp = X[:, self.p - 1].t().contiguous().t()
+        return dgemm(dgemm(Xp, self.R, transb="c"), self.Q, transb="c")
+
+    def matrix(self):
+        """Matrix(A) = Q R P' (src/pqr.jl:56)."""
+        torch = _torch()
+        return self.rapply(torch.eye(self.Q.shape[0], dtype=torch.float64, device="cuda"))
+
+    def ldiv(self, B):
+        """A \\ B, the minimum-norm least-squares solve P R^+ Q' B (ldiv!, src/pqr.jl:165-175).
+        The reference forms the normal equations (R R') \\ (Q' B) then R' times that; here R^+
+        comes from the one-sided Jacobi SVD of R' (n x k) = U diag(s) V', so
+        R^+ = U diag(1/s) V', which equals R' (R R')^-1 for the full-row-rank R that pqrfact
+        returns without squaring its condition number."""
+        torch = _torch()
+        B = to_device(B)
+        if B.dim() == 1:
+            B = B.reshape(-1, 1)
+        U, s, V = _gesvj(self.R.t())
+        Y = dgemm(U, _scale_rows(1.0 / s, dgemm(V, dgemm(self.Q, B, transa="c"), transa="c")))
+        out = torch.empty_like(Y)
+        out[self.p - 1, :] = Y
+        return out.t().contiguous().t()
 
 
 @dataclasses.dataclass

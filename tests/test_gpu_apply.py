@@ -51,3 +51,42 @@ def test_partial_factor_apply(gpu, orc):
     Me = (host(Fe.vectors) * host(Fe.values)) @ host(Fe.vectors).T
     Xe = rng.standard_normal((150, 3))
     assert np.allclose(host(Fe.apply(Xe)), Me @ Xe, atol=1e-9 * np.abs(S).max())
+
+
+@pytest.mark.parametrize("trans", ["n", "c"])
+def test_partial_qr_arithmetic(gpu, trans):
+    """PartialQR products on both sides, with and without adjoint, Matrix(A) and the
+    least-squares solve (src/pqr.jl:56, 86-175) against numpy on the same factors."""
+    rng = np.random.default_rng(5)
+    m, n = 260, 190
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = np.asfortranarray((U * np.exp(-0.1 * np.arange(n))) @ V.T)
+    print(f"partial qr arithmetic trans={trans}", flush=True)
+    F = gpu.pqrfact(trans, A, rtol=1e-6, seed=7)
+    Q, R, p = host(F.Q), host(F.R), host(F.p)
+    P, N = Q.shape[0], R.shape[1]
+    assert not np.tril(R, -1).any()  # upper trapezoidal, stored dense
+    M = np.empty((P, N))
+    M[:, p - 1] = Q @ R
+    assert np.allclose(host(F.matrix()), M, atol=1e-13)
+    X = rng.standard_normal((N, 4))
+    Y = rng.standard_normal((P, 3))
+    Z = rng.standard_normal((5, P))
+    W = rng.standard_normal((2, N))
+    tol = 1e-12 * np.abs(M).max() * max(P, N)
+    assert np.abs(host(F.apply(X)) - M @ X).max() <= tol
+    assert np.abs(host(F.apply(Y, "c")) - M.T @ Y).max() <= tol
+    assert np.abs(host(F.rapply(Z)) - Z @ M).max() <= tol
+    assert np.abs(host(F.rapply(W, "c")) - W @ M.T).max() <= tol
+    # A \ B = P R^+ Q' B, the minimum-norm least-squares solution: against numpy's
+    # least-squares solve of the same R, and the normal equations M'(M x - B) = 0
+    B = rng.standard_normal((P, 3))
+    x = host(F.ldiv(B))
+    xr = np.empty((N, 3))
+    xr[p - 1, :] = np.linalg.lstsq(R, Q.T @ B, rcond=None)[0]
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max(), np.abs(x - xr).max()
+    assert np.abs(M.T @ (M @ x - B)).max() <= 1e-8 * np.abs(M).max() * np.abs(B).max()
+    b = rng.standard_normal(P)
+    assert np.allclose(host(F.ldiv(b))[:, 0], np.linalg.lstsq(M, b, rcond=1e-9)[0],
+                       atol=1e-8 * np.abs(xr).max())
