@@ -394,6 +394,14 @@ def _scale_rows(d, X):
     return X
 
 
+def _right_apply(F, X, trans: str):
+    """X op(F) through F's left product: X op(F) = (op(F)' X')', returned column-major."""
+    X = to_device(X)
+    if X.dim() == 1:
+        X = X.reshape(1, -1)
+    return F.apply(X.t(), "c" if trans == "n" else "n").contiguous().t()
+
+
 @dataclasses.dataclass
 class ID:
     """ID (src/id.jl:212-235): A ~ C V with C = A[:, sk] (m x k) and V = [I T] P'."""
@@ -836,6 +844,21 @@ class PartialSVD:
             return dgemm(self.U, _scale_rows(self.S, dgemm(self.Vt, X)))
         return dgemm(self.Vt, _scale_rows(self.S, dgemm(self.U, X, transa="c")), transa="c")
 
+    def rapply(self, X, trans: str = "n"):
+        """X A (trans "n") or X A' ("c") (mul!, src/psvd.jl:99-116)."""
+        return _right_apply(self, X, trans)
+
+    def matrix(self):
+        """Matrix(A) = (U diag(S)) Vt (src/psvd.jl:38)."""
+        torch = _torch()
+        return self.apply(torch.eye(self.Vt.shape[1], dtype=torch.float64, device="cuda"))
+
+    def ldiv(self, B):
+        """A \\ B = V (diag(S)^-1 (U' B)), the pseudo-inverse product (ldiv!,
+        src/psvd.jl:119-124)."""
+        return dgemm(self.Vt, _scale_rows(1.0 / self.S, dgemm(self.U, B, transa="c")),
+                     transa="c")
+
 
 def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
     """psvdfact(A, opts) (src/psvd.jl:238-272)."""
@@ -880,10 +903,26 @@ class PartialHermEigen:
     values: object
     vectors: object
 
-    def apply(self, X):
-        """A X = V (w (V' X)) (mul!, src/pheig.jl:54-100)."""
+    def apply(self, X, trans: str = "n"):
+        """A X = V (w (V' X)) (mul!, src/pheig.jl:54-100); A' = A for the real Hermitian
+        factorization, so trans is accepted for symmetry with the other factors."""
         return dgemm(self.vectors, _scale_rows(self.values, dgemm(self.vectors, X,
                                                                    transa="c")))
+
+    def rapply(self, X, trans: str = "n"):
+        """X A = X A' (mul!, src/pheig.jl:114-145)."""
+        return _right_apply(self, X, trans)
+
+    def matrix(self):
+        """Matrix(A) = (vectors diag(values)) vectors' (src/pheig.jl:31)."""
+        torch = _torch()
+        return self.apply(torch.eye(self.vectors.shape[0], dtype=torch.float64, device="cuda"))
+
+    def ldiv(self, B):
+        """A \\ B = V (diag(w)^-1 (V' B)), the pseudo-inverse product (ldiv!,
+        src/pheig.jl:147-152)."""
+        return dgemm(self.vectors, _scale_rows(1.0 / self.values,
+                                               dgemm(self.vectors, B, transa="c")))
 
 
 def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
@@ -1008,6 +1047,15 @@ class CUR:
         if trans == "n":
             return dgemm(self.C, self.U.apply(dgemm(self.R, X)))
         return dgemm(self.R, self.U.apply(dgemm(self.C, X, transa="c"), "c"), transa="c")
+
+    def rapply(self, X, trans: str = "n"):
+        """X A (trans "n") or X A' ("c") (*, src/cur.jl:462-523)."""
+        return _right_apply(self, X, trans)
+
+    def matrix(self):
+        """Matrix(A) = C (U R) (src/cur.jl:153)."""
+        torch = _torch()
+        return self.apply(torch.eye(self.R.shape[1], dtype=torch.float64, device="cuda"))
 
 
 def curfact(A, opts: "LRAOptions" = None, **kw) -> CURPackedU:

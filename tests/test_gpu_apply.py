@@ -90,3 +90,52 @@ def test_partial_qr_arithmetic(gpu, trans):
     b = rng.standard_normal(P)
     assert np.allclose(host(F.ldiv(b))[:, 0], np.linalg.lstsq(M, b, rcond=1e-9)[0],
                        atol=1e-8 * np.abs(xr).max())
+
+
+def test_psvd_pheig_cur_arithmetic(gpu, orc):
+    """Right products, Matrix(A) and the pseudo-inverse solve of PartialSVD and
+    PartialHermEigen (src/psvd.jl:38, 99-124; src/pheig.jl:31, 114-152) and CUR's products
+    (src/cur.jl:153, 436-523) against numpy on the same factors."""
+    rng = np.random.default_rng(6)
+    A = orc.fourier_real(rng.random(210), rng.random(160))
+    m, n = A.shape
+    print("psvd/pheig/cur arithmetic", flush=True)
+    Fs = gpu.psvdfact(A, rtol=1e-8, seed=3)
+    U, S, Vt = host(Fs.U), host(Fs.S), host(Fs.Vt)
+    Md = (U * S) @ Vt
+    assert np.allclose(host(Fs.matrix()), Md, atol=1e-13)
+    Z = rng.standard_normal((3, m))
+    W = rng.standard_normal((2, n))
+    assert np.allclose(host(Fs.rapply(Z)), Z @ Md, atol=1e-11)
+    assert np.allclose(host(Fs.rapply(W, "c")), W @ Md.T, atol=1e-11)
+    B = rng.standard_normal((m, 3))
+    x = host(Fs.ldiv(B))
+    assert np.allclose(x, Vt.T @ ((U.T @ B) / S[:, None]), rtol=1e-10,
+                       atol=1e-12 * np.abs(x).max())
+    # the minimum-norm least-squares solution of the factored matrix
+    assert np.allclose(x, np.linalg.lstsq(Md, B, rcond=S[-1] / S[0] / 2)[0],
+                       atol=1e-6 * np.abs(x).max())
+
+    Sm = A[:150, :] @ A[:150, :].T
+    Sm = (Sm + Sm.T) / 2
+    Fe = gpu.pheigfact(Sm, rtol=1e-8, seed=4)
+    w, Ve = host(Fe.values), host(Fe.vectors)
+    Me = (Ve * w) @ Ve.T
+    sc = np.abs(Sm).max()
+    assert np.allclose(host(Fe.matrix()), Me, atol=1e-13 * sc)
+    Ze = rng.standard_normal((4, 150))
+    assert np.allclose(host(Fe.rapply(Ze)), Ze @ Me, atol=1e-11 * sc)
+    Be = rng.standard_normal((150, 2))
+    xe = host(Fe.ldiv(Be))
+    assert np.allclose(xe, Ve @ ((Ve.T @ Be) / w[:, None]), rtol=1e-10,
+                       atol=1e-12 * np.abs(xe).max())
+
+    F = gpu.CUR.from_packed(A, gpu.curfact(A, rtol=1e-6, seed=5))
+    Cm, Rm = host(F.C), host(F.R)
+    Um = host(F.U.matrix())
+    Mc = Cm @ Um @ Rm
+    # rounding scale of C U R products: |C| |U| |R| k^2 eps
+    tol = 1e-14 * np.abs(Cm).max() * np.abs(Um).max() * np.abs(Rm).max() * Um.shape[0] ** 2
+    assert np.abs(host(F.matrix()) - Mc).max() <= tol
+    assert np.abs(host(F.rapply(Z)) - Z @ Mc).max() <= tol * np.abs(Z).sum(1).max()
+    assert np.abs(host(F.rapply(W, "c")) - W @ Mc.T).max() <= tol * np.abs(W).sum(1).max()
