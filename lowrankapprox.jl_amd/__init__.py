@@ -62,6 +62,13 @@ class LRAOptions:
     precision: str = "f64"
     seed: int = 0
 
+    @classmethod
+    def for_type(cls, eltype: str = "f64", **kw) -> "LRAOptions":
+        """LRAOptions(T) (src/LowRankApprox.jl:96-119): rtol = 5 eps(T) and
+        pheig_orthtol = sqrt(eps(T)) for the element type T ("f64" or "f32")."""
+        eps = _EPS if eltype == "f64" else float(np.finfo(np.float32).eps)
+        return cls(rtol=5 * eps, pheig_orthtol=math.sqrt(eps)).copy(**kw)
+
     def copy(self, **kw) -> "LRAOptions":
         o = dataclasses.replace(self)
         for k, v in kw.items():
@@ -121,9 +128,29 @@ def context(device: Optional[int] = None) -> _lib.Context:
     return ctx
 
 
+def _eltype(A) -> str:
+    """The reference's element type of an input: "f32" for float32 data, "f64" otherwise.
+    Float32 inputs take Float32's default tolerances and are factored in FP64 arithmetic.
+    Complex inputs raise: no complex kernels are built."""
+    torch = _torch()
+    dt = getattr(A, "dtype", None)
+    if dt is None:
+        return "f64"
+    if isinstance(A, np.ndarray):
+        if np.iscomplexobj(A):
+            raise TypeError("complex element types are not supported")
+        return "f32" if A.dtype == np.float32 else "f64"
+    if isinstance(A, torch.Tensor):
+        if A.is_complex():
+            raise TypeError("complex element types are not supported")
+        return "f32" if A.dtype == torch.float32 else "f64"
+    return "f64"
+
+
 def to_device(A, dtype=None):
     """Column-major float64 CUDA tensor for A (numpy array or tensor)."""
     torch = _torch()
+    _eltype(A)  # complex data is rejected, never silently truncated to its real part
     if isinstance(A, np.ndarray):
         A = torch.from_numpy(np.asfortranarray(A, dtype=np.float64 if dtype is None else dtype))
     if A.dim() == 1:
@@ -172,7 +199,7 @@ def gaussian(rows: int, cols: int, seed: int = 0, stream: int = 0):
 def sketch(side: str, trans: str, A, order: int, opts: LRAOptions = None, omega=None,
            stream: int = 0, **kw):
     """sketch (src/sketch.jl:35-50), Gaussian case: B = Omega*op(A) or op(A)*Omega."""
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     if order < 0:
         raise ValueError("order")
     if side not in _lib.SIDE: raise ValueError("side")
@@ -429,7 +456,7 @@ def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
         trans, A, opts = "n", trans, A
     if isinstance(A, LinearOperator):
         return A.idfact(trans, opts, **kw)
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     opts.pqrfact_retval = "t"
     ctx = context()
     A = to_device(A)
@@ -462,7 +489,8 @@ class BatchedID:
 
     def __init__(self, blocks, opts: LRAOptions = None, ldt: Optional[int] = None, **kw):
         torch = _torch()
-        opts = (opts or LRAOptions()).copy(**kw)
+        first = blocks[0] if len(blocks) else None
+        opts = (opts or LRAOptions.for_type(_eltype(first))).copy(**kw)
         opts.pqrfact_retval = "t"
         self.opts = opts
         self.ctx = context()
@@ -770,7 +798,7 @@ def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
     torch = _torch()
     if A is None or isinstance(A, LRAOptions):
         trans, A, opts = "n", trans, A
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -801,7 +829,7 @@ def prange(trans, A=None, opts: LRAOptions = None, **kw):
         trans, A, opts = "n", trans, A
     if trans not in _lib.PRANGE_TRANS:
         raise ValueError("trans")
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -862,7 +890,7 @@ class PartialSVD:
 
 def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
     """psvdfact(A, opts) (src/psvd.jl:238-272)."""
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -879,7 +907,7 @@ def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
 def psvdvals(A, opts: LRAOptions = None, **kw):
     """psvdvals(A, opts) (src/psvd.jl:274-290)."""
     torch = _torch()
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -927,7 +955,7 @@ class PartialHermEigen:
 
 def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
     """pheigfact(A, opts) (src/pheig.jl:276-297) for a symmetric A."""
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     n = A.shape[0]
@@ -945,7 +973,7 @@ def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
 def pheigvals(A, opts: LRAOptions = None, **kw):
     """pheigvals(A, opts) (src/pheig.jl:298-314)."""
     torch = _torch()
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     n = A.shape[0]
@@ -971,7 +999,7 @@ def sketchfact(trans, A=None, opts: "LRAOptions" = None, stream_base: int = 0, *
     torch = _torch()
     if A is None or isinstance(A, LRAOptions):
         trans, A, opts = "n", trans, A
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -1061,7 +1089,7 @@ class CUR:
 def curfact(A, opts: "LRAOptions" = None, **kw) -> CURPackedU:
     """curfact(A, opts) (src/cur.jl:532-564)."""
     torch = _torch()
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -1094,7 +1122,7 @@ def snorm(A, opts: LRAOptions = None, x0=None, **kw) -> float:
     """Spectral norm by power iteration on A'A (A itself when Hermitian)."""
     if isinstance(A, LinearOperator):
         return A.snorm(opts, x0, **kw)
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -1108,7 +1136,7 @@ def snorm(A, opts: LRAOptions = None, x0=None, **kw) -> float:
 
 def snormdiff(A, B, opts: LRAOptions = None, x0=None, **kw) -> float:
     """snorm(A - B); B may be a dense matrix or an IDPackedV of A (applied matrix-free)."""
-    opts = (opts or LRAOptions()).copy(**kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -1155,7 +1183,7 @@ class Server:
 
     def submit(self, A, kind: str = "idfact", opts: LRAOptions = None, **kw) -> int:
         torch = _torch()
-        opts = (opts or LRAOptions()).copy(**kw)
+        opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
         if kind == "idfact":
             opts.pqrfact_retval = "t"
         A = to_device(A)
