@@ -13,6 +13,7 @@ and the serving loop under Poisson arrivals (p50/p99 latency and completed reque
 from __future__ import annotations
 
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
@@ -114,6 +115,11 @@ def kernel_roofline(lra, torch, A, reps=5):
     e1.synchronize()
     ms_g = e0.elapsed_time(e1) / reps
     gflops = 2.0 * ell * n * n
+    cfg, spl, bm, bn = (ctypes.c_int() for _ in range(4))
+    tile = None
+    if lra._lib.load().lra_gemm_choice(0, 1, 0, ell, n, n, ctypes.byref(cfg), ctypes.byref(spl),
+                                       ctypes.byref(bm), ctypes.byref(bn)) == 0:
+        tile = f"{bm.value}x{bn.value} tile, split-K {spl.value}"
     # final-round QRCP on a sketch of A (the pivoted QR is in place: factor fresh copies)
     B0 = lra.sketch("left", "n", A, ell, seed=5, stream=4)
     Bs = [B0.clone() for _ in range(reps + 1)]
@@ -138,7 +144,7 @@ def kernel_roofline(lra, torch, A, reps=5):
              "note": "latency-bound: one grid-wide argmax + reflector exchange per pivot; "
                      f"{1e3 * ms_q / max(k, 1):.2f} us per step"},
             {"kernel": f"gemm_kc_f64_kernel, autotuned tile (sketch {ell}x{n}x{n})",
-             "bound": "mfma",
+             "tile": tile, "bound": "mfma",
              "achieved": round(g, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
              "frac": round(g / PEAK_FP64_TFLOPS, 4), "ms_per_launch": round(ms_g, 4),
              "flops_per_launch": gflops})

@@ -136,6 +136,9 @@ SIGNATURES = [
     ("lra_pheigvals", C.c_int, [C.c_void_p, i64, P_D, i64, C.POINTER(lra_opts), P_D,
                                 C.POINTER(i64)]),
     ("lra_gesvj_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64, P_D, P_D, i64]),
+    ("lra_gemm_choice", C.c_int, [C.c_int, C.c_int, C.c_int, i64, i64, i64,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
     ("lra_dgemm", C.c_int, [C.c_void_p, C.c_int, C.c_int, i64, i64, i64, C.c_double, P_D, i64,
                             P_D, i64, C.c_double, P_D, i64]),
     ("lra_id_apply", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_I, P_I, P_D, i64, i64, P_D, i64,
