@@ -204,10 +204,14 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     const i64* __restrict__ idx, int ell, double* __restrict__ B, i64 ldb) {
   constexpr int L = L1 * L2;
   constexpr int QC = 256 / (L1 > L2 ? L1 : L2);  // sequences per slab
+  // LDS padding: each k1 block of Y and each row of Z is one complex longer than its data, so
+  // the lanes of a wave that read different k1 blocks (pass 2) or different frequency rows
+  // (the accumulation) start on different banks instead of all on one
+  constexpr int YB = L2 * QC + 1, ZS = QC + 1;
   extern __shared__ double lds[];
-  cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2][c]
-  cplx* Z = Y + (size_t)L * QC;           // [k][c]
-  cplx* tw = Z + (size_t)L * QC;          // w_l^j, j < l
+  cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2][c], k1 blocks of YB
+  cplx* Z = Y + (size_t)L1 * YB;          // [k][c], rows of ZS
+  cplx* tw = Z + (size_t)L * ZS;          // w_l^j, j < l
   const int tid = threadIdx.x;
   const i64 col = blockIdx.x;
   const double* x = A + col * lda;
@@ -262,6 +266,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     }
   };
   load(0);
+  __syncthreads();  // the twiddles are written by other waves than the ones reading them
   for (i64 q0 = 0; q0 < nq; q0 += 2 * QC) {
     // pass 1
     if (p1act) {
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
 #pragma unroll
       for (int a = 0; a < L1; ++a) {
         const cplx t = (a * p2 == 0) ? v[a] : cmul(tw[(a * p2) % L], v[a]);
-        Y[((size_t)a * L2 + p2) * QC + c1] = t;
+        Y[(size_t)a * YB + p2 * QC + c1] = t;
       }
     }
     if (q0 + 2 * QC < nq) load(q0 + 2 * QC);  // next slab, in flight during this one
@@ -281,10 +286,10 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     if (p2act) {
       cplx v[L2];
 #pragma unroll
-      for (int b = 0; b < L2; ++b) v[b] = Y[((size_t)k1 * L2 + b) * QC + c2];
+      for (int b = 0; b < L2; ++b) v[b] = Y[(size_t)k1 * YB + b * QC + c2];
       dft_reg<L2>(v);
 #pragma unroll
-      for (int b = 0; b < L2; ++b) Z[((size_t)k1 + L1 * b) * QC + c2] = v[b];
+      for (int b = 0; b < L2; ++b) Z[((size_t)k1 + L1 * b) * ZS + c2] = v[b];
     }
     __syncthreads();
     // accumulate every output frequency (unpacking the two real spectra of a sequence)
@@ -300,8 +305,8 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
       for (int c = 0; c < QC; ++c) {
         const i64 q = q0 + 2 * c;
         if (q >= nq) break;
-        const cplx zc = Z[(size_t)fcol[u] * QC + c];
-        const cplx zn = Z[(size_t)fconj[u] * QC + c];
+        const cplx zc = Z[(size_t)fcol[u] * ZS + c];
+        const cplx zn = Z[(size_t)fconj[u] * ZS + c];
         const cplx x1 = {0.5 * (zc.re + zn.re), 0.5 * (zc.im - zn.im)};
         const cplx x2 = {0.5 * (zc.im + zn.im), -0.5 * (zc.re - zn.re)};
         cplx t = cmul(wq, x1);
@@ -333,7 +338,8 @@ int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 
                     const double* d, const i64* idx, int ell, double* B, i64 ldb) {
   constexpr int L = L1 * L2;
   constexpr int QC = 256 / (L1 > L2 ? L1 : L2);
-  const size_t bytes = (size_t)2 * L * QC * 16 + (size_t)L * 16;
+  // Y (L1 blocks of L2 QC + 1), Z (L rows of QC + 1) and the L twiddles, 16 bytes each
+  const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + (size_t)L * (QC + 1) + L) * 16;
   if (bytes > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute((const void*)srft_fft_kernel<L1, L2>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
