@@ -198,6 +198,12 @@ __device__ __forceinline__ void dft_reg(cplx* x) {
 // HBM, transforms them in registers and applies w_l^(k1 p2); pass 2 (a thread per (sequence,
 // k1)) transforms the L2 values from LDS; every output slot then accumulates its frequency as
 // in srft_kernel.  The next slab's inputs are loaded while the current slab is transformed.
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations, not for its
+// outstanding global loads (a __syncthreads() fence waits for both), so a slab's prefetch
+// stays in flight across the barriers of the slab before it.  The kernel below exchanges
+// data between threads through LDS only.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 template <int L1, int L2>
 __global__ __launch_bounds__(256) void srft_fft_kernel(
     i64 m, const double* __restrict__ A, i64 lda, i64 mm, const double* __restrict__ d,
@@ -249,30 +255,35 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   const bool p2act = tid < QC * L1;
   const int c2 = tid % QC, k1 = tid / QC;
   const i64 nq = mm;
-  cplx in[L1];
-  // (every load is unconditional, from a clamped address, with the mask applied afterwards:
-  // loads under branches would each wait for the previous one)
+  // raw x and d of the next slab, multiplied only in its pass 1, so the wait for the loads
+  // falls there and not where they are issued.  (Every load is unconditional, from a clamped
+  // address, with the mask applied in pass 1: loads under branches would each wait for the
+  // previous one.)
+  double rx[L1][2], rd[L1][2];
   auto load = [&](i64 q0) {
     const i64 q = q0 + 2 * c1;
     const i64 qa = min(q, nq - 1), qb = min(q + 1, nq - 1);
-    const bool ma = p1act && q < nq, mb = p1act && q + 1 < nq;
     const int pr = p1act ? p2 : 0;
 #pragma unroll
     for (int a = 0; a < L1; ++a) {
       const i64 base = (i64)(a * L2 + pr) * mm;
-      const double x0 = d[base + qa] * x[base + qa];
-      const double x1 = d[base + qb] * x[base + qb];
-      in[a] = {ma ? x0 : 0.0, mb ? x1 : 0.0};
+      rx[a][0] = x[base + qa];
+      rx[a][1] = x[base + qb];
+      rd[a][0] = d[base + qa];
+      rd[a][1] = d[base + qb];
     }
   };
   load(0);
-  __syncthreads();  // the twiddles are written by other waves than the ones reading them
+  lds_barrier();  // the twiddles are written by other waves than the ones reading them
   for (i64 q0 = 0; q0 < nq; q0 += 2 * QC) {
     // pass 1
     if (p1act) {
+      const i64 q = q0 + 2 * c1;
+      const bool ma = q < nq, mb = q + 1 < nq;
       cplx v[L1];
 #pragma unroll
-      for (int a = 0; a < L1; ++a) v[a] = in[a];
+      for (int a = 0; a < L1; ++a)
+        v[a] = {ma ? rd[a][0] * rx[a][0] : 0.0, mb ? rd[a][1] * rx[a][1] : 0.0};
       dft_reg<L1>(v);
 #pragma unroll
       for (int a = 0; a < L1; ++a) {
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
       }
     }
     if (q0 + 2 * QC < nq) load(q0 + 2 * QC);  // next slab, in flight during this one
-    __syncthreads();
+    lds_barrier();
     // pass 2
     if (p2act) {
       cplx v[L2];
@@ -291,7 +302,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
 #pragma unroll
       for (int b = 0; b < L2; ++b) Z[((size_t)k1 + L1 * b) * ZS + c2] = v[b];
     }
-    __syncthreads();
+    lds_barrier();
     // accumulate every output frequency (unpacking the two real spectra of a sequence)
 #pragma unroll
     for (int u = 0; u < MAXS; ++u) {
