@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   }
   const int nslots = (ell + 1) / 2;
   constexpr int MAXS = 4;
-  cplx acc[MAXS], wstep[MAXS];
+  cplx acc[MAXS], wstep[MAXS], wstep2[MAXS];
   int fcol[MAXS], fconj[MAXS];
   i64 ff[MAXS];
 #pragma unroll
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     acc[u] = {0.0, 0.0};
     fcol[u] = fconj[u] = 0;
     ff[u] = 0;
-    wstep[u] = {1.0, 0.0};
+    wstep[u] = wstep2[u] = {1.0, 0.0};
     if (s < nslots) {
       const i64 f = idx[2 * s] - 1;
       ff[u] = f;
@@ -246,6 +246,8 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
       double sn, cs;
       sincospi(-2.0 * (double)f / (double)m, &sn, &cs);
       wstep[u] = {cs, sn};
+      sincospi(-4.0 * (double)f / (double)m, &sn, &cs);
+      wstep2[u] = {cs, sn};
     }
   }
   // pass-1 role: sequence c1, column p2 of the reshape
@@ -318,16 +320,18 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
         if (q >= nq) break;
         const cplx zc = Z[(size_t)fcol[u] * ZS + c];
         const cplx zn = Z[(size_t)fconj[u] * ZS + c];
-        const cplx x1 = {0.5 * (zc.re + zn.re), 0.5 * (zc.im - zn.im)};
-        const cplx x2 = {0.5 * (zc.im + zn.im), -0.5 * (zc.re - zn.re)};
-        cplx t = cmul(wq, x1);
-        a.re += t.re; a.im += t.im;
-        wq = cmul(wq, wstep[u]);
+        // twice the two real spectra: 2 X1 = zc + conj(zn), 2 X2 = (zc - conj(zn)) / i; the
+        // factor 1/2 is applied once at the end.  Columns q and q + 1 enter as
+        // w^q (X1 + w X2), and w^q advances by w^2.
+        cplx x = {zc.re + zn.re, zc.im - zn.im};
         if (q + 1 < nq) {
-          t = cmul(wq, x2);
-          a.re += t.re; a.im += t.im;
+          const cplx x2 = {zc.im + zn.im, zn.re - zc.re};
+          const cplx t = cmul(wstep[u], x2);
+          x.re += t.re; x.im += t.im;
         }
-        wq = cmul(wq, wstep[u]);
+        const cplx t = cmul(wq, x);
+        a.re += t.re; a.im += t.im;
+        wq = cmul(wq, wstep2[u]);
       }
       acc[u] = a;
     }
@@ -339,8 +343,8 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     const int s = tid + u * 256;
     if (s >= nslots) continue;
     double* y = B + col * ldb;
-    y[2 * s] = acc[u].re;
-    if (2 * s + 1 < ell) y[2 * s + 1] = acc[u].im;
+    y[2 * s] = 0.5 * acc[u].re;
+    if (2 * s + 1 < ell) y[2 * s + 1] = 0.5 * acc[u].im;
   }
 }
 
