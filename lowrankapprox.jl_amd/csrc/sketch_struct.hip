@@ -230,17 +230,17 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   constexpr int MAXS = 4;
   cplx acc[MAXS], wstep[MAXS], wstep2[MAXS];
   int fcol[MAXS], fconj[MAXS];
-  i64 ff[MAXS];
+  i64 e0[MAXS], estep[MAXS];  // e0 = f q0 mod m, advanced by f 2 QC mod m per slab
 #pragma unroll
   for (int u = 0; u < MAXS; ++u) {
     const int s = tid + u * 256;
     acc[u] = {0.0, 0.0};
     fcol[u] = fconj[u] = 0;
-    ff[u] = 0;
+    e0[u] = estep[u] = 0;
     wstep[u] = wstep2[u] = {1.0, 0.0};
     if (s < nslots) {
       const i64 f = idx[2 * s] - 1;
-      ff[u] = f;
+      estep[u] = (f * (2 * QC)) % m;
       fcol[u] = (int)(f % L);
       fconj[u] = (int)((L - f % L) % L);
       double sn, cs;
@@ -310,9 +310,10 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     for (int u = 0; u < MAXS; ++u) {
       const int s = tid + u * 256;
       if (s >= nslots) continue;
-      const i64 e0 = (ff[u] * (q0 % m)) % m;
       double sn, cs;
-      sincospi(-2.0 * (double)e0 / (double)m, &sn, &cs);
+      sincospi(-2.0 * (double)e0[u] / (double)m, &sn, &cs);
+      e0[u] += estep[u];
+      if (e0[u] >= m) e0[u] -= m;
       cplx wq = {cs, sn};
       cplx a = acc[u];
       for (int c = 0; c < QC; ++c) {
