@@ -357,13 +357,8 @@ int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 
   // Y (L1 blocks of L2 QC + 1), Z (L rows of QC + 1) and the L twiddles, 16 bytes each
   const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + (size_t)L * (QC + 1) + L) * 16;
   if (bytes > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)srft_fft_kernel<L1, L2>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)bytes);
-    if (e != hipSuccess) {
-      lra_set_error("srft: LDS attribute: %s", hipGetErrorString(e));
-      return LRA_E_HIP;
-    }
+    static std::atomic<uint64_t> lds_set{0};  // per device, set once
+    lra_lds_limit_once(lds_set, (const void*)srft_fft_kernel<L1, L2>, (int)bytes);
   }
   hipLaunchKernelGGL((srft_fft_kernel<L1, L2>), dim3((unsigned)Q), dim3(256), bytes, st, P, X,
                      ldx, mm, d, idx, ell, B, ldb);
@@ -475,9 +470,10 @@ int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
   };
   while (qc > 1 && need(qc) > lds_cap) qc >>= 1;
   if (need(qc) > lds_cap) { lra_set_error("srft: l=%lld too large", (long long)l); return LRA_E_UNSUPPORTED; }
-  if (need(qc) > 64 * 1024)
-    LRA_HIP(hipFuncSetAttribute((const void*)srft_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)need(qc)));
+  if (need(qc) > 64 * 1024) {
+    static std::atomic<uint64_t> lds_set{0};  // per device, set once to the cap need(qc) obeys
+    lra_lds_limit_once(lds_set, (const void*)srft_kernel, (int)lds_cap);
+  }
   hipLaunchKernelGGL(srft_kernel, dim3((unsigned)Q), dim3(SRFT_THREADS), need(qc), c->stream, P,
                      X, ldx, (int)l, mm, qc, pow2 ? 1 : 0, logl, d, idx, (int)ell, B, ldb);
   LRA_LAUNCH_CHECK("srft");
