@@ -421,6 +421,13 @@ def _scale_rows(d, X):
     return X
 
 
+def _as_matrix(X):
+    """X on the device as a matrix: a vector becomes one column (the reference's methods on
+    AbstractVector, e.g. src/psvd.jl:61-66, are those on a one-column matrix)."""
+    X = to_device(X)
+    return X.reshape(-1, 1) if X.dim() == 1 else X
+
+
 def _right_apply(F, X, trans: str):
     """X op(F) through F's left product: X op(F) = (op(F)' X')', returned column-major."""
     X = to_device(X)
@@ -868,6 +875,7 @@ class PartialSVD:
 
     def apply(self, X, trans: str = "n"):
         """A X = U (S (Vt X)) or A' X = V (S (U' X)) (mul!, src/psvd.jl:61-124)."""
+        X = _as_matrix(X)
         if trans == "n":
             return dgemm(self.U, _scale_rows(self.S, dgemm(self.Vt, X)))
         return dgemm(self.Vt, _scale_rows(self.S, dgemm(self.U, X, transa="c")), transa="c")
@@ -884,6 +892,7 @@ class PartialSVD:
     def ldiv(self, B):
         """A \\ B = V (diag(S)^-1 (U' B)), the pseudo-inverse product (ldiv!,
         src/psvd.jl:119-124)."""
+        B = _as_matrix(B)
         return dgemm(self.Vt, _scale_rows(1.0 / self.S, dgemm(self.U, B, transa="c")),
                      transa="c")
 
@@ -934,6 +943,7 @@ class PartialHermEigen:
     def apply(self, X, trans: str = "n"):
         """A X = V (w (V' X)) (mul!, src/pheig.jl:54-100); A' = A for the real Hermitian
         factorization, so trans is accepted for symmetry with the other factors."""
+        X = _as_matrix(X)
         return dgemm(self.vectors, _scale_rows(self.values, dgemm(self.vectors, X,
                                                                    transa="c")))
 
@@ -949,6 +959,7 @@ class PartialHermEigen:
     def ldiv(self, B):
         """A \\ B = V (diag(w)^-1 (V' B)), the pseudo-inverse product (ldiv!,
         src/pheig.jl:147-152)."""
+        B = _as_matrix(B)
         return dgemm(self.vectors, _scale_rows(1.0 / self.values,
                                                dgemm(self.vectors, B, transa="c")))
 

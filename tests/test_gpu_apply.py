@@ -112,6 +112,10 @@ def test_psvd_pheig_cur_arithmetic(gpu, orc):
     x = host(Fs.ldiv(B))
     assert np.allclose(x, Vt.T @ ((U.T @ B) / S[:, None]), rtol=1e-10,
                        atol=1e-12 * np.abs(x).max())
+    bv = rng.standard_normal(m)  # vectors act as one-column matrices
+    assert host(Fs.apply(rng.standard_normal(n))).shape == (m, 1)
+    assert np.allclose(host(Fs.ldiv(bv))[:, 0], Vt.T @ ((U.T @ bv) / S), rtol=1e-10,
+                       atol=1e-12 * np.abs(x).max())
     # the minimum-norm least-squares solution of the factored matrix
     assert np.allclose(x, np.linalg.lstsq(Md, B, rcond=S[-1] / S[0] / 2)[0],
                        atol=1e-6 * np.abs(x).max())
@@ -126,6 +130,8 @@ def test_psvd_pheig_cur_arithmetic(gpu, orc):
     Ze = rng.standard_normal((4, 150))
     assert np.allclose(host(Fe.rapply(Ze)), Ze @ Me, atol=1e-11 * sc)
     Be = rng.standard_normal((150, 2))
+    ve = rng.standard_normal(150)
+    assert np.allclose(host(Fe.apply(ve))[:, 0], Me @ ve, atol=1e-11 * sc)
     xe = host(Fe.ldiv(Be))
     assert np.allclose(xe, Ve @ ((Ve.T @ Be) / w[:, None]), rtol=1e-10,
                        atol=1e-12 * np.abs(xe).max())
