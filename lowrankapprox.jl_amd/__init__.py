@@ -451,6 +451,7 @@ class ID:
 
     def apply(self, X, trans: str = "n"):
         """A X = C (V X) (trans "n") or A' X = V' (C' X) (trans "c") (src/id.jl:236-331)."""
+        X = _as_matrix(X)
         if trans == "n":
             return dgemm(self.C, self.V.apply(X, "n"))
         return self.V.apply(dgemm(self.C, X, transa="c"), "c")

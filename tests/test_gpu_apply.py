@@ -29,6 +29,9 @@ def test_idpackedv_and_id_apply(gpu, orc):
     assert np.allclose(host(F.apply(X)), Cd @ (Vd @ X), atol=1e-11)
     Z = rng.standard_normal((180, 4))
     assert np.allclose(host(F.apply(Z, "c")), Vd.T @ (Cd.T @ Z), atol=1e-11)
+    z = Z[:, 0].copy()  # a vector acts as a one-column matrix
+    assert np.allclose(host(F.apply(z, "c"))[:, 0], Vd.T @ (Cd.T @ z), atol=1e-11)
+    assert np.allclose(host(F.apply(X[:, 0].copy()))[:, 0], Cd @ (Vd @ X[:, 0]), atol=1e-11)
 
 
 def test_partial_factor_apply(gpu, orc):
