@@ -46,6 +46,25 @@ def test_python_options_mirror_reference_defaults(lra):
     assert c.pqrfact_retval == 3 and c.precision == 0
 
 
+def test_options_follow_the_element_type(lra):
+    """LRAOptions(T) (src/LowRankApprox.jl:96-119): rtol = 5 eps(T), pheig_orthtol =
+    sqrt(eps(T)); float32 inputs take Float32's, complex inputs are refused on the host
+    before any device work."""
+    import pytest
+    import torch
+    e32 = float(np.finfo(np.float32).eps)
+    o = lra.LRAOptions.for_type("f32")
+    assert o.rtol == 5 * e32 and abs(o.pheig_orthtol - np.sqrt(e32)) < 1e-20
+    assert lra.LRAOptions.for_type("f64") == lra.LRAOptions()
+    assert lra.LRAOptions.for_type("f32", rank=7).rank == 7
+    assert lra._eltype(np.zeros((2, 2), np.float32)) == "f32"
+    assert lra._eltype(torch.zeros(2, 2, dtype=torch.float32)) == "f32"
+    assert lra._eltype(np.zeros((2, 2))) == "f64"
+    for z in (np.zeros((2, 2), np.complex128), torch.zeros(2, 2, dtype=torch.complex64)):
+        with pytest.raises(TypeError):
+            lra.to_device(z)
+
+
 def test_options_validation(lra):
     import pytest
     for bad in (dict(atol=-1), dict(nb=0), dict(rtol=-1), dict(sketch="foo"),
