@@ -150,6 +150,17 @@ int lra_device_info(lra_ctx* ctx, int64_t* num_cu, int64_t* lds_bytes);
 int lra_gaussian_fill(lra_ctx* ctx, uint64_t seed, uint64_t stream_id, int64_t rows,
                       int64_t cols, double* out, int64_t ld);
 
+/* The random objects a left sketch of an m-row operand draws from (seed, stream_id), exactly
+ * those the sketch kernels and the adaptive loop (round r uses stream_base + r) generate
+ * internally, so that a caller can inject the same bits into another implementation:
+ *   randn: d = Omega (ell x m, column-major), idx unused (crandn, src/util.jl:4-11);
+ *   srft:  d = signs (m), idx = frequency samples (ell, 1-based) (srft_rand, src/sketch.jl:338-361);
+ *   sub:   idx = sampled rows (ell, 1-based), d unused (rand(1:m, k), src/sketch.jl:252);
+ *   sprn:  d = coefficients (m), idx = permutation (m, 1-based) (src/sketch.jl:575-579).
+ * Device buffers. */
+int lra_sketch_draw(lra_ctx* ctx, int sketch, uint64_t seed, uint64_t stream_id, int64_t m,
+                    int64_t ell, double* d, int64_t* idx);
+
 /* Gaussian sketch B = Omega*op(A) (side left) or op(A)*Omega (side right) with op = A or
  * A' (trans), Omega either injected (omega != NULL: ell x K left, K x ell right,
  * column-major, ld = rows) or generated from (seed, stream_id) as lra_gaussian_fill does.
@@ -265,10 +276,47 @@ int lra_scale_rows(lra_ctx* ctx, int64_t m, int64_t n, const double* d, double* 
 int lra_idfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
                const lra_opts* opts, lra_id_out* out);
 
-/* idfact on a matrix-free operator (idfact(L::LinearOperator), src/id.jl:434-447 with
- * src/linop.jl:105-113): the sketch is one application of L' to ell vectors. */
+/* ---- matrix-free entry points (AbstractLinearOperator arguments) ---------------------
+ * The operator L (m x n) is reached only through `apply` (see lra_apply_fn).  isherm != 0
+ * declares a Hermitian operator (HermitianLinearOperator, src/linop.jl:29-40; m == n): the
+ * reference's ishermitian(L) is true for it and false for a plain LinearOperator.  As
+ * chkopts!(opts, L) does (src/LowRankApprox.jl:142-148), a sketch other than randn is
+ * replaced by randn (with a warning on stderr when opts->verb != 0); the operator is applied
+ * in FP64, so opts->precision is ignored. */
+
+/* idfact(trans, L, opts) (src/id.jl:434-447 through src/linop.jl:105-113): each adaptive
+ * round applies op(L)' to ell Gaussian vectors; sketch_randn_niter power iterations apply
+ * op(L) and op(L)' once more each (src/sketch.jl:129-174). */
 int lra_idfact_op(lra_ctx* ctx, int trans, int64_t m, int64_t n, lra_apply_fn apply,
-                  void* user, const lra_opts* opts, lra_id_out* out);
+                  void* user, int isherm, const lra_opts* opts, lra_id_out* out);
+
+/* pqrfact(trans, L, opts) (src/pqr.jl:285-320): the ID of op(L), then the skeleton columns
+ * op(L)[:, sk] from one application to k one-hot vectors (getcols, src/util.jl:30-34 and
+ * src/linop.jl:54-67) and their thin QR. */
+int lra_pqrfact_op(lra_ctx* ctx, int trans, int64_t m, int64_t n, lra_apply_fn apply,
+                   void* user, int isherm, const lra_opts* opts, lra_pqr_out* out);
+
+/* psvdfact(L, opts) / psvdvals (src/psvd.jl:238-290) on an operator. */
+int lra_psvdfact_op(lra_ctx* ctx, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                    int isherm, const lra_opts* opts, lra_psvd_out* out);
+int lra_psvdvals_op(lra_ctx* ctx, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                    int isherm, const lra_opts* opts, double* S, int64_t* k);
+
+/* pheigfact(L, opts) / pheigvals (src/pheig.jl:276-314) on a Hermitian operator (n x n; the
+ * caller guarantees L' = L).  L[sk, sk] comes from one application to k one-hot vectors
+ * (src/linop.jl:70-74). */
+int lra_pheigfact_op(lra_ctx* ctx, int64_t n, lra_apply_fn apply, void* user,
+                     const lra_opts* opts, lra_pheig_out* out);
+int lra_pheigvals_op(lra_ctx* ctx, int64_t n, lra_apply_fn apply, void* user,
+                     const lra_opts* opts, double* values, int64_t* k);
+
+/* curfact(L, opts) (src/cur.jl:527-564): isherm != 0 takes one column sketchfact and
+ * rows = cols; otherwise the first sketchfact applies the operator, the selected rows
+ * (columns) of L come from one application to one-hot vectors, and the second sketchfact runs
+ * on that dense block.  rows, cols as in lra_curfact. */
+int lra_curfact_op(lra_ctx* ctx, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                   int isherm, const lra_opts* opts, int64_t* rows, int64_t* cols,
+                   int64_t* k);
 
 /* sketchfact(:left, trans, A, opts) with pqrfact_retval = "" (src/sketch.jl:52-66 and the
  * adaptive drivers :223-240, :313-330, :545-562, :674-690; sketch = none is
@@ -281,7 +329,8 @@ int lra_sketchfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* 
 
 /* curfact(A, opts) (src/cur.jl:532-564; sketch = none: curfact_none, :574-606): row and
  * column skeletons.  rows and cols (device, min(m,n) entries each) receive k 1-based
- * indices, *k (host).  An exactly symmetric A (ishermitian for real A) takes one column
+ * indices, *k (host); k <= min(m,n) always (the first stage's longer skeleton stays in the
+ * library's workspace and is cut to min(kr, kc) as the reference does).  An exactly symmetric A (ishermitian for real A) takes one column
  * sketchfact and rows = cols; otherwise two chained sketchfacts, the second drawing from
  * Philox streams 2^32 + round. */
 int lra_curfact(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,

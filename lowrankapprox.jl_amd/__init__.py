@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import math
+import warnings
 from typing import Optional
 
 import numpy as np
@@ -27,7 +28,7 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
            "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
-           "BatchedID", "idfact_batched", "Server", "prange", "CUR"]
+           "BatchedID", "idfact_batched", "Server", "prange", "CUR", "sketch_draw"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -196,6 +197,38 @@ def gaussian(rows: int, cols: int, seed: int = 0, stream: int = 0):
     return out
 
 
+def sketch_draw(sketch: str, m: int, order: int, seed: int = 0, stream: int = 0):
+    """The random objects of a left sketch of order `order` on an m-row operand, as the
+    sketch kernels draw them from (seed, stream): randn -> Omega (order x m); srft -> (signs
+    (m), samples (order, 1-based)); sub -> rows (order, 1-based); sprn -> (permutation (m,
+    1-based), coefficients (m)) (src/sketch.jl:91-110, 244-330, 338-361, 566-590)."""
+    torch = _torch()
+    ctx = context()
+    L = _lib.load()
+    f64 = dict(dtype=torch.float64, device="cuda")
+    i64t = dict(dtype=torch.int64, device="cuda")
+    if sketch == "randn":
+        O = empty_colmajor(order, m)
+        check(L.lra_sketch_draw(ctx.h, 1, seed, stream, m, order, _ptr(O), None), "lra_sketch_draw")
+        return O
+    if sketch == "srft":
+        d, idx = torch.empty(max(m, 1), **f64), torch.empty(max(order, 1), **i64t)
+        check(L.lra_sketch_draw(ctx.h, 2, seed, stream, m, order, _ptr(d), _ptr(idx)),
+              "lra_sketch_draw")
+        return d[:m], idx[:order]
+    if sketch == "sub":
+        r = torch.empty(max(order, 1), **i64t)
+        check(L.lra_sketch_draw(ctx.h, 3, seed, stream, m, order, None, _ptr(r)),
+              "lra_sketch_draw")
+        return r[:order]
+    if sketch == "sprn":
+        s, r = torch.empty(max(m, 1), **f64), torch.empty(max(m, 1), **i64t)
+        check(L.lra_sketch_draw(ctx.h, 4, seed, stream, m, order, _ptr(s), _ptr(r)),
+              "lra_sketch_draw")
+        return r[:m], s[:m]
+    raise ValueError(sketch)
+
+
 def sketch(side: str, trans: str, A, order: int, opts: LRAOptions = None, omega=None,
            stream: int = 0, **kw):
     """sketch (src/sketch.jl:35-50), Gaussian case: B = Omega*op(A) or op(A)*Omega."""
@@ -361,11 +394,9 @@ class IDPackedV:
 
     def apply(self, X, trans: str = "n"):
         """V X (trans "n", X n x nrhs) or V' X (trans "c", X k x nrhs) for V = [I T] P'
-        (mul!/mulc! of IDPackedV, src/id.jl:94-200)."""
+        (mul!/mulc! of IDPackedV, src/id.jl:94-152); a vector gives a vector."""
         ctx = context()
-        X = to_device(X)
-        if X.dim() == 1:
-            X = X.reshape(-1, 1)
+        X, vec = _in_mat(X)
         nrhs = X.shape[1]
         rows = self.k if trans == "n" else self.n
         Y = empty_colmajor(rows, nrhs)
@@ -373,12 +404,24 @@ class IDPackedV:
         check(_lib.load().lra_id_apply(ctx.h, 0 if trans == "n" else 1, self.k, self.n,
                                        _ptr(self.sk), _ptr(self.rd), _ptr(self.T), ldt, nrhs,
                                        _ptr(X), _ld(X), _ptr(Y), max(1, rows)), "lra_id_apply")
-        return Y
+        return _out_mat(Y, vec)
+
+    def rapply(self, X, trans: str = "n"):
+        """X V (trans "n", X m x k) or X V' (trans "c", X m x n): the right products of
+        src/id.jl:154-200, as (V' X')' and (V X')'."""
+        return _right_apply(self, X, trans)
 
     def matrix(self):
-        """The k x n interpolation matrix [I T] P' (Matrix(V), src/id.jl:66)."""
+        """The k x n interpolation matrix [I T] P' (Matrix(V), src/id.jl:59-66): column
+        sk[i] is e_i and column rd[j] is T[:, j], scattered on the device."""
         torch = _torch()
-        return self.apply(torch.eye(self.n, dtype=torch.float64, device="cuda"))
+        M = empty_colmajor(self.k, self.n)
+        M.zero_()
+        if self.k:
+            M[torch.arange(self.k, device="cuda"), self.sk - 1] = 1.0
+            if self.n > self.k:
+                M[:, self.rd - 1] = to_device(self.T)
+        return M
 
 
 def dgemm(A, B, transa: str = "n", transb: str = "n", alpha: float = 1.0, C=None,
@@ -421,19 +464,31 @@ def _scale_rows(d, X):
     return X
 
 
-def _as_matrix(X):
-    """X on the device as a matrix: a vector becomes one column (the reference's methods on
-    AbstractVector, e.g. src/psvd.jl:61-66, are those on a one-column matrix)."""
+def _in_mat(X):
+    """(X on the device as a matrix, whether X was a vector): a vector is one column (the
+    reference's methods on AbstractVector, e.g. src/psvd.jl:61-66, are those on a one-column
+    matrix, and return a vector)."""
     X = to_device(X)
-    return X.reshape(-1, 1) if X.dim() == 1 else X
+    return (X.reshape(-1, 1), True) if X.dim() == 1 else (X, False)
+
+
+def _out_mat(Y, vec: bool):
+    return Y[:, 0] if vec else Y
+
+
+def _as_matrix(X):
+    return _in_mat(X)[0]
 
 
 def _right_apply(F, X, trans: str):
-    """X op(F) through F's left product: X op(F) = (op(F)' X')', returned column-major."""
+    """X op(F) through F's left product: X op(F) = (op(F)' X')', returned column-major; a
+    vector x is the row x' and gives a vector."""
     X = to_device(X)
-    if X.dim() == 1:
+    vec = X.dim() == 1
+    if vec:
         X = X.reshape(1, -1)
-    return F.apply(X.t(), "c" if trans == "n" else "n").contiguous().t()
+    Y = F.apply(X.t(), "c" if trans == "n" else "n").contiguous().t()
+    return Y[0, :] if vec else Y
 
 
 @dataclasses.dataclass
@@ -444,17 +499,38 @@ class ID:
 
     @classmethod
     def from_matrix(cls, A, V: IDPackedV, trans: str = "n"):
-        torch = _torch()
+        """ID(trans, A, V) (src/id.jl:219-223): C = getcols(trans, A, sk); A may be a
+        LinearOperator (columns from one-hot applications)."""
+        if isinstance(A, LinearOperator):
+            return cls(to_device(A.getcols(V.sk, trans)), V)
         A = to_device(A)
         X = A if trans == "n" else A.t()
         return cls(X[:, V.sk - 1].t().contiguous().t(), V)
 
     def apply(self, X, trans: str = "n"):
-        """A X = C (V X) (trans "n") or A' X = V' (C' X) (trans "c") (src/id.jl:236-331)."""
-        X = _as_matrix(X)
+        """A X = C (V X) (trans "n") or A' X = V' (C' X) (trans "c") (src/id.jl:263-300)."""
+        X, vec = _in_mat(X)
         if trans == "n":
-            return dgemm(self.C, self.V.apply(X, "n"))
-        return self.V.apply(dgemm(self.C, X, transa="c"), "c")
+            Y = dgemm(self.C, self.V.apply(X, "n"))
+        else:
+            Y = self.V.apply(dgemm(self.C, X, transa="c"), "c")
+        return _out_mat(Y, vec)
+
+    def rapply(self, X, trans: str = "n"):
+        """X A = (X C) V (trans "n") or X A' = (X V') C' ("c") (src/id.jl:302-330)."""
+        X = to_device(X)
+        vec = X.dim() == 1
+        if vec:
+            X = X.reshape(1, -1)
+        if trans == "n":
+            Y = self.V.rapply(dgemm(X, self.C), "n")
+        else:
+            Y = dgemm(self.V.rapply(X, "c"), self.C, transb="c")
+        return Y[0, :] if vec else Y
+
+    def matrix(self):
+        """Matrix(A) = C V (src/id.jl:244)."""
+        return dgemm(self.C, self.V.matrix())
 
 
 def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
@@ -678,10 +754,21 @@ class LinearOperator:
             raise LraError(f"{what}: the operator raised {self._error!r}") from self._error
         check(status, what)
 
-    def idfact(self, trans="n", opts: "LRAOptions" = None, **kw) -> "IDPackedV":
-        """idfact on the operator (src/id.jl:434-447 through src/linop.jl:105-113)."""
-        torch = _torch()
+    def _opts(self, opts, kw) -> "LRAOptions":
+        """copy(opts; kw...) then chkopts!(opts, L) (src/LowRankApprox.jl:142-148): a sketch
+        other than randn is replaced by randn with a warning."""
         opts = (opts or LRAOptions()).copy(**kw)
+        if opts.sketch != "randn":
+            warnings.warn('invalid sketch method; using "randn"', stacklevel=3)
+            opts.sketch = "randn"
+        return opts
+
+    def idfact(self, trans="n", opts: "LRAOptions" = None, **kw) -> "IDPackedV":
+        """idfact on the operator (src/id.jl:434-447 through src/linop.jl:105-113), with
+        sketch_randn_niter power iterations through L and L' (src/sketch.jl:129-174)."""
+        torch = _torch()
+        opts = self._opts(opts, kw)
+        opts.pqrfact_retval = "t"
         ctx = context()
         N = self.n if trans == "n" else self.m
         sk = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
@@ -692,10 +779,110 @@ class LinearOperator:
         copts = opts.to_c()
         cb = self._callback()
         self._check(_lib.load().lra_idfact_op(ctx.h, _lib.TRANS[trans], self.m, self.n, cb,
-                                              None, C.byref(copts), C.byref(out)),
-                    "lra_idfact_op")
+                                              None, int(self.hermitian), C.byref(copts),
+                                              C.byref(out)), "lra_idfact_op")
         k = int(out.k)
         return IDPackedV(sk[:k], rd[: N - k], _borrow_T(out, k, N - k), int(out.rounds))
+
+    def pqrfact(self, trans="n", opts: "LRAOptions" = None, **kw):
+        """pqrfact on the operator (src/pqr.jl:285-320): skeleton columns op(L)[:, sk] from
+        one application to one-hot vectors (src/util.jl:30-34, src/linop.jl:54-67)."""
+        opts = self._opts(opts, kw)
+        ctx = context()
+        out = _lib.lra_pqr_out()
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_pqrfact_op(ctx.h, _lib.TRANS[trans], self.m, self.n, cb,
+                                               None, int(self.hermitian), C.byref(copts),
+                                               C.byref(out)), "lra_pqrfact_op")
+        N = self.n if trans == "n" else self.m
+        P = self.m if trans == "n" else self.n
+        return _pqr_result(out, opts.pqrfact_retval, P, N)
+
+    def psvdfact(self, opts: "LRAOptions" = None, **kw) -> "PartialSVD":
+        """psvdfact on the operator (src/psvd.jl:238-272)."""
+        opts = self._opts(opts, kw)
+        ctx = context()
+        out = _lib.lra_psvd_out()
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_psvdfact_op(ctx.h, self.m, self.n, cb, None,
+                                                int(self.hermitian), C.byref(copts),
+                                                C.byref(out)), "lra_psvdfact_op")
+        return _psvd_result(out, self.m, self.n)
+
+    def psvdvals(self, opts: "LRAOptions" = None, **kw):
+        """psvdvals on the operator (src/psvd.jl:274-290)."""
+        torch = _torch()
+        opts = self._opts(opts, kw)
+        ctx = context()
+        S = torch.empty(max(min(self.m, self.n), 1), dtype=torch.float64, device="cuda")
+        k = _lib.i64(0)
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_psvdvals_op(ctx.h, self.m, self.n, cb, None,
+                                                int(self.hermitian), C.byref(copts), _ptr(S),
+                                                C.byref(k)), "lra_psvdvals_op")
+        return S[:int(k.value)]
+
+    def pheigfact(self, opts: "LRAOptions" = None, **kw) -> "PartialHermEigen":
+        """pheigfact on a Hermitian operator (src/pheig.jl:276-297): L[sk, sk] from one
+        application to one-hot vectors (src/linop.jl:70-74)."""
+        if self.m != self.n or not self.hermitian:
+            raise ValueError("matrix must be Hermitian")
+        opts = self._opts(opts, kw)
+        ctx = context()
+        out = _lib.lra_pheig_out()
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_pheigfact_op(ctx.h, self.n, cb, None, C.byref(copts),
+                                                 C.byref(out)), "lra_pheigfact_op")
+        return _pheig_result(out, self.n)
+
+    def pheigvals(self, opts: "LRAOptions" = None, **kw):
+        """pheigvals on a Hermitian operator (src/pheig.jl:298-314)."""
+        torch = _torch()
+        if self.m != self.n or not self.hermitian:
+            raise ValueError("matrix must be Hermitian")
+        opts = self._opts(opts, kw)
+        ctx = context()
+        v = torch.empty(max(self.n, 1), dtype=torch.float64, device="cuda")
+        k = _lib.i64(0)
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_pheigvals_op(ctx.h, self.n, cb, None, C.byref(copts),
+                                                 _ptr(v), C.byref(k)), "lra_pheigvals_op")
+        return v[:int(k.value)]
+
+    def curfact(self, opts: "LRAOptions" = None, **kw) -> "CURPackedU":
+        """curfact on the operator (src/cur.jl:527-564)."""
+        torch = _torch()
+        opts = self._opts(opts, kw)
+        ctx = context()
+        mn = max(min(self.m, self.n), 1)
+        rows = torch.empty(mn, dtype=torch.int64, device="cuda")
+        cols = torch.empty(mn, dtype=torch.int64, device="cuda")
+        k = _lib.i64(0)
+        copts = opts.to_c()
+        cb = self._callback()
+        self._check(_lib.load().lra_curfact_op(ctx.h, self.m, self.n, cb, None,
+                                               int(self.hermitian), C.byref(copts),
+                                               _ptr(rows), _ptr(cols), C.byref(k)),
+                    "lra_curfact_op")
+        k = int(k.value)
+        return CURPackedU(rows[:k], cols[:k])
+
+    def getcols(self, cols, trans: str = "n"):
+        """op(L)[:, cols] (getcols, src/util.jl:30-34) by one application to one-hot
+        vectors (src/linop.jl:54-67)."""
+        torch = _torch()
+        cols = torch.as_tensor(cols, dtype=torch.int64, device="cuda")
+        R = self.n if trans == "n" else self.m
+        S = empty_colmajor(R, int(cols.shape[0]))
+        S.zero_()
+        if cols.numel():
+            S[cols - 1, torch.arange(int(cols.shape[0]), device="cuda")] = 1.0
+        return self.mul(S) if trans == "n" else self.mulc(S)
 
     def snorm(self, opts: "LRAOptions" = None, x0=None, **kw) -> float:
         """snorm on the operator (src/snorm.jl:14-45)."""
@@ -745,29 +932,32 @@ class PartialQR:
         """op(A) X ~ Q R P' X = Q (R X[p, :]) (trans "n"), or op(A)' X = P R' (Q' X) ("c")
         (mul!, src/pqr.jl:86-131)."""
         torch = _torch()
-        X = to_device(X)
-        if X.dim() == 1:
-            X = X.reshape(-1, 1)
+        X, vec = _in_mat(X)
         if trans == "n":
             Xp = X[self.p - 1, :].t().contiguous().t()
-            return dgemm(self.Q, dgemm(self.R, Xp))
+            return _out_mat(dgemm(self.Q, dgemm(self.R, Xp)), vec)
         Y = dgemm(self.R, dgemm(self.Q, X, transa="c"), transa="c")  # R' (Q' X), rows in p order
         out = torch.empty_like(Y)
         out[self.p - 1, :] = Y
-        return out.t().contiguous().t()
+        return _out_mat(out.t().contiguous().t(), vec)
 
     def rapply(self, X, trans: str = "n"):
         """X op(A) = ((X Q) R) P' (trans "n") or X op(A)' = ((X P) R') Q' ("c")
         (right multiplication, src/pqr.jl:133-158)."""
         torch = _torch()
         X = to_device(X)
+        vec = X.dim() == 1
+        if vec:
+            X = X.reshape(1, -1)
         if trans == "n":
             Y = dgemm(dgemm(X, self.Q), self.R)  # columns in p order
             out = torch.empty_like(Y)
             out[:, self.p - 1] = Y
-            return out.t().contiguous().t()
-        Xp = X[:, self.p - 1].t().contiguous().t()
-        return dgemm(dgemm(Xp, self.R, transb="c"), self.Q, transb="c")
+            Y = out.t().contiguous().t()
+        else:
+            Xp = X[:, self.p - 1].t().contiguous().t()
+            Y = dgemm(dgemm(Xp, self.R, transb="c"), self.Q, transb="c")
+        return Y[0, :] if vec else Y
 
     def matrix(self):
         """Matrix(A) = Q R P' (src/pqr.jl:56)."""
@@ -781,14 +971,12 @@ class PartialQR:
         R^+ = U diag(1/s) V', which equals R' (R R')^-1 for the full-row-rank R that pqrfact
         returns without squaring its condition number."""
         torch = _torch()
-        B = to_device(B)
-        if B.dim() == 1:
-            B = B.reshape(-1, 1)
+        B, vec = _in_mat(B)
         U, s, V = _gesvj(self.R.t())
         Y = dgemm(U, _scale_rows(1.0 / s, dgemm(V, dgemm(self.Q, B, transa="c"), transa="c")))
         out = torch.empty_like(Y)
         out[self.p - 1, :] = Y
-        return out.t().contiguous().t()
+        return _out_mat(out.t().contiguous().t(), vec)
 
 
 @dataclasses.dataclass
@@ -801,23 +989,10 @@ class PQRFactors:
     T: object
 
 
-def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
-    """pqrfact(trans, A, opts) (src/pqr.jl:285-320); PartialQR when pqrfact_retval is "qr"."""
+def _pqr_result(out, rv: str, P: int, N: int):
+    """PartialQR (retval "qr") or PQRFactors from a filled lra_pqr_out."""
     torch = _torch()
-    if A is None or isinstance(A, LRAOptions):
-        trans, A, opts = "n", trans, A
-    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
-    ctx = context()
-    A = to_device(A)
-    m, n = A.shape
-    N = n if trans == "n" else m
-    P = m if trans == "n" else n
-    out = _lib.lra_pqr_out()
-    copts = opts.to_c()
-    check(_lib.load().lra_pqrfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
-                                  C.byref(copts), C.byref(out)), "lra_pqrfact")
     k = int(out.k)
-    rv = opts.pqrfact_retval
     Q = _own(out.Q, P, k, int(out.ldq)) if "q" in rv and k > 0 else None
     R = _own(out.R, k, N, int(out.ldr)) if "r" in rv and k > 0 else None
     if "q" in rv and k == 0: Q = empty_colmajor(P, 0)
@@ -828,6 +1003,25 @@ def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
     if "q" in rv and "r" in rv and "t" not in rv:
         return PartialQR(Q, R, p)
     return PQRFactors(Q, R, p, k, T)
+
+
+def pqrfact(trans, A=None, opts: LRAOptions = None, **kw):
+    """pqrfact(trans, A, opts) (src/pqr.jl:285-320); PartialQR when pqrfact_retval is "qr"."""
+    if A is None or isinstance(A, LRAOptions):
+        trans, A, opts = "n", trans, A
+    if isinstance(A, LinearOperator):
+        return A.pqrfact(trans, opts, **kw)
+    opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
+    ctx = context()
+    A = to_device(A)
+    m, n = A.shape
+    N = n if trans == "n" else m
+    P = m if trans == "n" else n
+    out = _lib.lra_pqr_out()
+    copts = opts.to_c()
+    check(_lib.load().lra_pqrfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
+                                  C.byref(copts), C.byref(out)), "lra_pqrfact")
+    return _pqr_result(out, opts.pqrfact_retval, P, N)
 
 
 def prange(trans, A=None, opts: LRAOptions = None, **kw):
@@ -876,10 +1070,12 @@ class PartialSVD:
 
     def apply(self, X, trans: str = "n"):
         """A X = U (S (Vt X)) or A' X = V (S (U' X)) (mul!, src/psvd.jl:61-124)."""
-        X = _as_matrix(X)
+        X, vec = _in_mat(X)
         if trans == "n":
-            return dgemm(self.U, _scale_rows(self.S, dgemm(self.Vt, X)))
-        return dgemm(self.Vt, _scale_rows(self.S, dgemm(self.U, X, transa="c")), transa="c")
+            Y = dgemm(self.U, _scale_rows(self.S, dgemm(self.Vt, X)))
+        else:
+            Y = dgemm(self.Vt, _scale_rows(self.S, dgemm(self.U, X, transa="c")), transa="c")
+        return _out_mat(Y, vec)
 
     def rapply(self, X, trans: str = "n"):
         """X A (trans "n") or X A' ("c") (mul!, src/psvd.jl:99-116)."""
@@ -893,13 +1089,22 @@ class PartialSVD:
     def ldiv(self, B):
         """A \\ B = V (diag(S)^-1 (U' B)), the pseudo-inverse product (ldiv!,
         src/psvd.jl:119-124)."""
-        B = _as_matrix(B)
-        return dgemm(self.Vt, _scale_rows(1.0 / self.S, dgemm(self.U, B, transa="c")),
-                     transa="c")
+        B, vec = _in_mat(B)
+        return _out_mat(dgemm(self.Vt, _scale_rows(1.0 / self.S, dgemm(self.U, B, transa="c")),
+                              transa="c"), vec)
+
+
+def _psvd_result(out, m: int, n: int) -> PartialSVD:
+    k = int(out.k)
+    return PartialSVD(_own(out.U, m, k, int(out.ldu)) if k else empty_colmajor(m, 0),
+                      _own_vec(out.S, k), _own(out.Vt, k, n, int(out.ldvt)) if k else
+                      empty_colmajor(0, n))
 
 
 def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
     """psvdfact(A, opts) (src/psvd.jl:238-272)."""
+    if isinstance(A, LinearOperator):
+        return A.psvdfact(opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
@@ -908,15 +1113,14 @@ def psvdfact(A, opts: LRAOptions = None, **kw) -> PartialSVD:
     copts = opts.to_c()
     check(_lib.load().lra_psvdfact(ctx.h, m, n, _ptr(A), _ld(A), C.byref(copts), C.byref(out)),
           "lra_psvdfact")
-    k = int(out.k)
-    return PartialSVD(_own(out.U, m, k, int(out.ldu)) if k else empty_colmajor(m, 0),
-                      _own_vec(out.S, k), _own(out.Vt, k, n, int(out.ldvt)) if k else
-                      empty_colmajor(0, n))
+    return _psvd_result(out, m, n)
 
 
 def psvdvals(A, opts: LRAOptions = None, **kw):
     """psvdvals(A, opts) (src/psvd.jl:274-290)."""
     torch = _torch()
+    if isinstance(A, LinearOperator):
+        return A.psvdvals(opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
@@ -944,9 +1148,9 @@ class PartialHermEigen:
     def apply(self, X, trans: str = "n"):
         """A X = V (w (V' X)) (mul!, src/pheig.jl:54-100); A' = A for the real Hermitian
         factorization, so trans is accepted for symmetry with the other factors."""
-        X = _as_matrix(X)
-        return dgemm(self.vectors, _scale_rows(self.values, dgemm(self.vectors, X,
-                                                                   transa="c")))
+        X, vec = _in_mat(X)
+        return _out_mat(dgemm(self.vectors, _scale_rows(self.values, dgemm(self.vectors, X,
+                                                                            transa="c"))), vec)
 
     def rapply(self, X, trans: str = "n"):
         """X A = X A' (mul!, src/pheig.jl:114-145)."""
@@ -960,13 +1164,22 @@ class PartialHermEigen:
     def ldiv(self, B):
         """A \\ B = V (diag(w)^-1 (V' B)), the pseudo-inverse product (ldiv!,
         src/pheig.jl:147-152)."""
-        B = _as_matrix(B)
-        return dgemm(self.vectors, _scale_rows(1.0 / self.values,
-                                               dgemm(self.vectors, B, transa="c")))
+        B, vec = _in_mat(B)
+        return _out_mat(dgemm(self.vectors, _scale_rows(1.0 / self.values,
+                                                        dgemm(self.vectors, B, transa="c"))),
+                        vec)
+
+
+def _pheig_result(out, n: int) -> PartialHermEigen:
+    k = int(out.k)
+    return PartialHermEigen(_own_vec(out.values, k),
+                            _own(out.vectors, n, k, int(out.ldv)) if k else empty_colmajor(n, 0))
 
 
 def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
     """pheigfact(A, opts) (src/pheig.jl:276-297) for a symmetric A."""
+    if isinstance(A, LinearOperator):
+        return A.pheigfact(opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
@@ -977,14 +1190,14 @@ def pheigfact(A, opts: LRAOptions = None, **kw) -> PartialHermEigen:
     copts = opts.to_c()
     check(_lib.load().lra_pheigfact(ctx.h, n, _ptr(A), _ld(A), C.byref(copts), C.byref(out)),
           "lra_pheigfact")
-    k = int(out.k)
-    return PartialHermEigen(_own_vec(out.values, k),
-                            _own(out.vectors, n, k, int(out.ldv)) if k else empty_colmajor(n, 0))
+    return _pheig_result(out, n)
 
 
 def pheigvals(A, opts: LRAOptions = None, **kw):
     """pheigvals(A, opts) (src/pheig.jl:298-314)."""
     torch = _torch()
+    if isinstance(A, LinearOperator):
+        return A.pheigvals(opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)
@@ -1081,12 +1294,12 @@ class CUR:
 
     def apply(self, X, trans: str = "n"):
         """C (U (R X)) (trans "n") or R' (U' (C' X)) ("c") (mul!, src/cur.jl:209-523)."""
-        X = to_device(X)
-        if X.dim() == 1:
-            X = X.reshape(-1, 1)
+        X, vec = _in_mat(X)
         if trans == "n":
-            return dgemm(self.C, self.U.apply(dgemm(self.R, X)))
-        return dgemm(self.R, self.U.apply(dgemm(self.C, X, transa="c"), "c"), transa="c")
+            Y = dgemm(self.C, self.U.apply(dgemm(self.R, X)))
+        else:
+            Y = dgemm(self.R, self.U.apply(dgemm(self.C, X, transa="c"), "c"), transa="c")
+        return _out_mat(Y, vec)
 
     def rapply(self, X, trans: str = "n"):
         """X A (trans "n") or X A' ("c") (*, src/cur.jl:462-523)."""
@@ -1101,6 +1314,8 @@ class CUR:
 def curfact(A, opts: "LRAOptions" = None, **kw) -> CURPackedU:
     """curfact(A, opts) (src/cur.jl:532-564)."""
     torch = _torch()
+    if isinstance(A, LinearOperator):
+        return A.curfact(opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     ctx = context()
     A = to_device(A)

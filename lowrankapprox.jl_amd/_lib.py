@@ -88,6 +88,8 @@ SIGNATURES = [
     ("lra_last_error", C.c_char_p, []),
     ("lra_device_info", C.c_int, [C.c_void_p, C.POINTER(i64), C.POINTER(i64)]),
     ("lra_gaussian_fill", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, i64, i64, P_D, i64]),
+    ("lra_sketch_draw", C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, i64, i64, P_D,
+                                  P_I]),
     ("lra_sketch_gaussian", C.c_int, [C.c_void_p, C.c_int, C.c_int, i64, i64, P_D, i64, i64,
                                       P_D, C.c_uint64, C.c_uint64, C.c_int, P_D, i64]),
     ("lra_sketch_srft", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, i64, P_D, P_I,
@@ -115,8 +117,20 @@ SIGNATURES = [
     ("lra_orgqr_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, P_D, i64]),
     ("lra_idfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                              C.POINTER(lra_id_out)]),
-    ("lra_idfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p,
+    ("lra_idfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
                                 C.POINTER(lra_opts), C.POINTER(lra_id_out)]),
+    ("lra_pqrfact_op", C.c_int, [C.c_void_p, C.c_int, i64, i64, APPLY_FN, C.c_void_p,
+                                 C.c_int, C.POINTER(lra_opts), C.POINTER(lra_pqr_out)]),
+    ("lra_psvdfact_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
+                                  C.POINTER(lra_opts), C.POINTER(lra_psvd_out)]),
+    ("lra_psvdvals_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
+                                  C.POINTER(lra_opts), P_D, C.POINTER(i64)]),
+    ("lra_pheigfact_op", C.c_int, [C.c_void_p, i64, APPLY_FN, C.c_void_p,
+                                   C.POINTER(lra_opts), C.POINTER(lra_pheig_out)]),
+    ("lra_pheigvals_op", C.c_int, [C.c_void_p, i64, APPLY_FN, C.c_void_p,
+                                   C.POINTER(lra_opts), P_D, C.POINTER(i64)]),
+    ("lra_curfact_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
+                                 C.POINTER(lra_opts), P_I, P_I, C.POINTER(i64)]),
     ("lra_sketchfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
                                  C.c_uint64, P_I, C.POINTER(i64)]),
     ("lra_curfact", C.c_int, [C.c_void_p, i64, i64, P_D, i64, C.POINTER(lra_opts), P_I, P_I,

@@ -327,6 +327,31 @@ static int sketch_gaussian_impl(lra_ctx* c, int side, int trans, i64 m, i64 n, c
   return lra_gemm(c, g);
 }
 
+extern "C" int lra_sketch_draw(lra_ctx* c, int sketch, uint64_t seed, uint64_t stream_id,
+                               int64_t m, int64_t ell, double* d, int64_t* idx) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(sketch >= LRA_SKETCH_RANDN && sketch <= LRA_SKETCH_SPRN, 2);
+  LRA_ARG(m >= 0, 5);
+  LRA_ARG(ell >= 0, 6);
+  LRA_HIP(hipSetDevice(c->device));
+  switch (sketch) {
+    case LRA_SKETCH_RANDN:
+      LRA_ARG(d != nullptr || m * ell == 0, 7);
+      return lra_launch_gaussian_fill(c->stream, seed, stream_id, ell, m, d, std::max<i64>(ell, 1));
+    case LRA_SKETCH_SRFT:
+      LRA_ARG(d != nullptr || m == 0, 7);
+      LRA_ARG(idx != nullptr || ell == 0, 8);
+      return lra_launch_srft_rand(c->stream, seed, stream_id, m, ell, d, idx);
+    case LRA_SKETCH_SUB:
+      LRA_ARG(idx != nullptr || ell == 0, 8);
+      return lra_launch_subset_rand(c->stream, seed, stream_id, m, ell, idx);
+    default:
+      LRA_ARG(d != nullptr || m == 0, 7);
+      LRA_ARG(idx != nullptr || m == 0, 8);
+      return lra_launch_sprn_rand(c->stream, seed, stream_id, m, d, idx);
+  }
+}
+
 extern "C" int lra_sketch_gaussian(lra_ctx* c, int side, int trans, int64_t m, int64_t n,
                                    const double* A, int64_t lda, int64_t ell,
                                    const double* omega, uint64_t seed, uint64_t stream_id,
@@ -557,55 +582,130 @@ static int orthrows(lra_ctx* c, i64 ell, i64 N, double* B) {
   return lra_launch_copy2d(c->stream, k, N, Qt, k, B, ell);
 }
 
+// Y (ell x Q) = X (ell x R) * M with M = A (which 0: R = m, Q = n) or A' (which 1: R = n,
+// Q = m).  A dense source is one GEMM; an operator is applied to X' and the result transposed
+// back (mul!(C, X, L), src/linop.jl:105-113, as (L' X')').
+static int src_rmul(lra_ctx* c, const LraSrc& s, int which, i64 ell, const double* X, i64 ldx,
+                    double* Y, i64 ldy) {
+  const i64 R = which == 0 ? s.m : s.n, Q = which == 0 ? s.n : s.m;
+  if (!s.op()) {
+    GemmArgs g;
+    g.bgather = nullptr; g.prec = LRA_PREC_F64; g.alpha = 1.0; g.beta = 0.0;
+    g.M = ell; g.N = Q; g.K = R; g.A = X; g.lda = ldx; g.transA = 0;
+    g.B = s.A; g.ldb = s.lda; g.transB = which; g.C = Y; g.ldc = ldy;
+    return lra_gemm(c, g);
+  }
+  double *Xt = nullptr, *Yt = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_OP2, (size_t)std::max<i64>(R * ell, 1), &Xt));
+  LRA_CHECK(lra_wsT(c, WS_OP3, (size_t)std::max<i64>(Q * ell, 1), &Yt));
+  LRA_CHECK(lra_launch_transpose(c->stream, ell, R, X, ldx, Xt, std::max<i64>(R, 1)));
+  const int st = s.apply(s.user, which == 0 ? 1 : 0, ell, Xt, std::max<i64>(R, 1), Yt,
+                         std::max<i64>(Q, 1), (void*)c->stream);
+  if (st != 0) {
+    lra_set_error("operator apply returned %d", st);
+    return LRA_E_STATE;
+  }
+  return lra_launch_transpose(c->stream, Q, ell, Yt, std::max<i64>(Q, 1), Y, ldy);
+}
+
 // sketch_randn_ln / _lc power iterations (src/sketch.jl:129-174): Bp (ell x N) alternately
 // orthonormalised and multiplied by op(A)' and op(A); a Hermitian A skips the second
-// orthonormalisation (the two sketches swap).
-static int power_iterate(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
-                         i64 ell, double* B, i64 niter, bool herm) {
-  const i64 N = trans == LRA_TRANS_N ? n : m, P = trans == LRA_TRANS_N ? m : n;
+// orthonormalisation (the two sketches swap).  A is a dense matrix or an operator.
+static int power_iterate(lra_ctx* c, int trans, const LraSrc& s, i64 ell, double* B, i64 niter,
+                         bool herm) {
+  const i64 N = trans == LRA_TRANS_N ? s.n : s.m, P = trans == LRA_TRANS_N ? s.m : s.n;
+  // op(A)' is A' (which 1) for trans n and A (which 0) for trans c; op(A) the other one
+  const int w_opt = trans == LRA_TRANS_N ? 1 : 0, w_op = 1 - w_opt;
   double* Bq = nullptr;
   LRA_CHECK(lra_wsT(c, WS_PI2, (size_t)ell * std::max(N, P), &Bq));
   for (i64 it = 0; it < niter; ++it) {
     LRA_CHECK(orthrows(c, ell, N, B));
-    // Bq (ell x P) = Bp op(A)'
-    GemmArgs g;
-    g.bgather = nullptr; g.prec = LRA_PREC_F64; g.alpha = 1.0; g.beta = 0.0;
-    g.M = ell; g.N = P; g.K = N; g.A = B; g.lda = ell; g.transA = 0;
-    g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 1 : 0; g.C = Bq; g.ldc = ell;
-    LRA_CHECK(lra_gemm(c, g));
+    LRA_CHECK(src_rmul(c, s, w_opt, ell, B, ell, Bq, ell));  // Bq (ell x P) = Bp op(A)'
     if (herm) {  // Bp, Bq = Bq, Bp (P == N)
       LRA_CHECK(lra_launch_copy2d(c->stream, ell, N, Bq, ell, B, ell));
       continue;
     }
     LRA_CHECK(orthrows(c, ell, P, Bq));
-    // Bp (ell x N) = Bq op(A)
-    g.M = ell; g.N = N; g.K = P; g.A = Bq; g.lda = ell; g.transA = 0;
-    g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 0 : 1; g.C = B; g.ldc = ell;
-    LRA_CHECK(lra_gemm(c, g));
+    LRA_CHECK(src_rmul(c, s, w_op, ell, Bq, ell, B, ell));  // Bp (ell x N) = Bq op(A)
   }
   return LRA_OK;
 }
 
-static int sketchfact_left(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
-                           const lra_opts* o, SketchFactOut* out, uint64_t stream_base = 0) {
-  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(A)
+void lra_opts_for_operator(lra_opts* o) {
+  if (o->sketch != LRA_SKETCH_RANDN) {
+    if (o->verb) fprintf(stderr, "lra: invalid sketch method for an operator; using randn\n");
+    o->sketch = LRA_SKETCH_RANDN;
+  }
+  o->precision = LRA_PREC_F64;  // the operator is applied in FP64; no entries to refine from
+}
+
+// Gaussian sketch B (ell x N) = Omega * op(L) of an operator, formed as (op(L)' Omega')':
+// one application of the caller's operator to ell vectors (Omega the ell x P Philox matrix of
+// the round, generated transposed).
+static int op_sketch(lra_ctx* c, int trans, const LraSrc& s, i64 ell, uint64_t seed,
+                     uint64_t stream, double* B) {
+  const i64 N = trans == LRA_TRANS_N ? s.n : s.m, P = trans == LRA_TRANS_N ? s.m : s.n;
+  double *X = nullptr, *Y = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_OP0, (size_t)std::max<i64>(P * ell, 1), &X));
+  LRA_CHECK(lra_wsT(c, WS_OP1, (size_t)std::max<i64>(N * ell, 1), &Y));
+  LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, seed, stream, ell, P, X, std::max<i64>(P, 1)));
+  const int st = s.apply(s.user, trans == LRA_TRANS_N ? 1 : 0, ell, X, std::max<i64>(P, 1), Y,
+                         std::max<i64>(N, 1), (void*)c->stream);
+  if (st != 0) {
+    lra_set_error("operator apply returned %d", st);
+    return LRA_E_STATE;
+  }
+  return lra_launch_transpose(c->stream, N, ell, Y, std::max<i64>(N, 1), B, ell);
+}
+
+// sketchfact(:left, trans, A) on a dense matrix (any sketch) or an operator (Gaussian only,
+// the caller has applied lra_opts_for_operator), with sketch_randn_niter power iterations.
+static int sketchfact_left(lra_ctx* c, int trans, const LraSrc& s, const lra_opts* o,
+                           SketchFactOut* out, uint64_t stream_base = 0) {
+  const i64 N = trans == LRA_TRANS_N ? s.n : s.m;  // columns of op(A)
   const i64 niter = o->sketch == LRA_SKETCH_RANDN ? o->sketch_randn_niter : 0;
   bool herm = false;
-  if (niter > 0 && m == n) {  // ishermitian(A) (src/sketch.jl:132)
-    int* flag = nullptr;
-    LRA_CHECK(lra_wsT(c, WS_PI2, 64, &flag));
-    LRA_CHECK(lra_launch_is_symmetric(c->stream, n, A, lda, flag));
-    int h = 0;
-    LRA_CHECK(read_scalars(c, flag, &h, 1));
-    herm = h != 0;
+  if (niter > 0 && s.m == s.n) {  // ishermitian(A) (src/sketch.jl:132)
+    if (s.op()) {
+      herm = s.herm;
+    } else {
+      int* flag = nullptr;
+      LRA_CHECK(lra_wsT(c, WS_PI2, 64, &flag));
+      LRA_CHECK(lra_launch_is_symmetric(c->stream, s.n, s.A, s.lda, flag));
+      int h = 0;
+      LRA_CHECK(read_scalars(c, flag, &h, 1));
+      herm = h != 0;
+    }
   }
   SketchFn make = [=](i64 ell, uint64_t stream, int prec, double* B) -> int {
-    LRA_CHECK(lra_sketch_any(c, o->sketch, trans, m, n, A, lda, ell, o->seed, stream, prec, B,
-                             ell));
-    if (niter > 0) LRA_CHECK(power_iterate(c, trans, m, n, A, lda, ell, B, niter, herm));
+    if (s.op()) LRA_CHECK(op_sketch(c, trans, s, ell, o->seed, stream, B));
+    else LRA_CHECK(lra_sketch_any(c, o->sketch, trans, s.m, s.n, s.A, s.lda, ell, o->seed,
+                                  stream, prec, B, ell));
+    if (niter > 0) LRA_CHECK(power_iterate(c, trans, s, ell, B, niter, herm));
     return LRA_OK;
   };
   return sketchfact_run(c, N, make, o, stream_base, out);
+}
+
+int lra_src_getcols(lra_ctx* c, const LraSrc& s, int trans, const i64* idx, i64 k, double* C,
+                    i64 ldc) {
+  if (k <= 0) return LRA_OK;
+  if (!s.op()) {
+    if (trans == LRA_TRANS_N) return lra_launch_gather_cols(c->stream, s.m, k, s.A, s.lda, idx, C, ldc);
+    return lra_launch_gather_rows_t(c->stream, s.n, k, s.A, s.lda, idx, C, ldc);
+  }
+  // op(L)[:, idx] = op(L) * S for the one-hot S (R x k) (src/linop.jl:54-67)
+  const i64 R = trans == LRA_TRANS_N ? s.n : s.m;
+  double* S = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_OP2, (size_t)std::max<i64>(R * k, 1), &S));
+  LRA_CHECK(lra_launch_onehot(c->stream, R, k, idx, S, std::max<i64>(R, 1)));
+  const int st = s.apply(s.user, trans == LRA_TRANS_N ? 0 : 1, k, S, std::max<i64>(R, 1), C, ldc,
+                         (void*)c->stream);
+  if (st != 0) {
+    lra_set_error("operator apply returned %d", st);
+    return LRA_E_STATE;
+  }
+  return LRA_OK;
 }
 
 // Low-precision sketch selected the skeleton: recompute T in FP64 from A's skeleton columns
@@ -661,19 +761,13 @@ using RefineFn = std::function<int(const i64*, const i64*, double*, i64)>;
 static int finish_id(lra_ctx* c, const SketchFactOut& f, i64 N, const lra_opts* op,
                      lra_id_out* out, const RefineFn& refine);
 
-extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
-                          int64_t lda, const lra_opts* opts, lra_id_out* out) {
-  LRA_ARG(c != nullptr, 1);
-  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
-  LRA_ARG(m >= 0, 3);
-  LRA_ARG(n >= 0, 4);
-  LRA_ARG(A != nullptr || m * n == 0, 5);
-  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
-  LRA_ARG(opts != nullptr, 7);
-  LRA_ARG(out != nullptr, 8);
+int lra_idfact_src(lra_ctx* c, int trans, const LraSrc& s, const lra_opts* opts,
+                   lra_id_out* out) {
   if (chkopts(opts) != LRA_OK) return -7;
   lra_opts o = *opts;
   o.pqrfact_retval = LRA_RET_T;
+  if (s.op()) lra_opts_for_operator(&o);
+  const i64 m = s.m, n = s.n;
   const i64 N = trans == LRA_TRANS_N ? n : m;
   SketchFactOut f{};
   if (o.sketch == LRA_SKETCH_NONE) {
@@ -681,8 +775,8 @@ extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const dou
     const i64 P = trans == LRA_TRANS_N ? m : n;
     double* B = nullptr;
     LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)std::max<i64>(P * N, 1), &B));
-    if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, A, lda, B, m));
-    else LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, B, n));
+    if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, s.A, s.lda, B, m));
+    else LRA_CHECK(lra_launch_transpose(c->stream, m, n, s.A, s.lda, B, n));
     i64* jpvt = nullptr;
     double* tau = nullptr;
     LRA_CHECK(lra_wsT(c, WS_JPVT, (size_t)std::max<i64>(N, 1), &jpvt));
@@ -693,11 +787,25 @@ extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const dou
     f.B = B; f.Bsave = nullptr; f.ell = P; f.N = N; f.jpvt = jpvt; f.tau = tau; f.k = k;
     f.rounds = 1; f.prec = LRA_PREC_F64; f.stream_id = 0;
   } else {
-    LRA_CHECK(sketchfact_left(c, trans, m, n, A, lda, &o, &f));
+    LRA_CHECK(sketchfact_left(c, trans, s, &o, &f));
   }
   return finish_id(c, f, N, &o, out, [&](const i64* sk, const i64* rd, double* T, i64 ldt) {
-    return mixed_refine_T(c, trans, m, n, A, lda, f, sk, rd, T, ldt, &o);
+    if (s.op()) return (int)LRA_OK;  // operators are sketched in FP64 (no refinement)
+    return mixed_refine_T(c, trans, m, n, s.A, s.lda, f, sk, rd, T, ldt, &o);
   });
+}
+
+extern "C" int lra_idfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                          int64_t lda, const lra_opts* opts, lra_id_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(A != nullptr || m * n == 0, 5);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(out != nullptr, 8);
+  return lra_idfact_src(c, trans, lra_src_dense(m, n, A, lda), opts, out);
 }
 
 // The tail of idfact shared by the dense and matrix-free drivers: sk/rd from the pivots,
@@ -740,62 +848,37 @@ static int finish_id(lra_ctx* c, const SketchFactOut& f, i64 N, const lra_opts* 
 // ---------------------------------------------------------------------------------------
 // matrix-free idfact (idfact(L::LinearOperator), src/id.jl:434-447 with the LinOp products
 // of src/linop.jl:105-113): the Gaussian sketch Omega*op(L) is formed as (op(L)' Omega')',
-// one application of the caller's operator to ell vectors per adaptive round.
+// one application of the caller's operator to ell vectors per adaptive round, and each power
+// iteration applies op(L) and op(L)' once more (src/sketch.jl:129-174).
 
 extern "C" int lra_idfact_op(lra_ctx* c, int trans, int64_t m, int64_t n, lra_apply_fn apply,
-                             void* user, const lra_opts* opts, lra_id_out* out) {
+                             void* user, int isherm, const lra_opts* opts, lra_id_out* out) {
   LRA_ARG(c != nullptr, 1);
   LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
   LRA_ARG(m >= 0, 3);
   LRA_ARG(n >= 0, 4);
   LRA_ARG(apply != nullptr, 5);
-  LRA_ARG(opts != nullptr, 7);
-  LRA_ARG(out != nullptr, 8);
-  if (chkopts(opts) != LRA_OK) return -7;
+  LRA_ARG(!isherm || m == n, 7);
+  LRA_ARG(opts != nullptr, 8);
+  LRA_ARG(out != nullptr, 9);
   LRA_HIP(hipSetDevice(c->device));
-  lra_opts o = *opts;
-  o.pqrfact_retval = LRA_RET_T;
-  if (o.sketch != LRA_SKETCH_RANDN) {
-    lra_set_error("matrix-free idfact takes the Gaussian sketch (sketch=%d needs matrix entries)",
-                  o.sketch);
-    return LRA_E_UNSUPPORTED;
-  }
-  o.precision = LRA_PREC_F64;  // the operator is applied in FP64; no matrix for a refinement
-  const i64 N = trans == LRA_TRANS_N ? n : m;  // columns of op(L)
-  const i64 P = trans == LRA_TRANS_N ? m : n;  // rows of op(L)
-  const int tr = trans == LRA_TRANS_N ? 1 : 0; // op(L)' = L' for :n, L for :c
-  SketchFn make = [&](i64 ell, uint64_t stream, int, double* B) -> int {
-    double *X = nullptr, *Y = nullptr;
-    LRA_CHECK(lra_wsT(c, WS_OP0, (size_t)std::max<i64>(P * ell, 1), &X));
-    LRA_CHECK(lra_wsT(c, WS_OP1, (size_t)std::max<i64>(N * ell, 1), &Y));
-    // X = Omega' (P x ell), Omega the ell x P Philox matrix of this round
-    LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, o.seed, stream, ell, P, X, std::max<i64>(P, 1)));
-    const int st = apply(user, tr, ell, X, std::max<i64>(P, 1), Y, std::max<i64>(N, 1),
-                         (void*)c->stream);
-    if (st != 0) {
-      lra_set_error("operator apply returned %d", st);
-      return LRA_E_STATE;
-    }
-    return lra_launch_transpose(c->stream, N, ell, Y, std::max<i64>(N, 1), B, ell);
-  };
-  SketchFactOut f{};
-  LRA_CHECK(sketchfact_run(c, N, make, &o, 0, &f));
-  return finish_id(c, f, N, &o, out, [](const i64*, const i64*, double*, i64) { return 0; });
+  return lra_idfact_src(c, trans, lra_src_op(m, n, apply, user, isherm != 0), opts, out);
 }
 
 // ---------------------------------------------------------------------------------------
 // sketchfact (src/sketch.jl:52-66) with pqrfact_retval = "": column pivots and rank of
 // op(A); sketch = none is pqrfact_backend! on op(A) itself (src/pqr.jl:323-327,343-346).
 
-static int pivots_of(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
-                     const lra_opts* o, uint64_t stream_base, SketchFactOut* f) {
+static int pivots_of(lra_ctx* c, int trans, const LraSrc& s, const lra_opts* o,
+                     uint64_t stream_base, SketchFactOut* f) {
+  const i64 m = s.m, n = s.n;
   const i64 N = trans == LRA_TRANS_N ? n : m;
-  if (o->sketch != LRA_SKETCH_NONE) return sketchfact_left(c, trans, m, n, A, lda, o, f, stream_base);
+  if (o->sketch != LRA_SKETCH_NONE) return sketchfact_left(c, trans, s, o, f, stream_base);
   const i64 P = trans == LRA_TRANS_N ? m : n;
   double* B = nullptr;
   LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)std::max<i64>(P * N, 1), &B));
-  if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, A, lda, B, std::max<i64>(m, 1)));
-  else LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, B, std::max<i64>(n, 1)));
+  if (trans == LRA_TRANS_N) LRA_CHECK(lra_launch_copy2d(c->stream, m, n, s.A, s.lda, B, std::max<i64>(m, 1)));
+  else LRA_CHECK(lra_launch_transpose(c->stream, m, n, s.A, s.lda, B, std::max<i64>(n, 1)));
   i64* jpvt = nullptr;
   double* tau = nullptr;
   LRA_CHECK(lra_wsT(c, WS_JPVT, (size_t)std::max<i64>(N, 1), &jpvt));
@@ -826,7 +909,7 @@ extern "C" int lra_sketchfact(lra_ctx* c, int trans, int64_t m, int64_t n, const
   lra_opts o = *opts;
   o.precision = LRA_PREC_F64;
   SketchFactOut f{};
-  LRA_CHECK(pivots_of(c, trans, m, n, A, lda, &o, stream_base, &f));
+  LRA_CHECK(pivots_of(c, trans, lra_src_dense(m, n, A, lda), &o, stream_base, &f));
   const i64 N = trans == LRA_TRANS_N ? n : m;
   if (N > 0)
     LRA_HIP(hipMemcpyAsync(p, f.jpvt, sizeof(i64) * N, hipMemcpyDeviceToDevice, c->stream));
@@ -835,7 +918,87 @@ extern "C" int lra_sketchfact(lra_ctx* c, int trans, int64_t m, int64_t n, const
 }
 
 // ---------------------------------------------------------------------------------------
-// curfact (src/cur.jl:532-564): row and column skeletons from two chained sketchfacts.
+// curfact (src/cur.jl:532-564): row and column skeletons from two chained sketchfacts.  The
+// first skeleton (of length kr or kc, which may exceed min(m,n) when the sketch shows more
+// rank than the matrix has, e.g. at rtol = 0) stays in a library workspace; only the final
+// min(kr, kc) indices are written to the caller's buffers.
+
+static int curfact_src(lra_ctx* c, const LraSrc& s, const lra_opts* opts, int64_t* rows,
+                       int64_t* cols, int64_t* k) {
+  if (chkopts(opts) != LRA_OK) return -6;
+  LRA_HIP(hipSetDevice(c->device));
+  lra_opts o = *opts;
+  o.pqrfact_retval = 0;
+  o.precision = LRA_PREC_F64;
+  if (s.op()) lra_opts_for_operator(&o);
+  const i64 m = s.m, n = s.n;
+  *k = 0;
+  if (m == 0 || n == 0) return LRA_OK;
+  // ishermitian(A) (exactly symmetric for real A): one column sketchfact, rows = cols
+  bool herm = false;
+  if (m == n) {
+    if (s.op()) {
+      herm = s.herm;
+    } else {
+      int* flag = nullptr;
+      LRA_CHECK(lra_wsT(c, WS_CUR0, 64, &flag));
+      LRA_CHECK(lra_launch_is_symmetric(c->stream, n, s.A, s.lda, flag));
+      int h = 0;
+      LRA_CHECK(read_scalars(c, flag, &h, 1));
+      herm = h != 0;
+    }
+  }
+  SketchFactOut f{};
+  if (herm) {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_N, s, &o, 0, &f));
+    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
+    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
+    *k = f.k;
+    LRA_HIP(hipStreamSynchronize(c->stream));
+    return LRA_OK;
+  }
+  // the second sketchfact draws from Philox streams 2^32 + round
+  const uint64_t base2 = 1ull << 32;
+  i64 kr = 0, kc = 0;
+  double* X = nullptr;
+  i64* first = nullptr;  // the first stage's skeleton (up to m or n entries)
+  LRA_CHECK(lra_wsT(c, WS_CUR1, (size_t)std::max<i64>(std::max(m, n), 1), &first));
+  if (m >= n) {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, s, &o, 0, &f));
+    kr = f.k;
+    if (kr > 0)
+      LRA_HIP(hipMemcpyAsync(first, f.jpvt, sizeof(i64) * kr, hipMemcpyDeviceToDevice, c->stream));
+    // A[rows, :] as the transpose of X = A[rows, :]' (n x kr), factored with trans = c
+    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(n * kr, 1) + 8, &X));
+    LRA_CHECK(lra_src_getcols(c, s, LRA_TRANS_C, first, kr, X, n));
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, lra_src_dense(n, kr, X, std::max<i64>(n, 1)), &o, base2,
+                        &f));
+    kc = f.k;
+    const i64 kk = std::min(kr, kc);
+    if (kk > 0) {
+      LRA_HIP(hipMemcpyAsync(rows, first, sizeof(i64) * kk, hipMemcpyDeviceToDevice, c->stream));
+      LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * kk, hipMemcpyDeviceToDevice, c->stream));
+    }
+  } else {
+    LRA_CHECK(pivots_of(c, LRA_TRANS_N, s, &o, 0, &f));
+    kc = f.k;
+    if (kc > 0)
+      LRA_HIP(hipMemcpyAsync(first, f.jpvt, sizeof(i64) * kc, hipMemcpyDeviceToDevice, c->stream));
+    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(m * kc, 1) + 8, &X));
+    LRA_CHECK(lra_src_getcols(c, s, LRA_TRANS_N, first, kc, X, m));
+    LRA_CHECK(pivots_of(c, LRA_TRANS_C, lra_src_dense(m, kc, X, std::max<i64>(m, 1)), &o, base2,
+                        &f));
+    kr = f.k;
+    const i64 kk = std::min(kr, kc);
+    if (kk > 0) {
+      LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * kk, hipMemcpyDeviceToDevice, c->stream));
+      LRA_HIP(hipMemcpyAsync(cols, first, sizeof(i64) * kk, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  *k = std::min(kr, kc);
+  LRA_HIP(hipStreamSynchronize(c->stream));
+  return LRA_OK;
+}
 
 extern "C" int lra_curfact(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t lda,
                            const lra_opts* opts, int64_t* rows, int64_t* cols, int64_t* k) {
@@ -848,60 +1011,25 @@ extern "C" int lra_curfact(lra_ctx* c, int64_t m, int64_t n, const double* A, in
   LRA_ARG(rows != nullptr || std::min(m, n) == 0, 7);
   LRA_ARG(cols != nullptr || std::min(m, n) == 0, 8);
   LRA_ARG(k != nullptr, 9);
-  if (chkopts(opts) != LRA_OK) return -6;
-  LRA_HIP(hipSetDevice(c->device));
-  lra_opts o = *opts;
-  o.pqrfact_retval = 0;
-  o.precision = LRA_PREC_F64;
-  *k = 0;
-  if (m == 0 || n == 0) return LRA_OK;
-  // ishermitian(A) (exactly symmetric for real A): one column sketchfact, rows = cols
-  bool herm = false;
-  if (m == n) {
-    int* flag = nullptr;
-    LRA_CHECK(lra_wsT(c, WS_CUR0, 64, &flag));
-    LRA_CHECK(lra_launch_is_symmetric(c->stream, n, A, lda, flag));
-    int h = 0;
-    LRA_CHECK(read_scalars(c, flag, &h, 1));
-    herm = h != 0;
-  }
-  SketchFactOut f{};
-  if (herm) {
-    LRA_CHECK(pivots_of(c, LRA_TRANS_N, m, n, A, lda, &o, 0, &f));
-    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
-    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * f.k, hipMemcpyDeviceToDevice, c->stream));
-    *k = f.k;
-    return LRA_OK;
-  }
-  // the second sketchfact draws from Philox streams 2^32 + round
-  const uint64_t base2 = 1ull << 32;
-  i64 kr = 0, kc = 0;
-  double* X = nullptr;
-  if (m >= n) {
-    LRA_CHECK(pivots_of(c, LRA_TRANS_C, m, n, A, lda, &o, 0, &f));
-    kr = f.k;
-    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * kr, hipMemcpyDeviceToDevice, c->stream));
-    // A[rows, :] as the transpose of X = A[rows, :]' (n x kr), factored with trans = c
-    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(n * kr, 1) + 8, &X));
-    if (kr > 0) LRA_CHECK(lra_launch_gather_rows_t(c->stream, n, kr, A, lda, rows, X, n));
-    LRA_CHECK(pivots_of(c, LRA_TRANS_C, n, kr, X, std::max<i64>(n, 1), &o, base2, &f));
-    kc = f.k;
-    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * std::min(kr, kc), hipMemcpyDeviceToDevice,
-                           c->stream));
-  } else {
-    LRA_CHECK(pivots_of(c, LRA_TRANS_N, m, n, A, lda, &o, 0, &f));
-    kc = f.k;
-    LRA_HIP(hipMemcpyAsync(cols, f.jpvt, sizeof(i64) * kc, hipMemcpyDeviceToDevice, c->stream));
-    LRA_CHECK(lra_wsT(c, WS_CUR0, (size_t)std::max<i64>(m * kc, 1) + 8, &X));
-    if (kc > 0) LRA_CHECK(lra_launch_gather_cols(c->stream, m, kc, A, lda, cols, X, m));
-    LRA_CHECK(pivots_of(c, LRA_TRANS_C, m, kc, X, std::max<i64>(m, 1), &o, base2, &f));
-    kr = f.k;
-    LRA_HIP(hipMemcpyAsync(rows, f.jpvt, sizeof(i64) * std::min(kr, kc), hipMemcpyDeviceToDevice,
-                           c->stream));
-  }
-  *k = std::min(kr, kc);
-  LRA_HIP(hipStreamSynchronize(c->stream));
-  return LRA_OK;
+  return curfact_src(c, lra_src_dense(m, n, A, lda), opts, rows, cols, k);
+}
+
+// curfact on an operator (src/cur.jl:527-564 with A a LinearOperator): the first sketchfact
+// applies the operator, A[rows, :] / A[:, cols] come from one application to one-hot vectors
+// (src/linop.jl:54-67), and the second sketchfact runs on that dense block.
+extern "C" int lra_curfact_op(lra_ctx* c, int64_t m, int64_t n, lra_apply_fn apply, void* user,
+                              int isherm, const lra_opts* opts, int64_t* rows, int64_t* cols,
+                              int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(apply != nullptr, 4);
+  LRA_ARG(!isherm || m == n, 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(rows != nullptr || std::min(m, n) == 0, 8);
+  LRA_ARG(cols != nullptr || std::min(m, n) == 0, 9);
+  LRA_ARG(k != nullptr, 10);
+  return curfact_src(c, lra_src_op(m, n, apply, user, isherm != 0), opts, rows, cols, k);
 }
 
 // ---------------------------------------------------------------------------------------

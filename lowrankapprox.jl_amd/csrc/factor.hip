@@ -154,8 +154,8 @@ struct IdOut {
   i64 rounds;
 };
 
-int run_id(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const lra_opts* o,
-           IdOut* r) {
+int run_id(lra_ctx* c, int trans, const LraSrc& src, const lra_opts* o, IdOut* r) {
+  const i64 m = src.m, n = src.n;
   const i64 N = trans == LRA_TRANS_N ? n : m;
   r->N = N;
   r->P = trans == LRA_TRANS_N ? m : n;
@@ -165,7 +165,7 @@ int run_id(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const 
   LRA_CHECK(lra_wsT(c, WS_F2, (size_t)std::max<i64>(N, 1), &p));
   lra_id_out out{};
   out.sk = sk; out.rd = rd; out.T = nullptr; out.ldt = 0;
-  LRA_CHECK(lra_idfact(c, trans, m, n, A, lda, o, &out));
+  LRA_CHECK(lra_idfact_src(c, trans, src, o, &out));
   r->k = out.k;
   r->sk = sk; r->rd = rd; r->p = p;
   r->T = out.T; r->ldt = out.ldt; r->rounds = out.rounds;
@@ -177,19 +177,11 @@ int run_id(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const 
   return LRA_OK;
 }
 
-// C (P x k) = op(A)[:, sk] (getcols, src/util.jl:25-34)
-int getcols(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const i64* sk,
-            i64 k, double* C, i64 ldc) {
-  if (k <= 0) return LRA_OK;
-  if (trans == LRA_TRANS_N) return lra_launch_gather_cols(c->stream, m, k, A, lda, sk, C, ldc);
-  return lra_launch_gather_rows_t(c->stream, n, k, A, lda, sk, C, ldc);
-}
-
 // Thin QR of the skeleton columns: Cq (P x k) overwritten by the geqrf factors, tau (k),
 // and (when Q != nullptr) the explicit Q (P x k).
-int skeleton_qr(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const IdOut& id,
-                double* Cq, double* tau, double* Q, i64 ldq) {
-  LRA_CHECK(getcols(c, trans, m, n, A, lda, id.sk, id.k, Cq, id.P));
+int skeleton_qr(lra_ctx* c, int trans, const LraSrc& src, const IdOut& id, double* Cq,
+                double* tau, double* Q, i64 ldq) {
+  LRA_CHECK(lra_src_getcols(c, src, trans, id.sk, id.k, Cq, id.P));
   LRA_CHECK(lra_geqrf_impl(c, id.P, id.k, Cq, id.P, tau));
   if (Q) LRA_CHECK(lra_orgqr_impl(c, id.P, id.k, Cq, id.P, tau, Q, ldq));
   return LRA_OK;
@@ -230,10 +222,10 @@ struct SvdCore {
   double *Q, *Q2, *Us, *W, *s;  // Q (P x k), Q2 (N x k), Us, W (k x k), s (k)
 };
 
-int svd_core(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, const lra_opts* o,
-             bool vectors, SvdCore* out) {
+int svd_core(lra_ctx* c, int trans, const LraSrc& src, const lra_opts* o, bool vectors,
+             SvdCore* out) {
   IdOut id;
-  LRA_CHECK(run_id(c, trans, m, n, A, lda, o, &id));
+  LRA_CHECK(run_id(c, trans, src, o, &id));
   const i64 k = id.k, P = id.P, N = id.N;
   out->k = k; out->P = P; out->N = N;
   if (k == 0) return LRA_OK;
@@ -241,7 +233,7 @@ int svd_core(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, cons
   LRA_CHECK(lra_wsT(c, WS_F3, (size_t)P * k, &Cq));
   LRA_CHECK(lra_wsT(c, WS_F4, (size_t)2 * k + 16, &tau));
   if (vectors) LRA_CHECK(lra_wsT(c, WS_F5, (size_t)P * k, &Q));
-  LRA_CHECK(skeleton_qr(c, trans, m, n, A, lda, id, Cq, tau, Q, P));
+  LRA_CHECK(skeleton_qr(c, trans, src, id, Cq, tau, Q, P));
   // Z = R1 [I T] (k x N): R1 = triu(Cq(0:k, 0:k)), Z(:, k:N) = R1 T
   LRA_CHECK(lra_wsT(c, WS_F6, (size_t)k * N, &Z));
   hipLaunchKernelGGL(triu_copy_kernel, dim3(grid_of(k * k)), dim3(256), 0, c->stream, k, k, Cq,
@@ -283,22 +275,18 @@ int svd_core(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda, cons
 // Q (P x k), R = [R1, R1 T] (k x N, pqrr :330-340), p = [sk; rd], T.  Which of Q, R, T are
 // formed follows pqrfact_retval (bits LRA_RET_Q/R/T); sketch = none runs the rank-adaptive
 // QRCP on op(A) itself (pqrfact_none -> pqrback_postproc, :323-327, :420-436).
-extern "C" int lra_pqrfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
-                           int64_t lda, const lra_opts* opts, lra_pqr_out* out) {
-  LRA_ARG(c != nullptr, 1);
-  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
-  LRA_ARG(m >= 0, 3);
-  LRA_ARG(n >= 0, 4);
-  LRA_ARG(A != nullptr || m * n == 0, 5);
-  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
-  LRA_ARG(opts != nullptr, 7);
-  LRA_ARG(out != nullptr, 8);
-  if (opts->sketch == LRA_SKETCH_NONE && opts->maxdet_tol >= 0) {
+static int pqrfact_src(lra_ctx* c, int trans, const LraSrc& src, const lra_opts* opts,
+                       lra_pqr_out* out) {
+  lra_opts o = *opts;
+  if (src.op()) lra_opts_for_operator(&o);
+  if (o.sketch == LRA_SKETCH_NONE && o.maxdet_tol >= 0) {
     lra_set_error("pqrfact: sketch = none with maxdet swaps is not supported");
     return LRA_E_UNSUPPORTED;
   }
   LRA_HIP(hipSetDevice(c->device));
-  lra_opts o = *opts;
+  const i64 m = src.m, n = src.n;
+  const double* A = src.A;
+  const i64 lda = src.lda;
   const int ret = o.pqrfact_retval;
   const i64 N = trans == LRA_TRANS_N ? n : m, P = trans == LRA_TRANS_N ? m : n;
   out->k = 0;
@@ -336,7 +324,7 @@ extern "C" int lra_pqrfact(lra_ctx* c, int trans, int64_t m, int64_t n, const do
     return lra_check_device_error(c);
   }
   IdOut id;
-  LRA_CHECK(run_id(c, trans, m, n, A, lda, &o, &id));
+  LRA_CHECK(run_id(c, trans, src, &o, &id));
   const i64 k = id.k;
   out->k = k;
   out->rounds = id.rounds;
@@ -352,7 +340,7 @@ extern "C" int lra_pqrfact(lra_ctx* c, int trans, int64_t m, int64_t n, const do
       if (!out->Q) { double* q = nullptr; LRA_CHECK(lra_wsT(c, WS_F5, (size_t)P * k, &q)); out->Q = q; out->ldq = P; }
       Q = out->Q;
     }
-    LRA_CHECK(skeleton_qr(c, trans, m, n, A, lda, id, Cq, tau, Q, out->ldq));
+    LRA_CHECK(skeleton_qr(c, trans, src, id, Cq, tau, Q, out->ldq));
     if (ret & LRA_RET_R) {
       if (!out->R) { double* r = nullptr; LRA_CHECK(lra_wsT(c, WS_F7, (size_t)k * N, &r)); out->R = r; out->ldr = k; }
       hipLaunchKernelGGL(triu_copy_kernel, dim3(grid_of(k * k)), dim3(256), 0, c->stream, k, k,
@@ -370,22 +358,45 @@ extern "C" int lra_pqrfact(lra_ctx* c, int trans, int64_t m, int64_t n, const do
   return lra_check_device_error(c);
 }
 
+extern "C" int lra_pqrfact(lra_ctx* c, int trans, int64_t m, int64_t n, const double* A,
+                           int64_t lda, const lra_opts* opts, lra_pqr_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(A != nullptr || m * n == 0, 5);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(out != nullptr, 8);
+  return pqrfact_src(c, trans, lra_src_dense(m, n, A, lda), opts, out);
+}
+
+// pqrfact on an operator (src/pqr.jl:285-320 with A a LinearOperator): the skeleton columns
+// come from one application to one-hot vectors; sketch none falls back to randn as chkopts!
+// does for operators (src/LowRankApprox.jl:142-148).
+extern "C" int lra_pqrfact_op(lra_ctx* c, int trans, int64_t m, int64_t n, lra_apply_fn apply,
+                              void* user, int isherm, const lra_opts* opts, lra_pqr_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(trans == LRA_TRANS_N || trans == LRA_TRANS_C, 2);
+  LRA_ARG(m >= 0, 3);
+  LRA_ARG(n >= 0, 4);
+  LRA_ARG(apply != nullptr, 5);
+  LRA_ARG(!isherm || m == n, 7);
+  LRA_ARG(opts != nullptr, 8);
+  LRA_ARG(out != nullptr, 9);
+  return pqrfact_src(c, trans, lra_src_op(m, n, apply, user, isherm != 0), opts, out);
+}
+
 // ---------------------------------------------------------------------------------------
 // psvdfact / psvdvals (src/psvd.jl:238-308)
-static int psvd_impl(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t lda,
-                     const lra_opts* opts, bool vectors, lra_psvd_out* out) {
-  LRA_ARG(c != nullptr, 1);
-  LRA_ARG(m >= 0, 2);
-  LRA_ARG(n >= 0, 3);
-  LRA_ARG(A != nullptr || m * n == 0, 4);
-  LRA_ARG(lda >= std::max<int64_t>(1, m), 5);
-  LRA_ARG(opts != nullptr, 6);
-  LRA_ARG(out != nullptr, 7);
+static int psvd_impl(lra_ctx* c, const LraSrc& src, const lra_opts* opts, bool vectors,
+                     lra_psvd_out* out) {
   LRA_HIP(hipSetDevice(c->device));
   const lra_opts o = *opts;
+  const i64 m = src.m, n = src.n;
   const int trans = m >= n ? LRA_TRANS_N : LRA_TRANS_C;
   SvdCore sv{};
-  LRA_CHECK(svd_core(c, trans, m, n, A, lda, &o, vectors, &sv));
+  LRA_CHECK(svd_core(c, trans, src, &o, vectors, &sv));
   out->k = 0;
   if (sv.k == 0) return LRA_OK;
   std::vector<double> hs;
@@ -413,18 +424,65 @@ static int psvd_impl(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t 
   return lra_check_device_error(c);
 }
 
+#define LRA_DENSE_ARGS(first)                          \
+  LRA_ARG(c != nullptr, 1);                            \
+  LRA_ARG(m >= 0, first);                              \
+  LRA_ARG(n >= 0, first + 1);                          \
+  LRA_ARG(A != nullptr || m * n == 0, first + 2);      \
+  LRA_ARG(lda >= std::max<int64_t>(1, m), first + 3)
+
 extern "C" int lra_psvdfact(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t lda,
                             const lra_opts* opts, lra_psvd_out* out) {
-  return psvd_impl(c, m, n, A, lda, opts, true, out);
+  LRA_DENSE_ARGS(2);
+  LRA_ARG(opts != nullptr, 6);
+  LRA_ARG(out != nullptr, 7);
+  return psvd_impl(c, lra_src_dense(m, n, A, lda), opts, true, out);
 }
 
 extern "C" int lra_psvdvals(lra_ctx* c, int64_t m, int64_t n, const double* A, int64_t lda,
                             const lra_opts* opts, double* S, int64_t* k) {
+  LRA_DENSE_ARGS(2);
+  LRA_ARG(opts != nullptr, 6);
   LRA_ARG(k != nullptr, 8);
   lra_psvd_out o{};
   o.S = S;
-  const int st = psvd_impl(c, m, n, A, lda, opts, false, &o);
+  const int st = psvd_impl(c, lra_src_dense(m, n, A, lda), opts, false, &o);
   *k = o.k;
+  return st;
+}
+
+// psvdfact / psvdvals on an operator (src/psvd.jl:238-290 with A a LinearOperator)
+extern "C" int lra_psvdfact_op(lra_ctx* c, int64_t m, int64_t n, lra_apply_fn apply,
+                               void* user, int isherm, const lra_opts* opts,
+                               lra_psvd_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(apply != nullptr, 4);
+  LRA_ARG(!isherm || m == n, 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(out != nullptr, 8);
+  lra_opts o = *opts;
+  lra_opts_for_operator(&o);
+  return psvd_impl(c, lra_src_op(m, n, apply, user, isherm != 0), &o, true, out);
+}
+
+extern "C" int lra_psvdvals_op(lra_ctx* c, int64_t m, int64_t n, lra_apply_fn apply,
+                               void* user, int isherm, const lra_opts* opts, double* S,
+                               int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(apply != nullptr, 4);
+  LRA_ARG(!isherm || m == n, 6);
+  LRA_ARG(opts != nullptr, 7);
+  LRA_ARG(k != nullptr, 9);
+  lra_opts o = *opts;
+  lra_opts_for_operator(&o);
+  lra_psvd_out r{};
+  r.S = S;
+  const int st = psvd_impl(c, lra_src_op(m, n, apply, user, isherm != 0), &o, false, &r);
+  *k = r.k;
   return st;
 }
 
@@ -434,18 +492,13 @@ extern "C" int lra_psvdvals(lra_ctx* c, int64_t m, int64_t n, const double* A, i
 // Hermitian(B) by one-sided Jacobi on the positive definite Hermitian(B) + mu I (mu = |B|_F,
 // so the singular pairs are the eigenpairs shifted by mu); pheigrank selection, the cluster
 // re-orthogonalisation pheigorth!, and vectors = Q * vectors.
-static int pheig_impl(lra_ctx* c, int64_t n, const double* A, int64_t lda, const lra_opts* opts,
-                      bool vectors, lra_pheig_out* out) {
-  LRA_ARG(c != nullptr, 1);
-  LRA_ARG(n >= 0, 2);
-  LRA_ARG(A != nullptr || n == 0, 3);
-  LRA_ARG(lda >= std::max<int64_t>(1, n), 4);
-  LRA_ARG(opts != nullptr, 5);
-  LRA_ARG(out != nullptr, 6);
+static int pheig_impl(lra_ctx* c, const LraSrc& src, const lra_opts* opts, bool vectors,
+                      lra_pheig_out* out) {
   LRA_HIP(hipSetDevice(c->device));
   const lra_opts o = *opts;
+  const i64 n = src.n;
   IdOut id;
-  LRA_CHECK(run_id(c, LRA_TRANS_N, n, n, A, lda, &o, &id));
+  LRA_CHECK(run_id(c, LRA_TRANS_N, src, &o, &id));
   const i64 k = id.k, N = n;
   out->k = out->kn = out->kp = 0;
   if (k == 0) return LRA_OK;
@@ -475,9 +528,8 @@ static int pheig_impl(lra_ctx* c, int64_t n, const double* A, int64_t lda, const
   double* scratch = sj + k;
   hipLaunchKernelGGL(triu_copy_kernel, dim3(grid_of(k * k)), dim3(256), 0, c->stream, k, k, Y, N,
                      R, k);
-  hipLaunchKernelGGL(gather2d_kernel, dim3(grid_of(k * k)), dim3(256), 0, c->stream, k, A, lda,
-                     id.sk, id.sk, Ask, k);
   LRA_HIP(hipGetLastError());
+  LRA_CHECK(lra_src_principal(c, src, id.sk, k, Ask, k));
   LRA_CHECK(gemm(c, k, k, k, 1.0, Ask, k, 0, R, k, 1, 0.0, W1, k));
   LRA_CHECK(gemm(c, k, k, k, 1.0, R, k, 0, W1, k, 0, 0.0, B, k));
   hipLaunchKernelGGL(frob2_kernel, dim3(1), dim3(256), 0, c->stream, k, B, k, scratch);
@@ -554,15 +606,73 @@ static int pheig_impl(lra_ctx* c, int64_t n, const double* A, int64_t lda, const
 
 extern "C" int lra_pheigfact(lra_ctx* c, int64_t n, const double* A, int64_t lda,
                              const lra_opts* opts, lra_pheig_out* out) {
-  return pheig_impl(c, n, A, lda, opts, true, out);
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(n >= 0, 2);
+  LRA_ARG(A != nullptr || n == 0, 3);
+  LRA_ARG(lda >= std::max<int64_t>(1, n), 4);
+  LRA_ARG(opts != nullptr, 5);
+  LRA_ARG(out != nullptr, 6);
+  return pheig_impl(c, lra_src_dense(n, n, A, lda), opts, true, out);
 }
 
 extern "C" int lra_pheigvals(lra_ctx* c, int64_t n, const double* A, int64_t lda,
                              const lra_opts* opts, double* values, int64_t* k) {
-  LRA_ARG(k != nullptr, 6);
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(n >= 0, 2);
+  LRA_ARG(A != nullptr || n == 0, 3);
+  LRA_ARG(lda >= std::max<int64_t>(1, n), 4);
+  LRA_ARG(opts != nullptr, 5);
+  LRA_ARG(k != nullptr, 7);
   lra_pheig_out o{};
   o.values = values;
-  const int st = pheig_impl(c, n, A, lda, opts, false, &o);
+  const int st = pheig_impl(c, lra_src_dense(n, n, A, lda), opts, false, &o);
   *k = o.k;
   return st;
+}
+
+// pheigfact / pheigvals on a Hermitian operator (src/pheig.jl:276-314 with A a
+// HermitianLinearOperator): A[sk, sk] comes from one application to one-hot vectors
+// (src/linop.jl:70-74).  The caller guarantees that the operator is Hermitian.
+extern "C" int lra_pheigfact_op(lra_ctx* c, int64_t n, lra_apply_fn apply, void* user,
+                                const lra_opts* opts, lra_pheig_out* out) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(n >= 0, 2);
+  LRA_ARG(apply != nullptr, 3);
+  LRA_ARG(opts != nullptr, 5);
+  LRA_ARG(out != nullptr, 6);
+  lra_opts o = *opts;
+  lra_opts_for_operator(&o);
+  return pheig_impl(c, lra_src_op(n, n, apply, user, true), &o, true, out);
+}
+
+extern "C" int lra_pheigvals_op(lra_ctx* c, int64_t n, lra_apply_fn apply, void* user,
+                                const lra_opts* opts, double* values, int64_t* k) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(n >= 0, 2);
+  LRA_ARG(apply != nullptr, 3);
+  LRA_ARG(opts != nullptr, 5);
+  LRA_ARG(k != nullptr, 7);
+  lra_opts o = *opts;
+  lra_opts_for_operator(&o);
+  lra_pheig_out r{};
+  r.values = values;
+  const int st = pheig_impl(c, lra_src_op(n, n, apply, user, true), &o, false, &r);
+  *k = r.k;
+  return st;
+}
+
+// D (k x k) = A[idx, idx]: a 2-D gather for a dense source, the operator applied to k one-hot
+// vectors and the rows idx of the result kept (getindex(L, rows, cols), src/linop.jl:70-74)
+int lra_src_principal(lra_ctx* c, const LraSrc& s, const i64* idx, i64 k, double* D, i64 ldd) {
+  if (k <= 0) return LRA_OK;
+  if (!s.op()) {
+    hipLaunchKernelGGL(gather2d_kernel, dim3(grid_of(k * k)), dim3(256), 0, c->stream, k, s.A,
+                       s.lda, idx, idx, D, ldd);
+    LRA_HIP(hipGetLastError());
+    return LRA_OK;
+  }
+  double* C = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_OP1, (size_t)std::max<i64>(s.m * k, 1), &C));
+  LRA_CHECK(lra_src_getcols(c, s, LRA_TRANS_N, idx, k, C, std::max<i64>(s.m, 1)));
+  return lra_launch_gather_rows(c->stream, k, k, C, std::max<i64>(s.m, 1), idx, D, ldd);
 }

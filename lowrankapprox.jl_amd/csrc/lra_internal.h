@@ -77,6 +77,42 @@ struct SpinSection {
 };
 void lra_spin_forget(lra_ctx* c);  // at context destruction
 
+// The matrix a factorization works on: its entries in device memory (A != nullptr), or a
+// matrix-free operator reached through the caller's apply callback (LinearOperator,
+// src/linop.jl:9-40).  herm: the operator is Hermitian (HermitianLinearOperator); for a dense
+// source the drivers test exact symmetry themselves where the reference calls ishermitian.
+struct LraSrc {
+  i64 m = 0, n = 0;
+  const double* A = nullptr;
+  i64 lda = 1;
+  lra_apply_fn apply = nullptr;
+  void* user = nullptr;
+  bool herm = false;
+  bool op() const { return apply != nullptr; }
+};
+inline LraSrc lra_src_dense(i64 m, i64 n, const double* A, i64 lda) {
+  LraSrc s;
+  s.m = m; s.n = n; s.A = A; s.lda = lda;
+  return s;
+}
+inline LraSrc lra_src_op(i64 m, i64 n, lra_apply_fn apply, void* user, bool herm) {
+  LraSrc s;
+  s.m = m; s.n = n; s.apply = apply; s.user = user; s.herm = herm;
+  return s;
+}
+// C (P x k) = op(A)[:, idx] for op(A) = A (trans n) or A' (trans c): a gather for a dense
+// source, one application to k one-hot vectors for an operator (getcols, src/util.jl:25-34)
+int lra_src_getcols(lra_ctx* c, const LraSrc& s, int trans, const i64* idx, i64 k, double* C,
+                    i64 ldc);
+// D (k x k) = A[idx, idx] (getindex(L, rows, cols), src/linop.jl:70-74)
+int lra_src_principal(lra_ctx* c, const LraSrc& s, const i64* idx, i64 k, double* D, i64 ldd);
+// idfact(trans, A, opts) on either kind of source (src/id.jl:434-447)
+int lra_idfact_src(lra_ctx* c, int trans, const LraSrc& s, const lra_opts* opts,
+                   lra_id_out* out);
+// the reference's chkopts!(opts, A) for an operator (src/LowRankApprox.jl:142-148): a sketch
+// other than randn is replaced by randn with a warning (verb != 0)
+void lra_opts_for_operator(lra_opts* o);
+
 // named workspace slots (each call path uses fixed slots so they can overlap safely)
 enum WsSlot {
   WS_OMEGA = 0, WS_SKETCH, WS_SKETCH2, WS_SPLITK, WS_JPVT, WS_TAU, WS_SCAL, WS_R,
@@ -84,7 +120,8 @@ enum WsSlot {
   WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
   WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
   WS_PI0, WS_PI1, WS_PI2, WS_QG, WS_PR0, WS_PR1, WS_PR2, WS_PR3, WS_PR4, WS_PR5, WS_PR6,
-  WS_PR7, WS_PR8, WS_PR9, WS_PR10, WS_PR11, WS_PR12, WS_CU0, WS_COUNT
+  WS_PR7, WS_PR8, WS_PR9, WS_PR10, WS_PR11, WS_PR12, WS_CU0, WS_CUR1, WS_OP2, WS_OP3,
+  WS_COUNT
 };
 static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
 
@@ -300,5 +337,10 @@ int lra_launch_transpose(hipStream_t st, i64 m, i64 n, const double* A, i64 lda,
 int lra_launch_copy2d(hipStream_t st, i64 m, i64 n, const double* A, i64 lda, double* out,
                       i64 ldo);
 int lra_launch_split_ids(hipStream_t st, i64 n, i64 k, const i64* jpvt, i64* sk, i64* rd);
+// S (rows x k) = one-hot columns e_idx[j] (idx 1-based)
+int lra_launch_onehot(hipStream_t st, i64 rows, i64 k, const i64* idx, double* S, i64 lds);
+// out (k x n) = A[rows, :] (rows 1-based, k entries)
+int lra_launch_gather_rows(hipStream_t st, i64 k, i64 n, const double* A, i64 lda,
+                           const i64* rows, double* out, i64 ldo);
 // *flag = 1 when the n x n matrix A is exactly symmetric, else 0 (device int)
 int lra_launch_is_symmetric(hipStream_t st, i64 n, const double* A, i64 lda, int* flag);

@@ -54,6 +54,26 @@ __global__ void copy2d_kernel(i64 m, i64 n, const double* A, i64 lda, double* ou
     out[i + j * ldo] = A[i + j * lda];
 }
 
+// S (rows x k) = [e_idx[0], ..., e_idx[k-1]]: the one-hot block a matrix-free operator is
+// applied to for its columns (getindex(L, :, cols), src/linop.jl:54-60)
+__global__ void onehot_kernel(i64 rows, i64 k, const i64* idx, double* S, i64 lds) {
+  const i64 j = blockIdx.y;
+  if (j >= k) return;
+  const i64 hot = idx[j] - 1;
+  double* dst = S + j * lds;
+  for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < rows; i += (i64)gridDim.x * blockDim.x)
+    dst[i] = i == hot ? 1.0 : 0.0;
+}
+
+// out (k x n) = A(rows - 1, :)
+__global__ void gather_rows_kernel(i64 k, i64 n, const double* A, i64 lda, const i64* rows,
+                                   double* out, i64 ldo) {
+  const i64 j = blockIdx.y;
+  if (j >= n) return;
+  for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < k; i += (i64)gridDim.x * blockDim.x)
+    out[i + j * ldo] = A[(rows[i] - 1) + j * lda];
+}
+
 __global__ void split_ids_kernel(i64 n, i64 k, const i64* jpvt, i64* sk, i64* rd) {
   for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
     if (i < k) { if (sk) sk[i] = jpvt[i]; }
@@ -117,6 +137,25 @@ int lra_launch_copy2d(hipStream_t st, i64 m, i64 n, const double* A, i64 lda, do
   }
   dim3 grid((unsigned)std::min<i64>((m + 255) / 256, 64), (unsigned)n);
   hipLaunchKernelGGL(copy2d_kernel, grid, dim3(256), 0, st, m, n, A, lda, out, ldo);
+  LRA_LAUNCH_CHECK();
+  return LRA_OK;
+}
+
+int lra_launch_onehot(hipStream_t st, i64 rows, i64 k, const i64* idx, double* S, i64 lds) {
+  if (rows <= 0 || k <= 0) return LRA_OK;
+  if (k > 65535) { lra_set_error("onehot: k too large"); return LRA_E_UNSUPPORTED; }
+  dim3 grid((unsigned)std::min<i64>((rows + 255) / 256, 64), (unsigned)k);
+  hipLaunchKernelGGL(onehot_kernel, grid, dim3(256), 0, st, rows, k, idx, S, lds);
+  LRA_LAUNCH_CHECK();
+  return LRA_OK;
+}
+
+int lra_launch_gather_rows(hipStream_t st, i64 k, i64 n, const double* A, i64 lda,
+                           const i64* rows, double* out, i64 ldo) {
+  if (k <= 0 || n <= 0) return LRA_OK;
+  if (n > 65535) { lra_set_error("gather_rows: n too large"); return LRA_E_UNSUPPORTED; }
+  dim3 grid((unsigned)std::min<i64>((k + 255) / 256, 64), (unsigned)n);
+  hipLaunchKernelGGL(gather_rows_kernel, grid, dim3(256), 0, st, k, n, A, lda, rows, out, ldo);
   LRA_LAUNCH_CHECK();
   return LRA_OK;
 }

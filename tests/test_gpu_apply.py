@@ -29,9 +29,11 @@ def test_idpackedv_and_id_apply(gpu, orc):
     assert np.allclose(host(F.apply(X)), Cd @ (Vd @ X), atol=1e-11)
     Z = rng.standard_normal((180, 4))
     assert np.allclose(host(F.apply(Z, "c")), Vd.T @ (Cd.T @ Z), atol=1e-11)
-    z = Z[:, 0].copy()  # a vector acts as a one-column matrix
-    assert np.allclose(host(F.apply(z, "c"))[:, 0], Vd.T @ (Cd.T @ z), atol=1e-11)
-    assert np.allclose(host(F.apply(X[:, 0].copy()))[:, 0], Cd @ (Vd @ X[:, 0]), atol=1e-11)
+    z = Z[:, 0].copy()  # a vector input gives a vector
+    assert host(F.apply(z, "c")).shape == (n,)
+    assert np.allclose(host(F.apply(z, "c")), Vd.T @ (Cd.T @ z), atol=1e-11)
+    assert np.allclose(host(F.apply(X[:, 0].copy())), Cd @ (Vd @ X[:, 0]), atol=1e-11)
+    assert np.allclose(host(V.apply(X[:, 1].copy())), Vd @ X[:, 1], atol=1e-12)
 
 
 def test_partial_factor_apply(gpu, orc):
@@ -91,7 +93,8 @@ def test_partial_qr_arithmetic(gpu, trans):
     assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max(), np.abs(x - xr).max()
     assert np.abs(M.T @ (M @ x - B)).max() <= 1e-8 * np.abs(M).max() * np.abs(B).max()
     b = rng.standard_normal(P)
-    assert np.allclose(host(F.ldiv(b))[:, 0], np.linalg.lstsq(M, b, rcond=1e-9)[0],
+    assert host(F.ldiv(b)).ndim == 1
+    assert np.allclose(host(F.ldiv(b)), np.linalg.lstsq(M, b, rcond=1e-9)[0],
                        atol=1e-8 * np.abs(xr).max())
 
 
@@ -116,8 +119,8 @@ def test_psvd_pheig_cur_arithmetic(gpu, orc):
     assert np.allclose(x, Vt.T @ ((U.T @ B) / S[:, None]), rtol=1e-10,
                        atol=1e-12 * np.abs(x).max())
     bv = rng.standard_normal(m)  # vectors act as one-column matrices
-    assert host(Fs.apply(rng.standard_normal(n))).shape == (m, 1)
-    assert np.allclose(host(Fs.ldiv(bv))[:, 0], Vt.T @ ((U.T @ bv) / S), rtol=1e-10,
+    assert host(Fs.apply(rng.standard_normal(n))).shape == (m,)
+    assert np.allclose(host(Fs.ldiv(bv)), Vt.T @ ((U.T @ bv) / S), rtol=1e-10,
                        atol=1e-12 * np.abs(x).max())
     # the minimum-norm least-squares solution of the factored matrix
     assert np.allclose(x, np.linalg.lstsq(Md, B, rcond=S[-1] / S[0] / 2)[0],
@@ -134,7 +137,7 @@ def test_psvd_pheig_cur_arithmetic(gpu, orc):
     assert np.allclose(host(Fe.rapply(Ze)), Ze @ Me, atol=1e-11 * sc)
     Be = rng.standard_normal((150, 2))
     ve = rng.standard_normal(150)
-    assert np.allclose(host(Fe.apply(ve))[:, 0], Me @ ve, atol=1e-11 * sc)
+    assert np.allclose(host(Fe.apply(ve)), Me @ ve, atol=1e-11 * sc)
     xe = host(Fe.ldiv(Be))
     assert np.allclose(xe, Ve @ ((Ve.T @ Be) / w[:, None]), rtol=1e-10,
                        atol=1e-12 * np.abs(xe).max())
@@ -148,3 +151,34 @@ def test_psvd_pheig_cur_arithmetic(gpu, orc):
     assert np.abs(host(F.matrix()) - Mc).max() <= tol
     assert np.abs(host(F.rapply(Z)) - Z @ Mc).max() <= tol * np.abs(Z).sum(1).max()
     assert np.abs(host(F.rapply(W, "c")) - W @ Mc.T).max() <= tol * np.abs(W).sum(1).max()
+
+
+def test_id_right_products(gpu, orc):
+    """The right products of IDPackedV and ID (src/id.jl:154-200, 302-330), i.e. the
+    B*V, B*V', B*A and B*A' identities of test/id.jl:34-57, and Matrix(V), Matrix(A), against
+    numpy on the same factors."""
+    rng = np.random.default_rng(11)
+    A = orc.fourier_real(rng.random(170), rng.random(130))
+    m, n = A.shape
+    V = gpu.idfact(A, seed=2, rtol=1e-10)
+    sk, rd, T = host(V.sk), host(V.rd), host(V.T)
+    k = V.k
+    Vd = np.zeros((k, n))
+    Vd[:, sk - 1] = np.eye(k)
+    Vd[:, rd - 1] = T
+    assert np.array_equal(host(V.matrix()), Vd)
+    Bk = rng.standard_normal((5, k))
+    Bn = rng.standard_normal((4, n))
+    assert np.allclose(host(V.rapply(Bk)), Bk @ Vd, atol=1e-12)        # B * V
+    assert np.allclose(host(V.rapply(Bn, "c")), Bn @ Vd.T, atol=1e-12)  # B * V'
+    F = gpu.ID.from_matrix(A, V)
+    Ad = A[:, sk - 1] @ Vd
+    assert np.allclose(host(F.matrix()), Ad, atol=1e-12)
+    Bm = rng.standard_normal((3, m))
+    assert np.allclose(host(F.rapply(Bm)), Bm @ Ad, atol=1e-11)        # B * A
+    assert np.allclose(host(F.rapply(Bn, "c")), Bn @ Ad.T, atol=1e-11)  # B * A'
+    x = rng.standard_normal(m)
+    y = host(F.rapply(x))
+    assert y.shape == (n,) and np.allclose(y, x @ Ad, atol=1e-11)
+    # the reference's bound on the approximation (test/id.jl:34-57 with rtol = 1e-10)
+    assert np.linalg.norm(A - Ad, 2) <= 1e-8 * np.linalg.norm(A, 2)

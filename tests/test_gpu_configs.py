@@ -1,0 +1,209 @@
+"""Parity on the benchmark configurations at their full sizes: the HIP path and the oracle
+on the same matrices with the same injected random draws (src/sketch.jl:223-240,
+src/pqr.jl:361-418, src/psvd.jl:238-272).
+
+  8192^2, sigma_i = 10^(-12(i-1)/511), rtol 1e-10: idfact (FP64 and the low-precision sketch
+      forms) and psvdfact against the oracle and the known singular values;
+  65536 x 4096, sigma_i = exp(-i/40), rank 300 at order 310 (sketchfact_adap = false):
+      Gaussian idfact against the oracle; the SRFT sketch against numpy.fft on sampled
+      columns, and the GPU's SRFT sketch through the oracle's geqp3_adap against idfact;
+  a Poisson stream of idfact / psvdfact requests (sizes log-uniform to 8192, decay rates
+      down to 0.01, rtol 1e-6 / 1e-10) through the server, each result against the known
+      singular values.
+
+Rules: k equal; pivots equal before the first step whose oracle norm gap is below 1e-10 of
+the largest column norm; ID errors (spectral norm, the library's 64-step power iteration)
+within 10x of the oracle's."""
+import numpy as np
+import pytest
+
+from test_gpu_idfact import pivots_match
+from workloads import DeviceRand, config2, config3, config5_requests, decay_matrix
+
+# full-size matrices: the oracle's CPU legs take seconds each
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def spec_err(lra, Ad, V):
+    """|A - A[:, sk] [I T] P'|_2 by the library's power iteration (64 steps)."""
+    return lra.snormdiff(Ad, V, snorm_niter=64, rtol=1e-8)
+
+
+def as_dev_id(lra, Vo):
+    import torch
+    T = torch.as_tensor(np.asfortranarray(Vo.T), device="cuda")
+    return lra.IDPackedV(torch.as_tensor(Vo.sk, device="cuda"),
+                         torch.as_tensor(Vo.rd, device="cuda"), T.t().contiguous().t())
+
+
+@pytest.fixture(scope="module")
+def c2(gpu, orc):
+    import torch
+    print("config 2 matrix 8192x8192", flush=True)
+    Ad, sig = config2(torch)
+    A = np.asfortranarray(host(Ad))
+    print("config 2 oracle idfact", flush=True)
+    Vo = orc.idfact("n", A, orc.LRAOptions().copy(rtol=1e-10), rand=DeviceRand(gpu, 11))
+    e_o = spec_err(gpu, Ad, as_dev_id(gpu, Vo))
+    yield Ad, A, sig, Vo, e_o
+    del Ad, A
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3", "bf16"])
+def test_config2_idfact_vs_oracle(gpu, c2, prec):
+    Ad, A, sig, Vo, e_o = c2
+    print(f"config 2 idfact precision={prec}", flush=True)
+    V = gpu.idfact(Ad, seed=11, rtol=1e-10, precision=prec)
+    assert Vo.k == 436 and Vo.rounds == 5
+    assert V.k == Vo.k and V.rounds == Vo.rounds
+    assert pivots_match(V, Vo) >= 100
+    e_g = spec_err(gpu, Ad, V)
+    print(f"  k={V.k} err={e_g:.3e} oracle err={e_o:.3e} (sigma_1 = 1)", flush=True)
+    assert e_g <= 10 * e_o
+    # the optimal rank-k error is sigma_{k+1}; the ID is within a modest factor of it
+    assert e_o <= 1e3 * sig[Vo.k]
+
+
+def test_config2_psvdfact_vs_oracle(gpu, orc, c2):
+    Ad, A, sig, Vo, e_o = c2
+    print("config 2 psvdfact", flush=True)
+    F = gpu.psvdfact(Ad, seed=11, rtol=1e-10)
+    Fo = orc.psvdfact(A, rand=DeviceRand(gpu, 11), rtol=1e-10)
+    S = host(F.S)
+    assert len(S) == len(Fo.S)
+    # both factor the same ID up to the noise-floor pivots: singular values agree to the
+    # ID's error, and match the known spectrum to it
+    assert np.abs(S - Fo.S).max() <= 10 * e_o
+    assert np.abs(S - sig[: len(S)]).max() <= 10 * e_o
+    U, Vt = F.U, F.Vt
+    k = len(S)
+    import torch
+    I = torch.eye(k, dtype=torch.float64, device="cuda")
+    assert float((U.t() @ U - I).abs().max()) <= 1e-10
+    assert float((Vt @ Vt.t() - I).abs().max()) <= 1e-10
+    e = gpu.snormdiff(Ad, F.matrix(), snorm_niter=64, rtol=1e-8)
+    assert e <= 10 * e_o + (10 * sig[k] if k < len(sig) else 0.0)
+
+
+@pytest.fixture(scope="module")
+def c3(gpu):
+    import torch
+    print("config 3 matrix 65536x4096", flush=True)
+    Ad, sig = config3(torch)
+    A = np.asfortranarray(host(Ad))
+    yield Ad, A, sig
+    del Ad, A
+    torch.cuda.empty_cache()
+
+
+C3_OPTS = dict(rank=300, sketchfact_adap=False)
+
+
+def test_config3_gaussian_vs_oracle(gpu, orc, c3):
+    Ad, A, sig = c3
+    print("config 3 Gaussian idfact, rank 300 at order 310", flush=True)
+    V = gpu.idfact(Ad, seed=12, sketchfact_randn_samp=(1, 10), **C3_OPTS)
+    Vo = orc.idfact("n", A, orc.LRAOptions().copy(sketchfact_randn_samp=(1, 10), **C3_OPTS),
+                    rand=DeviceRand(gpu, 12))
+    assert V.k == Vo.k == 300
+    assert pivots_match(V, Vo) >= 50
+    e_g, e_o = spec_err(gpu, Ad, V), spec_err(gpu, Ad, as_dev_id(gpu, Vo))
+    print(f"  err={e_g:.3e} oracle err={e_o:.3e} sigma_301={sig[300]:.3e}", flush=True)
+    assert e_g <= 10 * e_o
+    assert e_o <= 1e3 * sig[300]
+
+
+def test_config3_srft_vs_fft_and_oracle_qrcp(gpu, orc, c3):
+    import torch
+    Ad, A, sig = c3
+    m, n, ell = A.shape[0], A.shape[1], 310
+    print("config 3 SRFT idfact, rank 300 at order 310", flush=True)
+    opts = dict(sketch="srft", sketchfact_srft_samp=(1, 10), seed=13, **C3_OPTS)
+    V = gpu.idfact(Ad, **opts)
+    # the sketch idfact's single round draws (Philox stream 0), against numpy.fft
+    B = gpu.sketch("left", "n", Ad, ell, sketch="srft", seed=13, stream=0)
+    d, idx = (host(t) for t in gpu.sketch_draw("srft", m, ell, 13, 0))
+    Bh = host(B)
+    cols = np.arange(0, n, 257)
+    F = np.fft.fft(d[:, None] * A[:, cols], axis=0)
+    scale = np.sqrt(m) * np.abs(A[:, cols]).max()
+    for i in range(0, ell, 2):
+        f = idx[i] - 1
+        assert np.abs(Bh[i, cols] - F[f].real).max() <= 1e-11 * scale
+        if i + 1 < ell:
+            assert np.abs(Bh[i + 1, cols] - F[f].imag).max() <= 1e-11 * scale
+    # that sketch through the oracle's rank-adaptive QRCP and T = R11 \ R12
+    Bf, p, tau, k, gap = orc.geqp3_adap(np.asfortranarray(Bh), 300, 0.0, 5 * np.finfo(float).eps,
+                                        32, want_gap=True)
+    assert V.k == k == 300
+    g = host(V.sk)
+    for i in range(k):
+        if g[i] != p[i]:
+            assert gap[i] < 1e-10, (i, gap[i])
+            break
+    else:
+        To = orc.maxdet_t(np.triu(Bf[:k, :]))
+        T = host(V.T)
+        assert np.abs(T - To).max() <= 1e-8 * max(1.0, np.abs(To).max())
+    e_g = spec_err(gpu, Ad, V)
+    print(f"  srft err={e_g:.3e} sigma_301={sig[300]:.3e}", flush=True)
+    assert e_g <= 1e3 * sig[300]
+    del B
+    torch.cuda.empty_cache()
+
+
+def test_config5_poisson_stream_vs_known_spectrum(gpu):
+    """A served stream of requests arriving as a Poisson process (rate 2000/s), the sizes and
+    kinds of the served workload plus one slow-decay request whose rank reaches the pivoted QR
+    in HBM; every result is checked against the matrix's known singular values."""
+    import time
+    import torch
+    reqs = config5_requests(24)
+    reqs.append(dict(m=4096, n=4096, kind="idfact", rtol=1e-10, decay=0.01, seed=999))
+    reqs.append(dict(m=8192, n=3000, kind="psvdfact", rtol=1e-6, decay=0.01, seed=998))
+    mats = []
+    for q in reqs:
+        A, sig = decay_matrix(torch, q["m"], q["n"], q["decay"], q["seed"])
+        mats.append((A, sig))
+    torch.cuda.synchronize()
+    srv = gpu.Server(hbm_budget=96 << 30, workers=4)
+    rng = np.random.default_rng(6)
+    gaps = rng.exponential(1.0 / 2000.0, size=len(reqs))
+    handles = []
+    t0 = time.perf_counter()
+    for i, (q, (A, sig)) in enumerate(zip(reqs, mats)):
+        while time.perf_counter() - t0 < gaps[: i + 1].sum():
+            pass
+        handles.append(srv.submit(A, q["kind"], rtol=q["rtol"], seed=q["seed"]))
+    print(f"config 5 stream: {len(reqs)} requests submitted", flush=True)
+    try:
+        for h, q, (A, sig) in zip(handles, reqs, mats):
+            assert srv.wait(h, 300.0) == "done", q
+            R = srv.result(h)
+            rt = q["rtol"]
+            # the rank the spectrum shows at rtol: sigma_i > rtol sigma_1
+            kt = int((sig > rt * sig[0]).sum())
+            if q["kind"] == "psvdfact":
+                # psvdrank cuts where the computed sigma_i drop below rtol sigma_1
+                S = host(R.S)
+                assert abs(len(S) - kt) <= 3, (q, len(S), kt)
+                kk = min(len(S), len(sig))
+                assert np.abs(S[:kk] - sig[:kk]).max() <= 100 * rt * sig[0], q
+            else:
+                # the pivoted QR's diagonal follows sigma_i within a modest factor, which
+                # moves the stopping index by a fraction of kt on these slow decays; the error
+                # bound is what the rank is chosen for
+                assert 0.5 * kt - 3 <= R.k <= 1.5 * kt + 10, (q, R.k, kt)
+                e = gpu.snormdiff(A, R, snorm_niter=64, rtol=1e-8)
+                assert e <= 1e3 * rt * sig[0], (q, e)
+                assert e >= 0.5 * (sig[R.k] if R.k < len(sig) else 0.0), (q, e)
+            srv.release(h)
+            print(f"  {q['kind']} {q['m']}x{q['n']} decay={q['decay']:.3f} rtol={rt:g} ok",
+                  flush=True)
+    finally:
+        srv.close()

@@ -43,11 +43,17 @@ def test_idfact_linop_vs_oracle(gpu, orc, trans):
     assert e_g <= 10 * max(e_o, 1e-15 * np.linalg.norm(A, 2))
 
 
-def test_idfact_linop_rejects_structured_sketch(gpu):
-    A = np.eye(50)
+def test_idfact_linop_structured_sketch_uses_randn(gpu):
+    """chkopts!(opts, L) (src/LowRankApprox.jl:142-148): a structured sketch on an operator
+    warns and runs the Gaussian sketch."""
+    A = np.asfortranarray(np.random.default_rng(3).standard_normal((60, 50)))
     L = gpu.LinearOperator.from_matrix(A)
-    with pytest.raises(gpu.LraError):
-        L.idfact("n", sketch="srft")
+    with pytest.warns(UserWarning, match="randn"):
+        V = L.idfact("n", sketch="srft", seed=1, rtol=1e-12)
+    W = L.idfact("n", sketch="randn", seed=1, rtol=1e-12)
+    assert V.k == W.k
+    assert np.array_equal(V.sk.cpu().numpy(), W.sk.cpu().numpy())
+    assert np.array_equal(V.T.cpu().numpy(), W.T.cpu().numpy())
 
 
 def test_linop_failing_operator_reports_status(gpu):
