@@ -137,7 +137,10 @@ def kernel_roofline(lra, torch, A, reps=5):
     q = qflops / (ms_q * 1e-3) / 1e12
     g = gflops / (ms_g * 1e-3) / 1e12
     return ({"kernel": f"qrcp_kernel (pivoted QR of the {ell}x{n} sketch, {k} steps)",
-             "bound": "mfma", "achieved": round(q, 3), "peak": PEAK_FP64_TFLOPS,
+             "bound": "mfma", "limiter": "latency: FP64 vector ALU work and one grid-wide "
+             "exchange per pivot, no MFMA; priced against the FP64 peak (78.6 TFLOP/s, shared "
+             "by the vector ALUs and MFMA)",
+             "achieved": round(q, 3), "peak": PEAK_FP64_TFLOPS,
              "unit": "TFLOP/s", "frac": round(q / PEAK_FP64_TFLOPS, 5),
              "traffic": TRAFFIC_QRCP_FINAL, "traffic_source": TRAFFIC_SOURCE,
              "ms_per_launch": round(ms_q, 4), "flops_per_launch": qflops,
@@ -428,24 +431,114 @@ def summarize_trace(path: str, names=("psvdfact", "pqrfact")):
             print(f"| `{k}` | {v[0]} | {v[1] / 1e3:.1f} | {v[1] / 1e3 / v[0]:.1f} |")
 
 
-def cpu_baseline(A_host, seed_rounds, threads, lra_omega):
-    """The oracle (CPU restatement of the reference: numpy/OpenBLAS dgemm + OpenBLAS dlaqps
-    through geqp3_adap_main!'s loop + dtrsm) on one ID of the same matrix with the same
-    injected Omegas."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
+    return oracle
+
+
+def host_info():
+    """CPU model, nproc and the thread budget of this process (OPENBLAS_NUM_THREADS, else
+    OMP_NUM_THREADS, else every core)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    env = os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS")
+    threads = min(nproc, int(env)) if env and env.isdigit() else nproc
+    return model, nproc, threads
+
+
+def cpu_idfact_times(A_host, omega, threads, warm, reps, **kw):
+    """Wall times of the oracle's idfact (CPU restatement of the reference: numpy/OpenBLAS
+    dgemm, OpenBLAS dlaqps in geqp3_adap_main!'s loop, OpenBLAS dtrsm for T) on the same matrix
+    with the same injected Omegas, `warm` untimed runs then `reps` timed ones."""
+    oracle = _oracle()
     oracle.openblas_set_threads(threads)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
 
     class Om:
         def gaussian(self, rnd, rows, cols):
-            return lra_omega[(rnd, rows, cols)]
+            return omega[(rnd, rows, cols)]
 
-    t0 = time.perf_counter()
-    V = oracle.idfact("n", A_host, rand=Om(), use_openblas=True, rtol=1e-10)
-    dt = time.perf_counter() - t0
-    return dt, V
+    for _ in range(warm):
+        oracle.idfact("n", A_host, rand=Om(), use_openblas=True, **kw)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        V = oracle.idfact("n", A_host, rand=Om(), use_openblas=True, **kw)
+        ts.append(time.perf_counter() - t0)
+    return np.array(ts), V
+
+
+def device_omegas(lra, n, seed=0, rounds=8):
+    """The Gaussian sketches of rounds 0..rounds-1 the GPU draws for an n-column idfact."""
+    om = {}
+    target = 32
+    for rnd in range(rounds):
+        ell = target + 8
+        om[(rnd, ell, n)] = np.asfortranarray(lra.gaussian(ell, n, seed, rnd).cpu().numpy())
+        target *= 2
+    return om
+
+
+def bench_config1(lra, torch, steps: int = 50):
+    """The reference's published case (README.md:27-31, ~0.02 s): idfact of the Hilbert
+    matrix A[i,j] = 1/(i+j-1), n = 1024, at the default rtol (5 eps, k = 26-27) and with
+    rank = 20; GPU latency beside the oracle on this host's cores (all, and one thread)."""
+    oracle = _oracle()
+    A_host = oracle.hilbert(1024)
+    A = lra.to_device(A_host)
+    model, nproc, threads = host_info()
+    out = {"workload": "idfact Hilbert 1024x1024 f64 (README.md:27-31)",
+           "reference_readme_s": 0.02}
+    for name, kw in (("default_rtol", {}), ("rank20", {"rank": 20})):
+        for w in range(5):
+            lra.idfact(A, seed=w, **kw)
+        torch.cuda.synchronize()
+        lat = []
+        for s in range(steps):
+            t0 = time.perf_counter()
+            V = lra.idfact(A, seed=s, **kw)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        om = device_omegas(lra, 1024, seed=0, rounds=3)
+        cpu = {}
+        for th in sorted({threads, 1}, reverse=True):
+            ts, Vc = cpu_idfact_times(A_host, om, th, 3, 10, **kw)
+            cpu[f"threads_{th}"] = {"median_s": float(np.median(ts)), "k": Vc.k}
+        out[name] = {"gpu_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                     "gpu_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                     "gpu_ids_per_s": round(steps / lat.sum(), 1), "k": V.k,
+                     "speedup_vs_readme": round(0.02 / float(np.percentile(lat, 50)), 1),
+                     "cpu_oracle": cpu,
+                     "cpu_note": "3 warm-ups, median of 10; oracle = the reference's loop "
+                                 "over OpenBLAS dgemm/dlaqps/dtrsm, same injected sketches"}
+        print(f"# config 1 {name}: {out[name]}", file=sys.stderr, flush=True)
+    out["host"] = {"cpu_model": model, "nproc": nproc}
+    return out
+
+
+def bench_psvd(lra, torch, A, steps: int = 5):
+    """psvdfact of the throughput workload matrix at rtol = 1e-10 (src/psvd.jl:238-272)."""
+    lra.psvdfact(A, rtol=1e-10, seed=100)
+    torch.cuda.synchronize()
+    lat = []
+    for s in range(steps):
+        t0 = time.perf_counter()
+        F = lra.psvdfact(A, rtol=1e-10, seed=s)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    return {"value": round(steps / lat.sum(), 2), "unit": "psvdfacts/s",
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3), "k": F.k}
 
 
 def main():
@@ -460,6 +553,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-srft", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch, for rocprofv3 counter runs: a 1 GiB device copy "
                          "(calibration), the final-round pivoted QR of the 8192^2 workload, "
@@ -514,6 +608,9 @@ def main():
                             "is about two thirds of the time")
     err64 = id_error(lra, A, V64)
     roof, roof_gemm = kernel_roofline(lra, torch, A)
+    psvd = bench_psvd(lra, torch, A)
+    print(f"# psvdfact: {psvd}", file=sys.stderr, flush=True)
+    config1 = None if args.no_config1 else bench_config1(lra, torch)
 
     batched = None
     if not args.no_batched:
@@ -528,24 +625,22 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        # the same Omegas the GPU used for seed 0
-        om = {}
+        model, nproc, threads = host_info()
         n = args.n
-        target, rnd = 32, 0
-        while rnd < 8:
-            ell = target + 8
-            om[(rnd, ell, n)] = np.asfortranarray(lra.gaussian(ell, n, 0, rnd).cpu().numpy())
-            target *= 2
-            rnd += 1
+        om = device_omegas(lra, n)  # the Omegas the GPU draws for seed 0
         A_host = np.asfortranarray(A.cpu().numpy())
-        dt, Vc = cpu_baseline(A_host, None, threads, om)
+        ts, Vc = cpu_idfact_times(A_host, om, threads, 1, 3, rtol=1e-10)
+        dt = float(np.median(ts))
         cpu = {"value": round(1.0 / dt, 4), "unit": "IDs/s", "cores": threads, "kind": "port",
-               "sample": f"1 idfact of the same {n}x{n} matrix, rtol=1e-10, same injected "
-                         f"Gaussian sketches (k={Vc.k}, rounds={Vc.rounds}); numpy/OpenBLAS "
-                         f"dgemm + OpenBLAS dlaqps in the restated geqp3_adap loop, "
-                         f"OPENBLAS threads={threads}"}
-        print(f"# cpu: {dt:.2f} s per ID", file=sys.stderr, flush=True)
+               "sample": f"idfact of the same {n}x{n} matrix, rtol=1e-10, same injected "
+                         f"Gaussian sketches (k={Vc.k}, rounds={Vc.rounds}): 1 warm-up, median "
+                         f"of 3 ({', '.join(f'{t:.2f}' for t in ts)} s); numpy/OpenBLAS dgemm, "
+                         f"OpenBLAS dlaqps in the restated geqp3_adap loop, OpenBLAS dtrsm for "
+                         f"T; OPENBLAS threads={threads}",
+               "cpu_model": model, "nproc": nproc,
+               "one_thread": "see config1.*.cpu_oracle.threads_1 (the 8192^2 ID takes minutes "
+                             "on one thread)"}
+        print(f"# cpu: {dt:.2f} s per ID (median of {len(ts)})", file=sys.stderr, flush=True)
 
     out = {
         "metric": "IDs/sec + p50/p99 latency at rtol=1e-10, mixed-prec vs plain FP64, 1 MI355X",
@@ -570,6 +665,8 @@ def main():
         "mixed_precision_variants": mixed[1:],
         "roofline": roof,
         "roofline_sketch_gemm": roof_gemm,
+        "psvdfact": psvd,
+        "config1": config1,
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

@@ -236,6 +236,24 @@ def maxdet_t(R: np.ndarray) -> np.ndarray:
     return T
 
 
+def maxdet_t_blas(R: np.ndarray) -> np.ndarray:
+    """T = R[:,1:k] \\ R[:,k+1:n] through OpenBLAS dtrsm (the BLAS call behind
+    ldiv!(UpperTriangular(R1), R2) at src/pqr.jl:441; called only): the CPU-baseline form of
+    maxdet_t."""
+    R = np.asfortranarray(R, dtype=np.float64)
+    k, n = R.shape
+    T = np.array(R[:, k:], dtype=np.float64, order="F", copy=True)
+    if k > 0 and n > k:
+        ob = openblas()
+        one = C.c_double(1.0)
+        side, uplo, tr, diag = (C.c_char(x) for x in (b"L", b"U", b"N", b"N"))
+        kk, nn, ld = i64(k), i64(n - k), i64(max(1, k))
+        ob.scipy_dtrsm_64_(C.byref(side), C.byref(uplo), C.byref(tr), C.byref(diag),
+                           C.byref(kk), C.byref(nn), C.byref(one), _pd(R), C.byref(ld), _pd(T),
+                           C.byref(ld), i64(1), i64(1), i64(1), i64(1))
+    return T
+
+
 def maxdet_swapcols(p: np.ndarray, T: np.ndarray, tol: float, niter: int, R1=None):
     """maxdet_swapcols!/maxdet_update! (src/pqr.jl:445-501) on T/p (and R1 if given)."""
     T = np.array(T, dtype=np.float64, order="F", copy=True)
@@ -377,8 +395,8 @@ class PQRFactors:
     gap: Optional[np.ndarray] = None  # relative pivot gaps of the accepted QRCP
 
 
-def pqrback_postproc(B, p, tau, k, opts: LRAOptions) -> PQRFactors:
-    """pqrback_postproc (src/pqr.jl:420-436)."""
+def pqrback_postproc(B, p, tau, k, opts: LRAOptions, use_openblas: bool = False) -> PQRFactors:
+    """pqrback_postproc (src/pqr.jl:420-436); use_openblas solves T with OpenBLAS dtrsm."""
     retq = "q" in opts.pqrfact_retval
     retr = "r" in opts.pqrfact_retval
     rett = "t" in opts.pqrfact_retval
@@ -388,7 +406,7 @@ def pqrback_postproc(B, p, tau, k, opts: LRAOptions) -> PQRFactors:
     if retq:
         Q = _orgqr(B[:, :k], tau[:k]) if k > 0 else np.zeros((B.shape[0], 0))
     R = np.triu(B[:k, :]) if (retr or rett or maxdet) else None
-    T = maxdet_t(R) if (rett or maxdet) else None
+    T = (maxdet_t_blas(R) if use_openblas else maxdet_t(R)) if (rett or maxdet) else None
     p = np.array(p, dtype=np.int64)
     if maxdet:
         p, T, R1, it = maxdet_swapcols(p, T, opts.maxdet_tol, opts.maxdet_niter,
@@ -448,7 +466,7 @@ def sketchfact(side: str, trans: str, A, opts: LRAOptions, rand: Rand,
                 Bf, p, tau, k, gap = geqp3_adap(B, opts_.rank, opts_.atol, opts_.rtol,
                                                 opts_.nb, want_gap=True)
             if k < n:
-                F = pqrback_postproc(Bf, p, tau, k, opts)
+                F = pqrback_postproc(Bf, p, tau, k, opts, use_openblas)
                 F.rounds = rnd + 1
                 F.gap = gap
                 return F
@@ -458,7 +476,7 @@ def sketchfact(side: str, trans: str, A, opts: LRAOptions, rand: Rand,
     B = sketch(side, trans, A, order, opts, rand, 0)
     Bf, p, tau, k = geqp3_adap(B, opts.rank, opts.atol, opts.rtol, opts.nb,
                                use_openblas=use_openblas)
-    return pqrback_postproc(Bf, p, tau, k, opts)
+    return pqrback_postproc(Bf, p, tau, k, opts, use_openblas)
 
 
 # ----------------------------------------------------------------------------------------

@@ -114,6 +114,23 @@ def test_maxdet_t_matches_solve_triangular(orc):
     assert np.allclose(T, sla.solve_triangular(R[:, :k], R[:, k:]), rtol=1e-12, atol=1e-12)
 
 
+def test_maxdet_t_blas_matches_restated(orc):
+    """The CPU baseline's OpenBLAS dtrsm T equals the C restatement and scipy's solve."""
+    rng = np.random.default_rng(4)
+    k, n = 45, 310
+    R = np.triu(rng.standard_normal((k, n)))
+    R[np.arange(k), np.arange(k)] += 4 * np.sign(R[np.arange(k), np.arange(k)])
+    T = orc.maxdet_t_blas(R)
+    assert T.shape == (k, n - k)
+    assert np.allclose(T, orc.maxdet_t(R), rtol=1e-12, atol=1e-12)
+    assert np.allclose(T, sla.solve_triangular(R[:, :k], R[:, k:]), rtol=1e-12, atol=1e-12)
+    # idfact with the OpenBLAS legs gives the restated idfact's factorization
+    A = orc.hilbert(300)
+    V1 = orc.idfact("n", A, rand=orc.Rand(5), rtol=1e-12)
+    V2 = orc.idfact("n", A, rand=orc.Rand(5), rtol=1e-12, use_openblas=True)
+    assert V1.k == V2.k and np.array_equal(V1.sk[:10], V2.sk[:10])
+
+
 def test_maxdet_swaps_bound_T_and_keep_the_id(orc):
     rng = np.random.default_rng(11)
     A = rng.standard_normal((80, 12)) @ rng.standard_normal((12, 150))
