@@ -233,6 +233,19 @@ int lra_snormdiff_id(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_
                      int64_t k, const int64_t* sk, const int64_t* rd, const double* T,
                      int64_t ldt, const lra_opts* opts, const double* x0, double* result);
 
+/* Allocation-free lra_snormdiff_id, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and result as lra_snormdiff_id; its vectors
+ * and partial sums live in a fixed 32 MiB device array per GPU (held until the process
+ * ends), so it needs n + m + k + 64 + chunks*m <= 4 Mi doubles (chunks <= 4 * CUs * 256 / m
+ * + 1; every shape up to about 10^6 rows fits) and returns LRA_E_UNSUPPORTED beyond.  It
+ * never calls hipMalloc/hipFree.  Calls on one GPU share that array: calls that overlap on
+ * one GPU (from different contexts or threads) race. */
+int lra_snormdiff_id_noalloc(lra_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t lda,
+                             int64_t k, const int64_t* sk, const int64_t* rd, const double* T,
+                             int64_t ldt, const lra_opts* opts, const double* x0,
+                             double* result);
+
 /* snorm of a matrix-free operator (snorm(L::LinearOperator), src/snorm.jl:14-45 with
  * src/linop.jl:105-113): `apply` is called with nrhs = 1 for L x (trans 0) and L' y
  * (trans 1); isherm != 0 iterates on L itself. */

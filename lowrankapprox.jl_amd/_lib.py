@@ -111,6 +111,9 @@ SIGNATURES = [
                                 C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_snormdiff_id", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, P_I, P_I, P_D, i64,
                                    C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
+    ("lra_snormdiff_id_noalloc", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, P_I, P_I, P_D,
+                                           i64, C.POINTER(lra_opts), P_D,
+                                           C.POINTER(C.c_double)]),
     ("lra_snorm_op", C.c_int, [C.c_void_p, i64, i64, APPLY_FN, C.c_void_p, C.c_int,
                                C.POINTER(lra_opts), P_D, C.POINTER(C.c_double)]),
     ("lra_geqrf_f64", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D]),

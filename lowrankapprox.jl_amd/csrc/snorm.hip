@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -120,16 +121,23 @@ inline int blocks_for(i64 n) { return (int)std::min<i64>(std::max<i64>((n + 255)
 
 }  // namespace
 
-// y = alpha * A[:, cols] x + beta * y  (cols optional, 1-based)
-int lra_gemv_n(lra_ctx* c, i64 m, i64 n, const double* A, i64 lda, const i64* cols,
-               const double* x, double alpha, double* y, double beta) {
-  if (m <= 0) return LRA_OK;
+// column chunks of gemv_n for an m x n product (the partial sums are nchunks x m)
+static i64 gemv_chunks(const lra_ctx* c, i64 m, i64 n, i64* chunk) {
   const i64 rt = (m + GV_ROWS - 1) / GV_ROWS;
   i64 nchunks = std::max<i64>(1, std::min<i64>((n + 255) / 256, (4ll * c->num_cu + rt - 1) / rt));
-  const i64 chunk = (n + nchunks - 1) / std::max<i64>(nchunks, 1);
-  nchunks = n > 0 ? (n + chunk - 1) / chunk : 1;
-  double* part = nullptr;
-  LRA_CHECK(lra_wsT(c, WS_X0, (size_t)nchunks * m, &part));
+  *chunk = (n + nchunks - 1) / std::max<i64>(nchunks, 1);
+  return n > 0 ? (n + *chunk - 1) / *chunk : 1;
+}
+
+// y = alpha * A[:, cols] x + beta * y  (cols optional, 1-based).  The partial sums go to
+// `part` (at least gemv_chunks(...) * m doubles) when given, else to a workspace slot.
+int lra_gemv_n(lra_ctx* c, i64 m, i64 n, const double* A, i64 lda, const i64* cols,
+               const double* x, double alpha, double* y, double beta, double* part = nullptr) {
+  if (m <= 0) return LRA_OK;
+  const i64 rt = (m + GV_ROWS - 1) / GV_ROWS;
+  i64 chunk = 0;
+  const i64 nchunks = gemv_chunks(c, m, n, &chunk);
+  if (!part) LRA_CHECK(lra_wsT(c, WS_X0, (size_t)nchunks * m, &part));
   if (n > 0) {
     hipLaunchKernelGGL(gemv_n_kernel, dim3((unsigned)rt, (unsigned)nchunks), dim3(256), 0, c->stream,
                        m, n, A, lda, cols, x, chunk, part);
@@ -153,7 +161,8 @@ int lra_gemv_t(lra_ctx* c, i64 m, i64 n, const double* A, i64 lda, const i64* co
 namespace {
 
 struct Op {
-  // (A - F) where F is an ID of A (k > 0) or nothing (k < 0)
+  // (A - F) where F is an ID of A (k > 0) or nothing (k < 0); `part`: caller-provided
+  // partial-sum buffer of the GEMVs (else a workspace slot)
   i64 m, n;
   const double* A;
   i64 lda;
@@ -166,6 +175,7 @@ struct Op {
   i64 ldb;
   lra_apply_fn apply = nullptr;  // matrix-free operator (replaces A when set)
   void* user = nullptr;
+  double* part = nullptr;
 };
 
 int op_apply(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) {
@@ -175,14 +185,14 @@ int op_apply(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) {
     if (st != 0) { lra_set_error("operator apply returned %d", st); return LRA_E_STATE; }
     return LRA_OK;
   }
-  LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.A, o.lda, nullptr, x, 1.0, y, 0.0));
-  if (o.B) LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.B, o.ldb, nullptr, x, -1.0, y, 1.0));
+  LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.A, o.lda, nullptr, x, 1.0, y, 0.0, o.part));
+  if (o.B) LRA_CHECK(lra_gemv_n(c, o.m, o.n, o.B, o.ldb, nullptr, x, -1.0, y, 1.0, o.part));
   if (o.k > 0) {
     const i64 nr = o.n - o.k;
     hipLaunchKernelGGL(id_apply_n_kernel, dim3((unsigned)((o.k + 3) / 4)), dim3(256), 0, c->stream,
                        o.k, nr, o.T, o.ldt, o.sk, o.rd, x, wk);
     LRA_HIP(hipGetLastError());
-    LRA_CHECK(lra_gemv_n(c, o.m, o.k, o.A, o.lda, o.sk, wk, -1.0, y, 1.0));
+    LRA_CHECK(lra_gemv_n(c, o.m, o.k, o.A, o.lda, o.sk, wk, -1.0, y, 1.0, o.part));
   }
   return LRA_OK;
 }
@@ -206,14 +216,24 @@ int op_apply_t(lra_ctx* c, const Op& o, const double* x, double* y, double* wk) 
   return LRA_OK;
 }
 
-// power iteration loop of snorm (src/snorm.jl:14-45)
+// Vectors of the power iteration: xn (n), xm (m), wk (max(k,1) + 64; its tail holds the
+// squared norm).
+struct PowerBufs {
+  double *xn, *xm, *wk;
+};
+
+// power iteration loop of snorm (src/snorm.jl:14-45); buffers from the workspace unless given
 int power_iteration(lra_ctx* c, const Op& o, int isherm, const lra_opts* opts, const double* x0,
-                    double* result) {
+                    double* result, const PowerBufs* given = nullptr) {
   const i64 m = o.m, n = o.n;
   double *xn = nullptr, *xm = nullptr, *wk = nullptr, *scal = nullptr;
-  LRA_CHECK(lra_wsT(c, WS_QR0, (size_t)std::max<i64>(n, 1), &xn));
-  LRA_CHECK(lra_wsT(c, WS_QR1, (size_t)std::max<i64>(m, 1), &xm));
-  LRA_CHECK(lra_wsT(c, WS_QR2, (size_t)std::max<i64>(std::max<i64>(o.k, 1), 1) + 64, &wk));
+  if (given) {
+    xn = given->xn; xm = given->xm; wk = given->wk;
+  } else {
+    LRA_CHECK(lra_wsT(c, WS_QR0, (size_t)std::max<i64>(n, 1), &xn));
+    LRA_CHECK(lra_wsT(c, WS_QR1, (size_t)std::max<i64>(m, 1), &xm));
+    LRA_CHECK(lra_wsT(c, WS_QR2, (size_t)std::max<i64>(std::max<i64>(o.k, 1), 1) + 64, &wk));
+  }
   scal = wk + std::max<i64>(o.k, 1);
   if (x0) LRA_HIP(hipMemcpyAsync(xn, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   else LRA_CHECK(lra_launch_gaussian_fill(c->stream, opts->seed, 0x5A0A5A0Aull, n, 1, xn, n));
@@ -308,4 +328,75 @@ extern "C" int lra_snorm_op(lra_ctx* c, int64_t m, int64_t n, lra_apply_fn apply
   o.apply = apply;
   o.user = user;
   return power_iteration(c, o, isherm, opts, x0, result);
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free variant of lra_snormdiff_id.  All of its device memory beyond the caller's
+// inputs is `scratch`, a file-scope device array of kScratchDoubles doubles (32 MiB) that
+// every GPU the library runs on holds until the process ends.  On the host the array's name
+// is only a stand-in: its device address is fetched per device with hipGetSymbolAddress on the
+// first call on that device (with the device current) and kept in g_scratch; only that
+// address is handed to kernels.  Calls on one GPU share its array (the last call's contents
+// stay behind; every element a call reads it has written first, so nothing is cleared), and
+// calls that overlap on one GPU race.
+
+static constexpr size_t kScratchDoubles = (size_t)4 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch[64] = {};
+
+int scratch_on(int device, double** out) {
+  if (device < 0 || device >= 64) {
+    lra_set_error("scratch: device %d out of range", device);
+    return LRA_E_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    LRA_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    LRA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return LRA_OK;
+}
+}  // namespace
+
+extern "C" int lra_snormdiff_id_noalloc(lra_ctx* c, int64_t m, int64_t n, const double* A,
+                                        int64_t lda, int64_t k, const int64_t* sk,
+                                        const int64_t* rd, const double* T, int64_t ldt,
+                                        const lra_opts* opts, const double* x0,
+                                        double* result) {
+  LRA_ARG(c != nullptr, 1);
+  LRA_ARG(m >= 0, 2);
+  LRA_ARG(n >= 0, 3);
+  LRA_ARG(lda >= std::max<int64_t>(1, m), 5);
+  LRA_ARG(k >= 0 && k <= n, 6);
+  LRA_ARG(ldt >= std::max<int64_t>(1, k), 10);
+  LRA_ARG(opts != nullptr, 11);
+  LRA_ARG(result != nullptr, 13);
+  // scratch layout: xn (n) | xm (m) | wk (max(k,1) + 64) | GEMV partial sums (chunks x m)
+  i64 chunk = 0;
+  const i64 nparts = std::max(gemv_chunks(c, m, n, &chunk), gemv_chunks(c, m, k, &chunk));
+  const size_t need = (size_t)std::max<i64>(n, 1) + (size_t)std::max<i64>(m, 1) +
+                      (size_t)std::max<i64>(k, 1) + 64 + (size_t)nparts * (size_t)m;
+  if (need > kScratchDoubles) {
+    lra_set_error("snormdiff_id_noalloc: %zu doubles of scratch needed for m=%lld n=%lld "
+                  "k=%lld, the maximum is %zu (use lra_snormdiff_id)", need, (long long)m,
+                  (long long)n, (long long)k, kScratchDoubles);
+    return LRA_E_UNSUPPORTED;
+  }
+  double* base = nullptr;
+  LRA_CHECK(scratch_on(c->device, &base));
+  LRA_HIP(hipSetDevice(c->device));
+  PowerBufs b;
+  b.xn = base;
+  b.xm = b.xn + std::max<i64>(n, 1);
+  b.wk = b.xm + std::max<i64>(m, 1);
+  Op o{m, n, A, lda, k, sk, rd, T, ldt, nullptr, 0};
+  o.part = b.wk + std::max<i64>(k, 1) + 64;
+  if (k == 0) o.k = -1;
+  return power_iteration(c, o, 0, opts, x0, result, &b);
 }
