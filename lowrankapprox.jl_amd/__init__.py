@@ -606,6 +606,7 @@ class BatchedID:
         self.k = np.full(nb, -1, dtype=np.int64)
         self.n_fallback = 0
         self.h = C.c_void_p()
+        _lib.register_owner(self)
         check(_lib.load().lra_idfact_batched_plan(
             self.ctx.h, nb, self.m.ctypes.data, self.n.ctypes.data, C.cast(self._A, C.c_void_p),
             self.lda.ctypes.data, C.byref(self._copts), C.cast(self._sk, C.c_void_p),
@@ -1406,6 +1407,7 @@ class Server:
         self.h = C.c_void_p()
         check(_lib.load().lra_server_create(self.device, int(hbm_budget), int(workers),
                                             C.byref(self.h)), "lra_server_create")
+        _lib.register_owner(self)
         self._keep = {}  # handle -> (A, kind): A stays alive while the request may read it
 
     def submit(self, A, kind: str = "idfact", opts: LRAOptions = None, **kw) -> int:

@@ -5,9 +5,11 @@ there is no CPU fallback in the product path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblra_hip.so")
@@ -227,6 +229,38 @@ def default_opts() -> lra_opts:
     return o
 
 
+# Objects holding library handles (servers, batch plans), closed before the contexts at exit.
+_OWNERS = weakref.WeakSet()
+
+
+def register_owner(obj) -> None:
+    """Track an object with a close() method that releases library handles."""
+    _OWNERS.add(obj)
+
+
+def _teardown() -> None:
+    """Release every library object while the HIP runtime is still loaded: servers and batch
+    plans first (their worker threads and graphs use the device), then the per-device
+    contexts (streams, events, workspaces).  Runs at interpreter exit, before module and
+    runtime teardown, so no stream or event outlives the runtime that owns it."""
+    if _lib is None:
+        return
+    for obj in list(_OWNERS):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    for dev, ctx in list(Context._ctx.items()):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+    Context._ctx.clear()
+
+
+atexit.register(_teardown)
+
+
 class Context:
     """One lra_ctx per (process, device); work is enqueued on torch's current stream."""
 
@@ -250,3 +284,9 @@ class Context:
 
     def sync(self) -> None:
         check(load().lra_sync(self.h), "lra_sync")
+
+    def close(self) -> None:
+        """Wait for the context's work and destroy it (lra_ctx_destroy)."""
+        if self.h:
+            load().lra_ctx_destroy(self.h)
+            self.h = C.c_void_p()
