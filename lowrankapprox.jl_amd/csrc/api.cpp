@@ -486,7 +486,7 @@ static double prec_floor(int prec) {
 using SketchFn = std::function<int(i64, uint64_t, int, double*)>;
 
 static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opts* o,
-                          uint64_t stream_base, SketchFactOut* out) {
+                          uint64_t stream_base, SketchFactOut* out, bool pair_ok = true) {
   const bool gauss = o->sketch == LRA_SKETCH_RANDN;
   int prec = gauss ? o->precision : LRA_PREC_F64;
   const bool adap = o->sketchfact_adap || o->rank < 0;
@@ -497,6 +497,60 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
   for (i64 rnd = 0;;) {
     const i64 ell = samp_of(o, target);
     if (ell <= 0) { lra_set_error("sketch order %lld", (long long)ell); return LRA_E_UNSUPPORTED; }
+    // Rounds 1+2, 3+4, ... of an FP64 adaptive loop run as pairs: both sketches are formed
+    // and factored in one co-resident launch, round r+1 on the larger share of the grid, so
+    // round r's pivoted QR costs no time of its own (dense matrices only: a caller's
+    // operator is applied exactly once per round, as the reference does).  Each round's
+    // factorization is exactly what it is alone (its own sketch, stop rule and stop_after);
+    // the first accepted round is taken, as the sequential loop would.  A pair that does not
+    // fit one grid runs round by round.
+    if (pair_ok && adap && prec == LRA_PREC_F64 && (rnd & 1) == 1) {
+      const i64 ell2 = samp_of(o, 2 * target);
+      if (ell2 > ell && ell2 < (1 << 20) && N < (1 << 30) &&
+          lra_qrcp_pair_fits(c, (int)std::min(ell, N), (int)std::min(ell2, N), (int)N) &&
+          ell <= N && ell2 <= N) {
+        const i64 ells[2] = {ell, ell2}, targets[2] = {target, 2 * target};
+        double* Bs[2] = {nullptr, nullptr};
+        i64* jp[2] = {nullptr, nullptr};
+        double* tu[2] = {nullptr, nullptr};
+        QrcpCall qs[2];
+        for (int h = 0; h < 2; ++h) {
+          LRA_CHECK(lra_wsT(c, h ? WS_SKETCH3 : WS_SKETCH, (size_t)ells[h] * std::max<i64>(N, 1), &Bs[h]));
+          LRA_CHECK(lra_wsT(c, h ? WS_JPVT2 : WS_JPVT, (size_t)std::max<i64>(N, 1), &jp[h]));
+          LRA_CHECK(lra_wsT(c, h ? WS_TAU2 : WS_TAU, (size_t)std::max<i64>(std::min(ells[h], N), 1), &tu[h]));
+        }
+        for (int h = 0; h < 2; ++h) {
+          LRA_CHECK(make(ells[h], stream_base + (uint64_t)(rnd + h), prec, Bs[h]));
+          const i64 l = std::min(ells[h], N);
+          const i64 kmax = (o->rank < 0 || o->rank > l) ? l : o->rank;
+          QrcpCall& q = qs[h];
+          q = QrcpCall{};
+          q.A = Bs[h]; q.lda = ells[h]; q.m = (int)ells[h]; q.n = (int)N; q.offset = 0;
+          q.kmax = (int)kmax; q.nb = (int)std::min<i64>(o->nb, std::max<i64>(kmax, 1));
+          q.mode = 0; q.atol = o->atol; q.rtol = o->rtol; q.jpvt = jp[h]; q.tau = tu[h];
+          q.stop_after = (int)std::min<i64>(targets[h], 1 << 30);
+          q.k_out = dk + h; q.early_out = nullptr;
+        }
+        const int st = lra_qrcp_pair(c, qs[0], qs[1]);
+        if (st == LRA_OK) {
+          i64 hv[2] = {0, 0};
+          LRA_CHECK(read_scalars(c, dk, hv, 2));
+          for (int h = 0; h < 2; ++h) {
+            if (hv[h] < targets[h]) {
+              out->B = Bs[h]; out->Bsave = nullptr; out->ell = ells[h]; out->N = N;
+              out->jpvt = jp[h]; out->tau = tu[h]; out->k = hv[h]; out->rounds = rnd + h + 1;
+              out->prec = prec; out->stream_id = stream_base + rnd + h;
+              return LRA_OK;
+            }
+          }
+          target *= 4;
+          rnd += 2;
+          continue;
+        }
+        if (st != LRA_E_UNSUPPORTED) return st;
+        (void)hipGetLastError();  // not launched: factor round by round below
+      }
+    }
     double* B = nullptr;
     LRA_CHECK(lra_wsT(c, WS_SKETCH, (size_t)ell * std::max<i64>(N, 1), &B));
     LRA_CHECK(make(ell, stream_base + (uint64_t)rnd, prec, B));
@@ -684,7 +738,7 @@ static int sketchfact_left(lra_ctx* c, int trans, const LraSrc& s, const lra_opt
     if (niter > 0) LRA_CHECK(power_iterate(c, trans, s, ell, B, niter, herm));
     return LRA_OK;
   };
-  return sketchfact_run(c, N, make, o, stream_base, out);
+  return sketchfact_run(c, N, make, o, stream_base, out, !s.op());
 }
 
 int lra_src_getcols(lra_ctx* c, const LraSrc& s, int trans, const i64* idx, i64 k, double* C,

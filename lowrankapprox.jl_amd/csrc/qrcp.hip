@@ -104,8 +104,9 @@ LRA_DEV u64x2 wait_chunk(const QrcpDev& p, const u64x2* src, uint64_t tag) {
   return w;
 }
 
+// One factorization on the blocks [0, p.G) of its grid; g = this block's index among them.
 template <int RPT, int BLOCK, bool COOP>
-__global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
+__device__ __forceinline__ void qrcp_body(const QrcpDev& p, const int g) {
   constexpr int NW = BLOCK / 64;
   constexpr int XS = 64 * RPT;  // rows held by one column group (>= m)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -113,7 +114,6 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
   const int lgpw = 6 - ltpc;  // columns per wave = 1 << lgpw
   const int lic = lane & (tpc - 1);
   const int lc = (wave << lgpw) + (lane >> ltpc);
-  const int g = blockIdx.x;
   const int col = g * p.cpw + lc;
   const bool have = (lc < p.cpw) && (col < p.n);
   const int m = p.m;
@@ -545,6 +545,30 @@ __global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
 }
 
 template <int RPT, int BLOCK, bool COOP>
+__global__ __launch_bounds__(BLOCK) void qrcp_kernel(QrcpDev p) {
+  qrcp_body<RPT, BLOCK, COOP>(p, blockIdx.x);
+}
+
+// Two independent factorizations (two adaptive rounds' sketches of one matrix) in one
+// co-resident grid: blocks [0, g1) factor d0, blocks [g1, g1 + d1.G) factor d1.  Each has its
+// own exchange slots and records, so neither waits for the other; the shorter one ends first
+// and leaves its CUs to the longer one.
+template <int RPT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void qrcp_pair_kernel(QrcpDev d0, QrcpDev d1, int g1) {
+  const bool second = (int)blockIdx.x >= g1;
+  qrcp_body<RPT, BLOCK, true>(second ? d1 : d0, second ? (int)blockIdx.x - g1 : (int)blockIdx.x);
+}
+
+template <int RPT, int BLOCK>
+hipError_t launch_pair_rpt(const QrcpDev& d0, const QrcpDev& d1, hipStream_t st) {
+  dim3 grid(d0.G + d1.G), block(BLOCK);
+  int g1 = d0.G;
+  void* args[] = {(void*)&d0, (void*)&d1, (void*)&g1};
+  return hipLaunchCooperativeKernel((const void*)qrcp_pair_kernel<RPT, BLOCK>, grid, block, args,
+                                    0, st);
+}
+
+template <int RPT, int BLOCK, bool COOP>
 hipError_t launch_rpt(const QrcpDev& d, hipStream_t st) {
   dim3 grid(d.G), block(BLOCK);
   if constexpr (COOP) {
@@ -594,6 +618,42 @@ hipError_t launch_any(int rpt, int block, const QrcpDev& d, hipStream_t st) {
     }
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_pair_any(int rpt, int block, const QrcpDev& d0, const QrcpDev& d1,
+                           hipStream_t st) {
+  if (block == 512) {
+    switch (rpt) {
+#define CASE(R) case R: return launch_pair_rpt<R, 512>(d0, d1, st);
+      LRA_RPT_512C(CASE)
+#undef CASE
+    }
+  } else if (block == 256) {
+    switch (rpt) {
+#define CASE(R) case R: return launch_pair_rpt<R, 256>(d0, d1, st);
+      LRA_RPT_256(CASE)
+#undef CASE
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+int occupancy_pair(int rpt, int block, int* nb) {
+  hipError_t e = hipErrorInvalidValue;
+  if (block == 256) {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_pair_kernel<R, 256>, 256, 0); break;
+      LRA_RPT_256(CASE)
+#undef CASE
+    }
+  } else if (block == 512) {
+    switch (rpt) {
+#define CASE(R) case R: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, qrcp_pair_kernel<R, 512>, 512, 0); break;
+      LRA_RPT_512C(CASE)
+#undef CASE
+    }
+  }
+  return e == hipSuccess ? 0 : 1;
 }
 
 // co-resident blocks per CU of the cooperative variant
@@ -680,6 +740,86 @@ static bool choose_shape(lra_ctx* ctx, int m, int n, QrcpShape* out) {
   return found;
 }
 
+static QrcpDev dev_of(lra_ctx* ctx, const QrcpCall& q, const QrcpShape& sh) {
+  QrcpDev d;
+  d.A = q.A; d.lda = q.lda; d.m = q.m; d.n = q.n; d.offset = q.offset; d.kmax = q.kmax;
+  d.nb = q.nb; d.mode = q.mode; d.stop_after = q.stop_after;
+  d.lastrk = std::min(q.m, q.n + q.offset);
+  d.atol = q.atol; d.rtol = q.rtol; d.jpvt = q.jpvt; d.tau = q.tau; d.vn1g = q.vn1;
+  d.vn2g = q.vn2; d.k_out = q.k_out; d.early_out = q.early_out;
+  d.G = sh.G; d.cpw = sh.cpw; d.tpc = sh.tpc;
+  d.ltpc = 0;
+  while ((1 << d.ltpc) < sh.tpc) ++d.ltpc;
+  // tau, beta and reflector rows up to the column group's padded height tpc*rpt
+  d.cand_stride = ((sh.tpc * sh.rpt + 2 + 7) / 8) * 8;
+  d.err = ctx->d_err;
+  return d;
+}
+
+// The shapes of a pair launch: the larger sketch (rows m1 >= m0) takes the shape it would
+// get alone; the smaller one takes the same rows per lane and block size with the fewest
+// lanes per column that hold it.  Both must be cooperative, each within its slot exchange, and
+// together co-resident.
+static bool pair_shapes(lra_ctx* ctx, int m0, int m1, int n, QrcpShape* s0, QrcpShape* s1) {
+  if (m0 > m1 || n <= 0) return false;
+  if (!choose_shape(ctx, m1, n, s1) || s1->G <= 1) return false;
+  int tpc = 1;
+  while (tpc * s1->rpt < m0) tpc <<= 1;
+  if (tpc > 64) return false;
+  const int cpw = s1->block / tpc;
+  const int G = (n + cpw - 1) / cpw;
+  if (G <= 1 || G > s1->block || s1->G > s1->block) return false;
+  *s0 = {tpc, s1->rpt, s1->block, G, cpw};
+  int per_cu = 0;
+  if (occupancy_pair(s1->rpt, s1->block, &per_cu) != 0 || per_cu < 1) return false;
+  return s0->G + s1->G <= per_cu * ctx->num_cu;
+}
+
+bool lra_qrcp_pair_fits(lra_ctx* ctx, int m0, int m1, int n) {
+  const char* e = getenv("LRA_QRCP_PAIR");
+  if (e && e[0] == '0') return false;
+  QrcpShape s0, s1;
+  return pair_shapes(ctx, m0, m1, n, &s0, &s1);
+}
+
+int lra_qrcp_pair(lra_ctx* ctx, const QrcpCall& q0, const QrcpCall& q1) {
+  QrcpShape s0, s1;
+  if (q0.mode != 0 || q1.mode != 0 || q0.n != q1.n || q0.kmax <= 0 || q1.kmax <= 0 ||
+      q0.nb > NB_MAX || q1.nb > NB_MAX || q0.n >= (int)POS_NONE ||
+      !pair_shapes(ctx, q0.m, q1.m, q0.n, &s0, &s1)) {
+    lra_set_error("qrcp pair: shapes %dx%d and %dx%d do not fit one co-resident grid", q0.m,
+                  q0.n, q1.m, q1.n);
+    return LRA_E_UNSUPPORTED;
+  }
+  QrcpDev d0 = dev_of(ctx, q0, s0), d1 = dev_of(ctx, q1, s1);
+  // exchange area: [header 256 B][slots 2 G0][slots 2 G1][records 2 G0 x stride0][2 G1 x stride1]
+  const size_t sb0 = 16 * 2 * (size_t)s0.G, sb1 = 16 * 2 * (size_t)s1.G;
+  const size_t cb0 = sb0 * d0.cand_stride, cb1 = sb1 * d1.cand_stride;
+  void* area = nullptr;
+  bool uc = false;
+  LRA_CHECK(lra_coop_area(ctx, 256 + sb0 + sb1 + cb0 + cb1, &area, &uc));
+  char* base = (char*)area;
+  d0.uncached = d1.uncached = uc ? 1 : 0;
+  d0.abort_flag = d1.abort_flag = (int*)base;
+  d0.slots = (uint64_t*)(base + 256);
+  d1.slots = (uint64_t*)(base + 256 + sb0);
+  d0.cand = (double*)(base + 256 + sb0 + sb1);
+  d1.cand = (double*)(base + 256 + sb0 + sb1 + cb0);
+  LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + sb0 + sb1) / 4));
+  d0.gen = d1.gen = ++ctx->qrcp_gen;
+  hipError_t e;
+  {
+    SpinSection gate(ctx);
+    e = launch_pair_any(s1.rpt, s1.block, d0, d1, ctx->stream);
+  }
+  if (e != hipSuccess) {
+    lra_set_error("qrcp pair launch (%dx%d tpc=%d, %dx%d tpc=%d, rpt=%d G=%d+%d): %s", q0.m, q0.n,
+                  s0.tpc, q1.m, q1.n, s1.tpc, s1.rpt, s0.G, s1.G, hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
 int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   if (q.kmax <= 0) {
     // nothing to factor: k = 0 (and jpvt = identity in mode 0)
@@ -702,18 +842,7 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   const bool force_global = fg && fg[0] == '1';
   if ((force_global && q.mode == 0 && q.offset == 0) || !choose_shape(ctx, q.m, q.n, &sh))
     return lra_qrcp_global(ctx, q);
-  QrcpDev d;
-  d.A = q.A; d.lda = q.lda; d.m = q.m; d.n = q.n; d.offset = q.offset; d.kmax = q.kmax;
-  d.nb = q.nb; d.mode = q.mode; d.stop_after = q.stop_after;
-  d.lastrk = std::min(q.m, q.n + q.offset);
-  d.atol = q.atol; d.rtol = q.rtol; d.jpvt = q.jpvt; d.tau = q.tau; d.vn1g = q.vn1;
-  d.vn2g = q.vn2; d.k_out = q.k_out; d.early_out = q.early_out;
-  d.G = sh.G; d.cpw = sh.cpw; d.tpc = sh.tpc;
-  d.ltpc = 0;
-  while ((1 << d.ltpc) < sh.tpc) ++d.ltpc;
-  // tau, beta and reflector rows up to the column group's padded height tpc*rpt
-  d.cand_stride = ((sh.tpc * sh.rpt + 2 + 7) / 8) * 8;
-  d.err = ctx->d_err;
+  QrcpDev d = dev_of(ctx, q, sh);
   if (sh.G > sh.block) {
     lra_set_error("qrcp: %d blocks exceed the %d-slot exchange", sh.G, sh.block);
     return LRA_E_UNSUPPORTED;
