@@ -396,6 +396,46 @@ int lra_pheigvals(lra_ctx* ctx, int64_t n, const double* A, int64_t lda, const l
 int lra_gesvj_f64(lra_ctx* ctx, int64_t m, int64_t n, double* X, int64_t ldx, double* U,
                   int64_t ldu, double* s, double* V, int64_t ldv);
 
+/* ---- complex element type (ComplexF64) --------------------------------------------------
+ * Complex matrices are column-major with interleaved (re, im) doubles; leading dimensions
+ * count complex elements.  These replace the reference's paths for T <: Complex
+ * (src/id.jl:434-447 with crandn, src/util.jl:4-11; the complex SRFT, src/sketch.jl:345-376,
+ * 452-472; zlaqps through src/lapack.jl:117-139).  The QRCP runs in HBM (three launches per
+ * pivot step); complex power iterations, maxdet swaps and the complex partial QR / SVD /
+ * eigen factorizations are not provided. */
+
+/* The random objects of a complex left sketch of round stream_id on an m-row operand:
+ *   randn: d = Phi (ell x m complex, 2*ell*m doubles), Phi[p,i] = G[2p,i] + i G[2p+1,i] for the
+ *          real 2ell x m Gaussian G that lra_sketch_draw(randn) gives (crandn's layout);
+ *   srft:  d = unit-modulus signs (m complex), idx = frequency samples (ell, 1-based);
+ *   sub:   idx = rows (ell, 1-based);
+ *   sprn:  d = coefficients (m complex), idx = permutation (m, 1-based). */
+int lra_zsketch_draw(lra_ctx* ctx, int sketch, uint64_t seed, uint64_t stream_id, int64_t m,
+                     int64_t ell, double* d, int64_t* idx);
+
+/* Complex left sketch B (ell x N, ldb = ell) = Phi * op(A), op = A or A^H, Phi of the sketch
+ * kind drawn from (seed, stream_id) or injected (d, idx as lra_zsketch_draw gives them; idx
+ * NULL draws them; randn always draws).  SRFT: B[p, j] = DFT(d .* op(A)[:, j])[idx[p]]. */
+int lra_zsketch(lra_ctx* ctx, int sketch, int trans, int64_t m, int64_t n, const double* A,
+                int64_t lda, int64_t ell, const double* d, const int64_t* idx, uint64_t seed,
+                uint64_t stream_id, double* B, int64_t ldb);
+
+/* Complex rank-adaptive GEQP3 (geqp3_adap!, src/pqr.jl:348-418, zlaqps arithmetic) in place:
+ * jpvt (n, 1-based), tau (min(m,n) complex), *k (host). */
+int lra_zgeqp3_adap(lra_ctx* ctx, int64_t m, int64_t n, double* B, int64_t ldb,
+                    int64_t rank_cap, double atol, double rtol, int64_t nb, int64_t* jpvt,
+                    double* tau, int64_t* k);
+
+/* Complex T = R11 \ R12 (maxdet_t, src/pqr.jl:438-442). */
+int lra_zid_T(lra_ctx* ctx, int64_t k, int64_t n, const double* R, int64_t ldr, double* T,
+              int64_t ldt);
+
+/* idfact(trans, A, opts) for a ComplexF64 A: sk, rd as lra_idfact, T complex (ldt in complex
+ * elements; T = NULL: library-owned).  Gaussian, SRFT, subset and sparse sketches and
+ * sketch = none; LRA_E_UNSUPPORTED for sketch_randn_niter > 0 or maxdet_tol >= 0. */
+int lra_zidfact(lra_ctx* ctx, int trans, int64_t m, int64_t n, const double* A, int64_t lda,
+                const lra_opts* opts, lra_id_out* out);
+
 /* Batched ID of many independent blocks (MI355X extension; no counterpart in the reference,
  * which factors blocks one idfact call at a time, src/id.jl:434-447).  Block b (m[b] x n[b],
  * A[b] with lda[b]) gets exactly the result of lra_idfact on it alone with the same opts: its

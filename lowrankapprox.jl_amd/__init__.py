@@ -28,7 +28,8 @@ __all__ = ["LRAOptions", "IDPackedV", "idfact", "id", "sketch", "sketchfact", "g
            "LinearOperator", "CURPackedU", "curfact", "cur", "snorm", "snormdiff", "geqrf",
            "orgqr", "qr", "PartialQR", "PQRFactors", "pqrfact", "pqr", "PartialSVD",
            "psvdfact", "psvdvals", "psvd", "PartialHermEigen", "pheigfact", "pheigvals", "pheig", "ID", "dgemm",
-           "BatchedID", "idfact_batched", "Server", "prange", "CUR", "sketch_draw"]
+           "BatchedID", "idfact_batched", "Server", "prange", "CUR", "sketch_draw",
+           "zsketch_draw", "zgemm"]
 
 _EPS = float(np.finfo(np.float64).eps)
 
@@ -67,7 +68,7 @@ class LRAOptions:
     def for_type(cls, eltype: str = "f64", **kw) -> "LRAOptions":
         """LRAOptions(T) (src/LowRankApprox.jl:96-119): rtol = 5 eps(T) and
         pheig_orthtol = sqrt(eps(T)) for the element type T ("f64" or "f32")."""
-        eps = _EPS if eltype == "f64" else float(np.finfo(np.float32).eps)
+        eps = _EPS if eltype in ("f64", "c128") else float(np.finfo(np.float32).eps)
         return cls(rtol=5 * eps, pheig_orthtol=math.sqrt(eps)).copy(**kw)
 
     def copy(self, **kw) -> "LRAOptions":
@@ -130,28 +131,38 @@ def context(device: Optional[int] = None) -> _lib.Context:
 
 
 def _eltype(A) -> str:
-    """The reference's element type of an input: "f32" for float32 data, "f64" otherwise.
-    Float32 inputs take Float32's default tolerances and are factored in FP64 arithmetic.
-    Complex inputs raise: no complex kernels are built."""
+    """The reference's element type of an input: "f32", "f64", "c64" or "c128".  Float32
+    (ComplexF32) inputs take Float32's default tolerances and are factored in FP64
+    (ComplexF64) arithmetic."""
     torch = _torch()
     dt = getattr(A, "dtype", None)
     if dt is None:
         return "f64"
     if isinstance(A, np.ndarray):
         if np.iscomplexobj(A):
-            raise TypeError("complex element types are not supported")
+            return "c64" if A.dtype == np.complex64 else "c128"
         return "f32" if A.dtype == np.float32 else "f64"
     if isinstance(A, torch.Tensor):
         if A.is_complex():
-            raise TypeError("complex element types are not supported")
+            return "c64" if A.dtype == torch.complex64 else "c128"
         return "f32" if A.dtype == torch.float32 else "f64"
     return "f64"
+
+
+def _is_complex(A) -> bool:
+    return _eltype(A) in ("c64", "c128")
+
+
+def _real_only(A, what: str) -> None:
+    if _is_complex(A):
+        raise TypeError(f"{what}: complex element types are supported by idfact/id only")
 
 
 def to_device(A, dtype=None):
     """Column-major float64 CUDA tensor for A (numpy array or tensor)."""
     torch = _torch()
-    _eltype(A)  # complex data is rejected, never silently truncated to its real part
+    if _is_complex(A):  # never silently truncated to its real part
+        raise TypeError("complex data on a real path (use idfact/id for complex matrices)")
     if isinstance(A, np.ndarray):
         A = torch.from_numpy(np.asfortranarray(A, dtype=np.float64 if dtype is None else dtype))
     if A.dim() == 1:
@@ -165,6 +176,71 @@ def to_device(A, dtype=None):
     if not colmajor:
         A = A.t().contiguous().t()
     return A
+
+
+def to_device_z(A):
+    """Column-major complex128 CUDA tensor for A (numpy array or tensor)."""
+    torch = _torch()
+    if isinstance(A, np.ndarray):
+        A = torch.from_numpy(np.asfortranarray(A, dtype=np.complex128))
+    if A.is_complex():
+        A = A.resolve_conj()  # a lazily conjugated view becomes real data
+    if A.dim() == 1:
+        return A.to("cuda", dtype=torch.complex128).contiguous()
+    if A.dtype != torch.complex128:
+        A = A.to(torch.complex128)
+    if not A.is_cuda:
+        A = A.cuda()
+    colmajor = ((A.shape[0] <= 1 or A.stride(0) == 1) and
+                (A.shape[1] <= 1 or A.stride(1) >= max(1, A.shape[0])))
+    if not colmajor:
+        A = A.t().contiguous().t()
+    return A
+
+
+def _z_compact(Z):
+    """Z as a compact column-major complex matrix (stride(1) == rows)."""
+    if Z.dim() == 1:
+        Z = Z.reshape(-1, 1)
+    if Z.stride(0) == 1 and (Z.shape[1] <= 1 or Z.stride(1) == Z.shape[0]):
+        return Z
+    return Z.t().contiguous().t()
+
+
+def _z_real_view(Z):
+    """The real 2p x r view of a compact column-major complex p x r matrix (interleaved
+    rows)."""
+    torch = _torch()
+    p, r = Z.shape
+    return torch.view_as_real(Z).as_strided((2 * p, r), (1, 2 * max(p, 1)))
+
+
+def _z_expand(A):
+    """The real 2p x 2q matrix of left multiplication by the complex p x q matrix A on
+    interleaved vectors: 2 x 2 blocks [Re -Im; Im Re]."""
+    torch = _torch()
+    p, q = A.shape
+    W = torch.empty((2 * q, 2 * p), dtype=torch.float64, device="cuda").t()
+    W[0::2, 0::2] = A.real
+    W[0::2, 1::2] = -A.imag
+    W[1::2, 0::2] = A.imag
+    W[1::2, 1::2] = A.real
+    return W
+
+
+def zgemm(A, B):
+    """The complex product A B on the library's FP64 MFMA GEMM (one real GEMM of the
+    expanded A with B's interleaved real view)."""
+    torch = _torch()
+    A, B = to_device_z(A), _z_compact(to_device_z(B))
+    p, r = A.shape[0], B.shape[1]
+    C = torch.empty((r, p), dtype=torch.complex128, device="cuda").t()
+    if p and r:
+        if A.shape[1] == 0:
+            C.zero_()
+        else:
+            dgemm(_z_expand(A), _z_real_view(B), C=_z_real_view(C))
+    return C
 
 
 def _ld(A) -> int:
@@ -395,6 +471,8 @@ class IDPackedV:
     def apply(self, X, trans: str = "n"):
         """V X (trans "n", X n x nrhs) or V' X (trans "c", X k x nrhs) for V = [I T] P'
         (mul!/mulc! of IDPackedV, src/id.jl:94-152); a vector gives a vector."""
+        if self.T.is_complex():
+            return self._zapply(X, trans)
         ctx = context()
         X, vec = _in_mat(X)
         nrhs = X.shape[1]
@@ -406,6 +484,25 @@ class IDPackedV:
                                        _ptr(X), _ld(X), _ptr(Y), max(1, rows)), "lra_id_apply")
         return _out_mat(Y, vec)
 
+    def _zapply(self, X, trans: str):
+        """Complex V X = X[sk] + T X[rd] or V^H X = P [I; T^H] X (src/id.jl:94-152)."""
+        torch = _torch()
+        X = to_device_z(X)
+        vec = X.dim() == 1
+        if vec:
+            X = X.reshape(-1, 1)
+        if trans == "n":
+            Y = X[self.sk - 1, :].clone()
+            if self.n > self.k:
+                Y += zgemm(self.T, X[self.rd - 1, :])
+        else:
+            Y = torch.empty((self.n, X.shape[1]), dtype=torch.complex128, device="cuda")
+            Y[self.sk - 1, :] = X
+            if self.n > self.k:
+                Y[self.rd - 1, :] = zgemm(self.T.conj().t(), X)
+        Y = _z_compact(Y)
+        return Y[:, 0] if vec else Y
+
     def rapply(self, X, trans: str = "n"):
         """X V (trans "n", X m x k) or X V' (trans "c", X m x n): the right products of
         src/id.jl:154-200, as (V' X')' and (V X')'."""
@@ -415,12 +512,15 @@ class IDPackedV:
         """The k x n interpolation matrix [I T] P' (Matrix(V), src/id.jl:59-66): column
         sk[i] is e_i and column rd[j] is T[:, j], scattered on the device."""
         torch = _torch()
-        M = empty_colmajor(self.k, self.n)
-        M.zero_()
+        if self.T.is_complex():
+            M = torch.zeros((self.n, self.k), dtype=torch.complex128, device="cuda").t()
+        else:
+            M = empty_colmajor(self.k, self.n)
+            M.zero_()
         if self.k:
             M[torch.arange(self.k, device="cuda"), self.sk - 1] = 1.0
             if self.n > self.k:
-                M[:, self.rd - 1] = to_device(self.T)
+                M[:, self.rd - 1] = self.T
         return M
 
 
@@ -482,13 +582,22 @@ def _as_matrix(X):
 
 def _right_apply(F, X, trans: str):
     """X op(F) through F's left product: X op(F) = (op(F)' X')', returned column-major; a
-    vector x is the row x' and gives a vector."""
-    X = to_device(X)
+    vector x is the row x' and gives a vector (' is the conjugate transpose)."""
+    X = to_device_z(X) if (_is_complex(X) or _factor_is_complex(F)) else to_device(X)
     vec = X.dim() == 1
     if vec:
         X = X.reshape(1, -1)
-    Y = F.apply(X.t(), "c" if trans == "n" else "n").contiguous().t()
+    Y = F.apply(X.mH, "c" if trans == "n" else "n").mH
+    Y = Y.t().contiguous().t()
     return Y[0, :] if vec else Y
+
+
+def _factor_is_complex(F) -> bool:
+    if isinstance(F, IDPackedV):
+        return F.T.is_complex()
+    if isinstance(F, ID):
+        return F.C.is_complex()
+    return False
 
 
 @dataclasses.dataclass
@@ -503,12 +612,26 @@ class ID:
         LinearOperator (columns from one-hot applications)."""
         if isinstance(A, LinearOperator):
             return cls(to_device(A.getcols(V.sk, trans)), V)
+        if _is_complex(A) or V.T.is_complex():
+            A = to_device_z(A)
+            X = A if trans == "n" else A.conj().t()
+            return cls(_z_compact(X[:, V.sk - 1]), V)
         A = to_device(A)
         X = A if trans == "n" else A.t()
         return cls(X[:, V.sk - 1].t().contiguous().t(), V)
 
     def apply(self, X, trans: str = "n"):
         """A X = C (V X) (trans "n") or A' X = V' (C' X) (trans "c") (src/id.jl:263-300)."""
+        if self.C.is_complex():
+            X = to_device_z(X)
+            vec = X.dim() == 1
+            if vec:
+                X = X.reshape(-1, 1)
+            if trans == "n":
+                Y = zgemm(self.C, self.V.apply(X, "n"))
+            else:
+                Y = self.V.apply(zgemm(self.C.conj().t(), X), "c")
+            return Y[:, 0] if vec else Y
         X, vec = _in_mat(X)
         if trans == "n":
             Y = dgemm(self.C, self.V.apply(X, "n"))
@@ -518,11 +641,17 @@ class ID:
 
     def rapply(self, X, trans: str = "n"):
         """X A = (X C) V (trans "n") or X A' = (X V') C' ("c") (src/id.jl:302-330)."""
-        X = to_device(X)
+        X = to_device_z(X) if self.C.is_complex() else to_device(X)
         vec = X.dim() == 1
         if vec:
             X = X.reshape(1, -1)
-        if trans == "n":
+        if self.C.is_complex():
+            X = to_device_z(X)
+            if trans == "n":
+                Y = self.V.rapply(zgemm(X, self.C), "n")
+            else:
+                Y = zgemm(self.V.rapply(X, "c"), self.C.conj().t())
+        elif trans == "n":
             Y = self.V.rapply(dgemm(X, self.C), "n")
         else:
             Y = dgemm(self.V.rapply(X, "c"), self.C, transb="c")
@@ -530,6 +659,8 @@ class ID:
 
     def matrix(self):
         """Matrix(A) = C V (src/id.jl:244)."""
+        if self.C.is_complex():
+            return zgemm(self.C, self.V.matrix())
         return dgemm(self.C, self.V.matrix())
 
 
@@ -542,6 +673,8 @@ def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
         return A.idfact(trans, opts, **kw)
     opts = (opts or LRAOptions.for_type(_eltype(A))).copy(**kw)
     opts.pqrfact_retval = "t"
+    if _is_complex(A):
+        return _zidfact(trans, A, opts)
     ctx = context()
     A = to_device(A)
     m, n = A.shape
@@ -557,6 +690,62 @@ def idfact(trans, A=None, opts: LRAOptions = None, **kw) -> IDPackedV:
     k = int(out.k)
     T = _borrow_T(out, k, N - k)
     return IDPackedV(sk[:k], rd[: N - k], T, int(out.rounds))
+
+
+def _zidfact(trans: str, A, opts: LRAOptions) -> IDPackedV:
+    """idfact for ComplexF64 (ComplexF32 inputs are widened) through lra_zidfact."""
+    torch = _torch()
+    ctx = context()
+    A = to_device_z(A)
+    m, n = A.shape
+    N = n if trans == "n" else m
+    sk = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+    rd = torch.empty(max(N, 1), dtype=torch.int64, device="cuda")
+    out = _lib.lra_id_out()
+    out.sk, out.rd = _ptr(sk), _ptr(rd)
+    out.T, out.ldt = None, 0
+    copts = opts.to_c()
+    check(_lib.load().lra_zidfact(ctx.h, _lib.TRANS[trans], m, n, _ptr(A), _ld(A),
+                                  C.byref(copts), C.byref(out)), "lra_zidfact")
+    k = int(out.k)
+    nr = N - k
+    T = torch.empty((nr, k), dtype=torch.complex128, device="cuda").t()
+    if k > 0 and nr > 0:
+        # T is complex (k x nr, ldt complex elements): copy it as 2k x nr doubles
+        _copy_device(T.data_ptr(), int(out.T), 2 * k, nr, 2 * int(out.ldt), 2 * k)
+    return IDPackedV(sk[:k], rd[:nr], T, int(out.rounds))
+
+
+def zsketch_draw(sketch: str, m: int, order: int, seed: int = 0, stream: int = 0):
+    """The random objects of a complex left sketch (lra_zsketch_draw): randn -> Phi
+    (order x m complex); srft -> (signs (m complex), samples (order)); sub -> rows (order);
+    sprn -> (permutation (m), coefficients (m complex))."""
+    torch = _torch()
+    ctx = context()
+    L = _lib.load()
+    c128 = dict(dtype=torch.complex128, device="cuda")
+    i64t = dict(dtype=torch.int64, device="cuda")
+    if sketch == "randn":
+        P = torch.empty((max(m, 1), max(order, 1)), **c128).t()
+        check(L.lra_zsketch_draw(ctx.h, 1, seed, stream, m, order, _ptr(P), None),
+              "lra_zsketch_draw")
+        return P[:order, :m]
+    if sketch == "srft":
+        d, idx = torch.empty(max(m, 1), **c128), torch.empty(max(order, 1), **i64t)
+        check(L.lra_zsketch_draw(ctx.h, 2, seed, stream, m, order, _ptr(d), _ptr(idx)),
+              "lra_zsketch_draw")
+        return d[:m], idx[:order]
+    if sketch == "sub":
+        r = torch.empty(max(order, 1), **i64t)
+        check(L.lra_zsketch_draw(ctx.h, 3, seed, stream, m, order, None, _ptr(r)),
+              "lra_zsketch_draw")
+        return r[:order]
+    if sketch == "sprn":
+        s, r = torch.empty(max(m, 1), **c128), torch.empty(max(m, 1), **i64t)
+        check(L.lra_zsketch_draw(ctx.h, 4, seed, stream, m, order, _ptr(s), _ptr(r)),
+              "lra_zsketch_draw")
+        return r[:m], s[:m]
+    raise ValueError(sketch)
 
 
 class BatchedID:

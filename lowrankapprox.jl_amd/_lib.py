@@ -163,6 +163,15 @@ SIGNATURES = [
     ("lra_id_apply", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_I, P_I, P_D, i64, i64, P_D, i64,
                                P_D, i64]),
     ("lra_scale_rows", C.c_int, [C.c_void_p, i64, i64, P_D, P_D, i64]),
+    ("lra_zsketch_draw", C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, i64, i64, P_D,
+                                   P_I]),
+    ("lra_zsketch", C.c_int, [C.c_void_p, C.c_int, C.c_int, i64, i64, P_D, i64, i64, P_D, P_I,
+                              C.c_uint64, C.c_uint64, P_D, i64]),
+    ("lra_zgeqp3_adap", C.c_int, [C.c_void_p, i64, i64, P_D, i64, i64, C.c_double, C.c_double,
+                                  i64, P_I, P_D, C.POINTER(i64)]),
+    ("lra_zid_T", C.c_int, [C.c_void_p, i64, i64, P_D, i64, P_D, i64]),
+    ("lra_zidfact", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64, C.POINTER(lra_opts),
+                              C.POINTER(lra_id_out)]),
     ("lra_idfact_batched", C.c_int, [C.c_void_p, i64, P_I, P_I, C.c_void_p, P_I,
                                      C.POINTER(lra_opts), C.c_void_p, C.c_void_p, C.c_void_p,
                                      P_I, P_I, C.POINTER(i64)]),

@@ -129,6 +129,12 @@ __global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, in
     int cp = sp[0];
     for (int w = 1; w < GT / 64; ++w)
       if (sv[w] > cv || (sv[w] == cv && sp[w] < cp)) { cv = sv[w]; cp = sp[w]; }
+    if (cp < s || cp >= n) {
+      // no candidate (NaN norms): stop here rather than pivot on a column that is not there
+      st->k = s;
+      st->done = 1;
+      cp = s;
+    }
     st->pvt = cp;
     st->s = s;
   }
@@ -262,7 +268,238 @@ __global__ void qg_finish(const GState* st, i64* k_out, int* early_out) {
   if (early_out) *early_out = st->early;
 }
 
+// ---------------------------------------------------------------------------------------
+// Complex (ComplexF64) steps: zlaqps' arithmetic (src/lapack.jl:117-139 calls zlaqps for
+// complex element types).  Columns are interleaved (re, im) pairs; the norms, the diagonal
+// and the stop rule are real.  H(s) = I - tau v v^H is applied as H^H from the left
+// (A(:, j) -= conj(tau) v (v^H A(:, j))), and zlarfg makes R's diagonal real.
+
+typedef double2 zc;
+LRA_DEV zc zmul(zc a, zc b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+LRA_DEV zc zconj_mul(zc a, zc b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+LRA_DEV double zabs2(zc a) { return a.x * a.x + a.y * a.y; }
+// 1 / z by Smith's scaling
+LRA_DEV zc zrecip(zc z) {
+  if (fabs(z.x) >= fabs(z.y)) {
+    const double r = z.y / z.x, d = z.x + z.y * r;
+    return make_double2(1.0 / d, -r / d);
+  }
+  const double r = z.x / z.y, d = z.y + z.x * r;
+  return make_double2(r / d, -1.0 / d);
+}
+// sqrt(x^2 + y^2 + z^2) without overflow (dlapy3)
+LRA_DEV double lapy3(double x, double y, double z) {
+  const double xa = fabs(x), ya = fabs(y), za = fabs(z);
+  const double w = fmax(xa, fmax(ya, za));
+  if (w == 0.0) return xa + ya + za;
+  return w * sqrt((xa / w) * (xa / w) + (ya / w) * (ya / w) + (za / w) * (za / w));
+}
+
+__global__ __launch_bounds__(256) void zq_norms(int m, int n, const zc* __restrict__ B, i64 ldb,
+                                                double* vn1, double* vn2, i64* jpvt) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const zc* c = B + (i64)j * ldb;
+  double ss = 0.0;
+  for (int i = lane; i < m; i += 64) ss += zabs2(c[i]);
+  ss = group_sum(ss, 64);
+  if (lane == 0) {
+    vn1[j] = vn2[j] = sqrt(ss);
+    jpvt[j] = j + 1;
+  }
+}
+
+// step s: swap columns s and pvt, zlarfg on column s in place, copy v
+__global__ __launch_bounds__(GT) void zq_refl(int s, int m, zc* __restrict__ B, i64 ldb,
+                                              i64* jpvt, double* vn1, double* vn2, zc* tau,
+                                              double* diag, zc* v, const GState* st) {
+  __shared__ double sw[GT / 64];
+  __shared__ zc s_alpha;
+  if (st->done) return;
+  const int pvt = st->pvt;
+  zc* cs = B + (i64)s * ldb;
+  if (pvt != s) {
+    zc* cp = B + (i64)pvt * ldb;
+    for (int i = threadIdx.x; i < m; i += GT) {
+      const zc t = cs[i];
+      cs[i] = cp[i];
+      cp[i] = t;
+    }
+    if (threadIdx.x == 0) {
+      const i64 t = jpvt[pvt];
+      jpvt[pvt] = jpvt[s];
+      jpvt[s] = t;
+      vn1[pvt] = vn1[s];
+      vn2[pvt] = vn2[s];
+    }
+  }
+  __syncthreads();
+  double ss = 0.0;
+  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) ss += zabs2(cs[i]);
+  ss = group_sum(ss, 64);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = ss;
+  if (threadIdx.x == 0) s_alpha = cs[s];
+  __syncthreads();
+  double tot = 0.0;
+  for (int w = 0; w < GT / 64; ++w) tot += sw[w];
+  double alphr = s_alpha.x, alphi = s_alpha.y, xnorm = sqrt(tot), fac = 1.0;
+  zc t = make_double2(0.0, 0.0), scal = make_double2(1.0, 0.0);
+  double beta = alphr;
+  bool reflect = m - s > 1 ? (xnorm != 0.0 || alphi != 0.0) : (alphi != 0.0);
+  if (reflect) {
+    beta = -copysign(lapy3(alphr, alphi, xnorm), alphr);
+    const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+    int knt = 0;
+    if (fabs(beta) < safmin) {
+      const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+      do {
+        ++knt;
+        fac *= rsafmn;
+        beta *= rsafmn;
+        alphr *= rsafmn;
+        alphi *= rsafmn;
+      } while (fabs(beta) < safmin && knt < 20);
+      __syncthreads();  // every thread has read sw
+      double s2 = 0.0;
+      for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
+        const zc x = cs[i];
+        s2 += zabs2(make_double2(x.x * fac, x.y * fac));
+      }
+      s2 = group_sum(s2, 64);
+      if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = s2;
+      __syncthreads();
+      double t2 = 0.0;
+      for (int w = 0; w < GT / 64; ++w) t2 += sw[w];
+      xnorm = sqrt(t2);
+      beta = -copysign(lapy3(alphr, alphi, xnorm), alphr);
+    }
+    t = make_double2((beta - alphr) / beta, -alphi / beta);
+    scal = zrecip(make_double2(alphr - beta, alphi));
+    for (int j = 0; j < knt; ++j) beta *= safmin;
+  }
+  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
+    const zc x0 = cs[i];
+    const zc x = zmul(make_double2(x0.x * fac, x0.y * fac), scal);
+    cs[i] = x;
+    v[i] = x;
+  }
+  if (threadIdx.x == 0) {
+    cs[s] = reflect ? make_double2(beta, 0.0) : s_alpha;
+    v[s] = make_double2(1.0, 0.0);
+    tau[s] = t;
+    diag[s] = reflect ? beta : hypot(s_alpha.x, s_alpha.y);
+  }
+}
+
+// step s: columns s+1..n-1 (one wave each): a_j -= conj(tau) v (v^H a_j), norm downdate
+__global__ __launch_bounds__(256) void zq_apply(int s, int m, int n, int lastrk,
+                                                zc* __restrict__ B, i64 ldb,
+                                                const zc* __restrict__ v,
+                                                const zc* __restrict__ tau, double* vn1,
+                                                double* vn2, GState* st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const int j = s + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
+  zc* c = B + (i64)j * ldb;
+  double wr = 0.0, wi = 0.0;
+  for (int i = s + lane; i < m; i += 64) {
+    const zc p = zconj_mul(v[i], c[i]);
+    wr += p.x;
+    wi += p.y;
+  }
+  wr = group_sum(wr, 64);
+  wi = group_sum(wi, 64);
+  const zc ct = make_double2(tau[s].x, -tau[s].y);
+  const zc tw = zmul(ct, make_double2(wr, wi));
+  double ss = 0.0;
+  zc x = make_double2(0.0, 0.0);
+  for (int i = s + lane; i < m; i += 64) {
+    const zc vv = v[i];
+    const zc p = zmul(tw, vv);
+    const zc y = make_double2(c[i].x - p.x, c[i].y - p.y);
+    c[i] = y;
+    if (i == s) x = y;
+    else ss += zabs2(y);
+  }
+  x.x = __shfl(x.x, 0, 64);
+  x.y = __shfl(x.y, 0, 64);
+  double a = vn1[j];
+  if (s + 1 < lastrk && a != 0.0) {
+    double temp = hypot(x.x, x.y) / a;
+    temp = (1.0 + temp) * (1.0 - temp);
+    temp = temp < 0.0 ? 0.0 : temp;
+    const double q = a / vn2[j];
+    const double temp2 = temp * q * q;
+    if (temp2 <= tol3z) {
+      ss = group_sum(ss, 64);
+      if (lane == 0) {
+        vn1[j] = vn2[j] = sqrt(ss);
+        __hip_atomic_store(&st->flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else if (lane == 0) {
+      vn1[j] = a * sqrt(temp);
+    }
+  }
+}
+
 }  // namespace
+
+// Complex rank-adaptive pivoted QR (geqp3_adap!, src/pqr.jl:348-418, with zlaqps' steps) on
+// an m x n ComplexF64 matrix in HBM (interleaved re/im, leading dimension ldb in complex
+// elements): jpvt (n, 1-based), tau (kmax complex), the rank into k_out.  stop_after as in
+// QrcpCall.
+int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, int nb,
+                     double atol, double rtol, int stop_after, i64* jpvt, double* tau,
+                     i64* k_out) {
+  zc* B = reinterpret_cast<zc*>(Bz);
+  if (kmax <= 0) {
+    if (n > 0) LRA_CHECK(lra_launch_fill_i64(c->stream, jpvt, n, 1));
+    LRA_HIP(hipMemsetAsync(k_out, 0, sizeof(i64), c->stream));
+    return LRA_OK;
+  }
+  double* w = nullptr;
+  // vn1, vn2, diag (kmax), v (m complex), state
+  const size_t nd = 2 * (size_t)n + (size_t)kmax + 2 * (size_t)m + 16;
+  LRA_CHECK(lra_wsT(c, WS_QG, nd, &w));
+  double* vn1 = w;
+  double* vn2 = vn1 + n;
+  double* diag = vn2 + n;
+  zc* v = reinterpret_cast<zc*>(diag + kmax + (kmax & 1));
+  GState* st = reinterpret_cast<GState*>(v + m);
+  const int lastrk = std::min(m, n);
+  hipStream_t s0 = c->stream;
+  zc* zt = reinterpret_cast<zc*>(tau);
+  hipLaunchKernelGGL(zq_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, B, ldb,
+                     vn1, vn2, jpvt);
+  hipLaunchKernelGGL(qg_init, dim3(1), dim3(GT), 0, s0, n, vn1, atol, rtol, kmax, nb, st);
+  LRA_HIP(hipGetLastError());
+  for (int s = 0; s <= kmax; ++s) {
+    hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, kmax, nb, stop_after, vn1, diag,
+                       st);
+    if (s == kmax) break;
+    hipLaunchKernelGGL(zq_refl, dim3(1), dim3(GT), 0, s0, s, m, B, ldb, jpvt, vn1, vn2, zt,
+                       diag, v, st);
+    if (s + 1 < n)
+      hipLaunchKernelGGL(zq_apply, dim3((unsigned)((n - s - 1 + 3) / 4)), dim3(256), 0, s0, s,
+                         m, n, lastrk, B, ldb, v, zt, vn1, vn2, st);
+    LRA_HIP(hipGetLastError());
+    if ((s & 63) == 63) {
+      int done = 0;
+      LRA_HIP(hipMemcpyAsync(c->host_pinned, &st->done, sizeof(int), hipMemcpyDeviceToHost, s0));
+      LRA_HIP(hipStreamSynchronize(s0));
+      std::memcpy(&done, c->host_pinned, sizeof(int));
+      if (done) break;
+    }
+  }
+  hipLaunchKernelGGL(qg_finish, dim3(1), dim3(1), 0, s0, st, k_out, nullptr);
+  LRA_HIP(hipGetLastError());
+  return LRA_OK;
+}
 
 // geqp3_adap (mode 0 of QrcpCall) on a sketch of any size, in HBM.
 int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {

@@ -236,6 +236,131 @@ def maxdet_t(R: np.ndarray) -> np.ndarray:
     return T
 
 
+# ----------------------------------------------------------------------------------------
+# complex element type (T <: Complex): zlaqps through the reference's loop
+
+def zlaqps(A: np.ndarray, offset: int, nb: int, jpvt, vn1, vn2):
+    """One OpenBLAS zlaqps block (the ccall at src/lapack.jl:117-139 for ComplexF64; called
+    only) on A in place; returns kb and tau (kb entries)."""
+    ob = openblas()
+    m, n = A.shape
+    kb = np.zeros(1, dtype=np.int64)
+    tau = np.zeros(max(1, nb), dtype=np.complex128)
+    auxv = np.zeros(max(1, nb), dtype=np.complex128)
+    F = np.zeros((max(1, n), max(1, nb)), dtype=np.complex128, order="F")
+    pz = lambda a: a.ctypes.data_as(C.c_void_p)
+    ob.scipy_zlaqps_64_(C.byref(i64(m)), C.byref(i64(n)), C.byref(i64(offset)),
+                        C.byref(i64(nb)), pz(kb), pz(A), C.byref(i64(max(1, m))), pz(jpvt),
+                        pz(tau), pz(vn1), pz(vn2), pz(auxv), pz(F), C.byref(i64(max(1, n))))
+    return int(kb[0]), tau[: int(kb[0])]
+
+
+def zgeqp3_adap(B: np.ndarray, rank: int, atol: float, rtol: float, nb: int):
+    """geqp3_adap! + geqp3_adap_main! (src/pqr.jl:348-418) for a complex B, restated over
+    OpenBLAS zlaqps exactly as the reference's complex branch calls it (rwork norms, blocks
+    of min(nb, k-j+1) columns with offset j-1, the |A(jn-1,jn-1)| <= ptol test after each
+    block).  Returns (Bfact, jpvt 1-based, tau, k)."""
+    A = np.array(B, dtype=np.complex128, order="F", copy=True)
+    m, n = A.shape
+    jpvt = np.arange(1, n + 1, dtype=np.int64)
+    l = min(m, n)
+    k = l if (rank < 0 or rank > l) else rank
+    tau = np.zeros(max(1, k), dtype=np.complex128)
+    if k <= 0:
+        return A, jpvt, tau[:0], k
+    nbk = min(nb, k)
+    rwork = np.zeros(2 * n)
+    for j in range(n):
+        rwork[j] = rwork[n + j] = np.linalg.norm(A[:, j])
+    ptol = max(atol, rtol * rwork[:n].max())
+    j = 1
+    while j <= k:
+        jb = min(nbk, k - j + 1)
+        sub = np.asfortranarray(A[:, j - 1:])
+        jp = np.ascontiguousarray(jpvt[j - 1:])
+        v1 = np.ascontiguousarray(rwork[j - 1:n])
+        v2 = np.ascontiguousarray(rwork[n + j - 1:2 * n])
+        fjb, t = zlaqps(sub, j - 1, jb, jp, v1, v2)
+        A[:, j - 1:] = sub
+        jpvt[j - 1:] = jp
+        rwork[j - 1:n] = v1
+        rwork[n + j - 1:2 * n] = v2
+        tau[j - 1:j - 1 + fjb] = t
+        jn = j + fjb
+        if abs(A[jn - 2, jn - 2]) <= ptol:
+            for i in range(j, jn):
+                if abs(A[i - 1, i - 1]) <= ptol:
+                    return A, jpvt, tau[:i - 1], i - 1
+        j = jn
+    return A, jpvt, tau[:k], k
+
+
+def zsketch_phi(sketch: str, m: int, order: int, draws) -> np.ndarray:
+    """The dense complex left-sketch matrix (order x m) of each kind, from injected draws:
+    randn: crandn (src/util.jl:4-11) given as the matrix itself; srft: Phi[p, t] =
+    d_t exp(-2 pi i f_p t / m), f_p = idx_p - 1 -- the sampled DFT the reference's six-step
+    complex SRFT computes (src/sketch.jl:366-376, 452-472); sub: one-hot rows (src/sketch.jl:
+    248-257); sprn: row p holds s at columns r of its group (src/sketch.jl:571-589)."""
+    if sketch == "randn":
+        return np.asarray(draws, dtype=np.complex128)
+    Phi = np.zeros((order, m), dtype=np.complex128)
+    if sketch == "srft":
+        d, idx = draws
+        f = np.asarray(idx, dtype=np.int64) - 1
+        t = np.arange(m, dtype=np.int64)
+        e = (f[:, None] * t[None, :]) % m
+        Phi = np.exp(-2j * np.pi * e / m) * np.asarray(d)[None, :]
+        return Phi
+    if sketch == "sub":
+        r = np.asarray(draws, dtype=np.int64)
+        Phi[np.arange(order), r - 1] = 1.0
+        return Phi
+    if sketch == "sprn":
+        r, sc = draws
+        o = 0
+        for p in range(order):
+            q = (m - p - 1) // order + 1 if m - p - 1 >= 0 else 0
+            Phi[p, np.asarray(r[o:o + q]) - 1] = sc[o:o + q]
+            o += q
+        return Phi
+    raise ValueError(sketch)
+
+
+def zidfact(trans: str, A, opts: LRAOptions = None, zrand=None, **kw):
+    """idfact (src/id.jl:434-447) for a complex A: the adaptive sketch loop
+    (src/sketch.jl:223-240 and siblings) with injected complex sketches (zrand.phi(sketch,
+    round, m, order) -> dense order x m matrix), geqp3_adap over OpenBLAS zlaqps and
+    T = R11 \\ R12 (scipy's solve_triangular, LAPACK ztrtrs).  sketch = none factors op(A)."""
+    import scipy.linalg as sla
+    opts = (opts or LRAOptions()).copy(**kw)
+    chkopts(opts)
+    A = np.asarray(A, dtype=np.complex128)
+    X = A if trans == "n" else A.conj().T
+    P, N = X.shape
+    rounds = 1
+    if opts.sketch == "none":
+        Bf, p, tau, k = zgeqp3_adap(X, opts.rank, opts.atol, opts.rtol, opts.nb)
+    else:
+        sampfn = {"randn": opts.sketchfact_randn_samp, "srft": opts.sketchfact_srft_samp,
+                  "sub": opts.sketchfact_sub_samp, "sprn": (1, 0)}[opts.sketch]
+        adap = opts.sketchfact_adap or opts.rank < 0
+        target = opts.nb if adap else opts.rank
+        rnd = 0
+        while True:
+            order = samp(sampfn, target)
+            B = zrand.phi(opts.sketch, rnd, P, order) @ X
+            Bf, p, tau, k = zgeqp3_adap(B, opts.rank, opts.atol, opts.rtol, opts.nb)
+            rounds = rnd + 1
+            if not adap or k < target:
+                break
+            target *= 2
+            rnd += 1
+    R = np.triu(Bf[:k, :])
+    T = sla.solve_triangular(R[:, :k], R[:, k:]) if 0 < k < N else np.zeros((k, N - k),
+                                                                            np.complex128)
+    return IDPackedV(p[:k].copy(), p[k:].copy(), T, rounds, None)
+
+
 def maxdet_t_blas(R: np.ndarray) -> np.ndarray:
     """T = R[:,1:k] \\ R[:,k+1:n] through OpenBLAS dtrsm (the BLAS call behind
     ldiv!(UpperTriangular(R1), R2) at src/pqr.jl:441; called only): the CPU-baseline form of
@@ -876,6 +1001,12 @@ def hilbert(m: int, n: int = None) -> np.ndarray:
     i = np.arange(1, m + 1)[:, None]
     j = np.arange(1, n + 1)[None, :]
     return np.asfortranarray(1.0 / (i + j - 1))
+
+
+def fourier(x, y) -> np.ndarray:
+    """matrixlib(:fourier, x, y) (src/matrixlib.jl:25-33): A[i,j] = exp(-2 pi i x_i y_j), the
+    complex test matrix of test/id.jl:7,24."""
+    return np.asfortranarray(np.exp(-2j * np.pi * np.outer(x, y)))
 
 
 def fourier_real(x, y) -> np.ndarray:

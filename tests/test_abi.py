@@ -61,8 +61,12 @@ def test_options_follow_the_element_type(lra):
     assert lra._eltype(torch.zeros(2, 2, dtype=torch.float32)) == "f32"
     assert lra._eltype(np.zeros((2, 2))) == "f64"
     for z in (np.zeros((2, 2), np.complex128), torch.zeros(2, 2, dtype=torch.complex64)):
-        with pytest.raises(TypeError):
+        with pytest.raises(TypeError):  # never truncated to the real part on a real path
             lra.to_device(z)
+    assert lra._eltype(np.zeros((2, 2), np.complex128)) == "c128"
+    assert lra._eltype(torch.zeros(2, 2, dtype=torch.complex64)) == "c64"
+    assert lra.LRAOptions.for_type("c64").rtol == 5 * e32
+    assert lra.LRAOptions.for_type("c128") == lra.LRAOptions()
 
 
 def test_options_validation(lra):

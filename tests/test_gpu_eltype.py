@@ -66,7 +66,11 @@ def test_psvd_float32(gpu, mats):
     assert np.linalg.norm(A64 - host(F.matrix()), 2) < 100 * rtol * np.linalg.norm(A64, 2)
 
 
-def test_complex_rejected(gpu, mats):
+def test_complex_on_real_only_paths_rejected(gpu, mats):
+    """Complex matrices run through idfact/id; the real-only factorizations refuse them
+    rather than dropping the imaginary part."""
     M, _ = mats
-    with pytest.raises(TypeError):
-        gpu.idfact(M.astype(np.complex128))
+    Z = M.astype(np.complex128) * (1 + 0.5j)
+    for f in (gpu.pqrfact, gpu.psvdfact, gpu.curfact, gpu.snorm):
+        with pytest.raises(TypeError):
+            f(Z)

@@ -305,3 +305,97 @@ def test_oracle_cur_build_exact_rank():
     W = A[np.ix_(np.asarray(rows) - 1, np.asarray(cols) - 1)]
     assert np.allclose(Wp, np.linalg.pinv(W), rtol=1e-8, atol=1e-10 * np.abs(Wp).max())
     assert np.abs(C @ Wp @ R - A).max() <= 1e-9 * np.abs(A).max()
+
+
+# ---------------------------------------------------------------------------------------
+# complex element type: zlaqps loop, complex sketches, complex idfact
+
+def _zlowrank(m, n, r, seed, decay=None):
+    rng = np.random.default_rng(seed)
+    U = rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))
+    V = rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r))
+    s = np.ones(r) if decay is None else decay ** np.arange(r)
+    return np.asfortranarray((U * s) @ V.conj().T)
+
+
+def test_zgeqp3_adap_full_rank_matches_zgeqp3(orc):
+    """The restated loop over OpenBLAS zlaqps against LAPACK zgeqp3 (scipy): same pivots and
+    |R| on a full-rank complex matrix, and B[:, p] = Q R."""
+    rng = np.random.default_rng(7)
+    m, n = 60, 40
+    B = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))) * np.exp(
+        -0.15 * np.arange(n))
+    Bf, p, tau, k = orc.zgeqp3_adap(B, -1, 0.0, 0.0, 8)
+    assert k == n
+    Q, R, P = sla.qr(B, pivoting=True, mode="economic")
+    assert np.array_equal(p - 1, P)
+    assert np.allclose(np.abs(np.diag(Bf)), np.abs(np.diag(R)), rtol=1e-10)
+    Rt = np.triu(Bf[:n, :])
+    Qo = np.linalg.qr(B[:, p - 1])[0]
+    assert np.allclose(np.abs(Qo.conj().T @ B[:, p - 1]), np.abs(Rt), atol=1e-10)
+
+
+def test_zgeqp3_adap_exact_rank_and_rank_cap(orc):
+    B = _zlowrank(80, 70, 13, 3)
+    _, p, _, k = orc.zgeqp3_adap(B, -1, 0.0, 1e-12, 32)
+    assert k == 13
+    _, p, _, k = orc.zgeqp3_adap(B, 5, 0.0, 1e-12, 32)
+    assert k == 5
+
+
+def test_zsketch_phi_closed_forms(orc):
+    rng = np.random.default_rng(5)
+    m, l = 96, 11
+    x = rng.standard_normal((m, 3)) + 1j * rng.standard_normal((m, 3))
+    g = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    d = g / np.abs(g)
+    idx = rng.integers(1, m + 1, size=l)
+    Phi = orc.zsketch_phi("srft", m, l, (d, idx))
+    F = np.fft.fft(d[:, None] * x, axis=0)
+    assert np.allclose(Phi @ x, F[idx - 1], atol=1e-12 * np.sqrt(m))
+    r = rng.integers(1, m + 1, size=l)
+    assert np.array_equal(orc.zsketch_phi("sub", m, l, r) @ x, x[r - 1])
+    perm = rng.permutation(m) + 1
+    sc = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    Ps = orc.zsketch_phi("sprn", m, l, (perm, sc))
+    assert np.count_nonzero(Ps) == m and np.array_equal(np.sort(np.nonzero(Ps)[1]), np.arange(m))
+    assert [np.count_nonzero(Ps[i]) for i in range(l)] == [(m - i - 1) // l + 1 for i in range(l)]
+
+
+class _ZRand:
+    def __init__(self, seed):
+        self.rng = np.random.default_rng(seed)
+
+    def phi(self, sketch, rnd, m, order):
+        r = self.rng
+        if sketch == "randn":
+            return r.standard_normal((order, m)) + 1j * r.standard_normal((order, m))
+        if sketch == "srft":
+            g = r.standard_normal(m) + 1j * r.standard_normal(m)
+            return _orc().zsketch_phi("srft", m, order, (g / np.abs(g), r.integers(1, m + 1, order)))
+        if sketch == "sub":
+            return _orc().zsketch_phi("sub", m, order, r.integers(1, m + 1, order))
+        return _orc().zsketch_phi("sprn", m, order, (r.permutation(m) + 1,
+                                                     r.standard_normal(m) + 1j * r.standard_normal(m)))
+
+
+def _orc():
+    import oracle
+    return oracle
+
+
+@pytest.mark.parametrize("sk", ["randn", "srft", "sub", "sprn", "none"])
+def test_zidfact_residual_and_rank(orc, sk):
+    """test/id.jl's bound for ComplexF64 (rtol 5 eps): |A[:,rd] - A[:,sk] T| < 100 rtol |A|,
+    and an exact rank-13 matrix gives k = 13 at rtol 1e-12."""
+    rng = np.random.default_rng(1)
+    A = orc.fourier(rng.random(128), rng.random(64))   # test/id.jl:7
+    nrm = np.linalg.norm(A)
+    for trans in ("n", "c"):
+        V = orc.zidfact(trans, A, zrand=_ZRand(3), sketch=sk)
+        X = A if trans == "n" else A.conj().T
+        E = X[:, V.rd - 1] - X[:, V.sk - 1] @ V.T
+        assert np.linalg.norm(E) < 100 * 5 * np.finfo(float).eps * nrm, (sk, trans, V.k)
+    B = _zlowrank(90, 70, 13, 4)
+    V = orc.zidfact("n", B, zrand=_ZRand(4), sketch=sk, rtol=1e-12)
+    assert V.k == 13
