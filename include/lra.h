@@ -400,7 +400,7 @@ int lra_gesvj_f64(lra_ctx* ctx, int64_t m, int64_t n, double* X, int64_t ldx, do
  * Complex matrices are column-major with interleaved (re, im) doubles; leading dimensions
  * count complex elements.  These replace the reference's paths for T <: Complex
  * (src/id.jl:434-447 with crandn, src/util.jl:4-11; the complex SRFT, src/sketch.jl:345-376,
- * 452-472; zlaqps through src/lapack.jl:117-139).  The QRCP runs in HBM (three launches per
+ * 452-472; zlaqps through src/lapack.jl:117-139).  The QRCP runs in HBM (two launches per
  * pivot step); complex power iterations, maxdet swaps and the complex partial QR / SVD /
  * eigen factorizations are not provided. */
 

@@ -20,7 +20,9 @@
 #include <cstdlib>
 #include <map>
 #include <type_traits>
+#include <condition_variable>
 #include <mutex>
+#include <set>
 #include <tuple>
 #include <vector>
 
@@ -622,7 +624,11 @@ hipError_t launch_prec(int prec, int id, const GemmDev& d, int blocks, hipStream
 int bk_of(int prec) { return prec == LRA_PREC_F64 ? 16 : 32; }
 
 // Autotuner: the first call for a shape class times every candidate (config, splits) on the
-// live operands and caches the fastest; later calls reuse the choice.
+// live operands and caches the fastest; later calls reuse the choice.  The cache is
+// process-wide and filled once per class: a call that finds the class being tuned by another
+// thread (another context, e.g. a serving worker) waits for that choice instead of timing on
+// its own, so every product of a class in the process uses one configuration and gives the
+// same bits in any order and under any overlap.
 struct TuneKey {
   int prec, transA, transB, mb, nb, kb;
   bool operator<(const TuneKey& o) const {
@@ -632,7 +638,9 @@ struct TuneKey {
 };
 struct TuneVal { int cfg, splits; };
 std::mutex g_tune_mu;
+std::condition_variable g_tune_cv;
 std::map<TuneKey, TuneVal> g_tune;
+std::set<TuneKey> g_tuning;  // classes being tuned right now
 int g_autotune = -1;
 
 int bucket(i64 x) {  // power-of-two bucket
@@ -816,11 +824,26 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
   TuneVal choice;
   bool have = false;
   {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) { choice = it->second; have = true; }
+    std::unique_lock<std::mutex> lk(g_tune_mu);
+    for (;;) {
+      auto it = g_tune.find(key);
+      if (it != g_tune.end()) { choice = it->second; have = true; break; }
+      if (!g_tuning.count(key)) { g_tuning.insert(key); break; }  // this call tunes it
+      g_tune_cv.wait(lk);
+    }
   }
   if (!have) {
+    // (on an error return the class is released so that a later call tunes it)
+    struct Release {
+      const TuneKey& k;
+      bool armed = true;
+      ~Release() {
+        if (!armed) return;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuning.erase(k);
+        g_tune_cv.notify_all();
+      }
+    } release{key};
     if (!g_autotune || g.beta != 0.0 || (g.M * g.N * (g.K > 0 ? g.K : 1)) < (1ll << 24)) {
       choice = heuristic(g, ctx->num_cu);
     } else {
@@ -864,8 +887,13 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
       hipEventDestroy(e0);
       hipEventDestroy(e1);
     }
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = choice;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = choice;
+      g_tuning.erase(key);
+      release.armed = false;
+    }
+    g_tune_cv.notify_all();
   }
   return run(ctx, g, choice.cfg, choice.splits);
 }

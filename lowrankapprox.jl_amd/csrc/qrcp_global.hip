@@ -2,12 +2,12 @@
 // dlaqps steps of src/lapack.jl:117-139) for sketches too large for the register-resident
 // kernel of qrcp.hip (e.g. order 1032 or 2056 on thousands of columns).
 //
-// The sketch stays in HBM (column-major, in place, LAPACK layout) and each pivot step is three
+// The sketch stays in HBM (column-major, in place, LAPACK layout) and each pivot step is two
 // launches on the context's stream, with every decision kept on the device:
-//   qg_pick   one workgroup: dlaqps block bookkeeping and the stop rule for the step just
-//             finished, then the first position of the largest partial norm (idamax);
-//   qg_refl   one workgroup: LAPACK's column swap (data, jpvt, vn1, vn2), dlarfg on the pivot
-//             column (R's diagonal in place, v below it, a contiguous copy of v);
+//   qg_step   one workgroup: dlaqps block bookkeeping and the stop rule for the step just
+//             finished, the first position of the largest partial norm (idamax), LAPACK's
+//             column swap (data, jpvt, vn1, vn2), dlarfg on the pivot column (R's diagonal in
+//             place, v below it, a contiguous copy of v);
 //   qg_apply  one wave per trailing column: H applied, LAWN 176 downdate; a column whose
 //             downdate fails the tol3z test gets its norm recomputed and ends the block.
 // Once the stop rule fires, the remaining launches return at once.  The arithmetic and the
@@ -68,11 +68,11 @@ __global__ __launch_bounds__(GT) void qg_init(int n, const double* vn1, double a
   }
 }
 
-// step s: bookkeeping of step s-1, then the pivot of step s
-__global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, int stop_after,
-                                              const double* vn1, const double* diag,
-                                              GState* st) {
-  __shared__ int s_done;
+// step s: bookkeeping of step s-1, then the pivot of step s; returns the pivot position, or
+// -1 (for every thread of the block) once the factorization has stopped
+__device__ int pick_body(int s, int n, int kmax, int nb, int stop_after, const double* vn1,
+                         const double* diag, GState* st) {
+  __shared__ int s_done, s_pvt;
   __shared__ double sv[GT / 64];
   __shared__ int sp[GT / 64];
   if (threadIdx.x == 0) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, in
     s_done = done;
   }
   __syncthreads();
-  if (s_done) return;
+  if (s_done) return -1;
   // idamax over positions s..n-1: largest vn1, then lowest position
   double bv = -1.0;
   int bp = INT_MAX;
@@ -133,21 +133,33 @@ __global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, in
       // no candidate (NaN norms): stop here rather than pivot on a column that is not there
       st->k = s;
       st->done = 1;
+      s_done = 1;
       cp = s;
     }
     st->pvt = cp;
     st->s = s;
+    s_pvt = cp;
   }
+  __syncthreads();
+  return s_done ? -1 : s_pvt;
 }
 
-// step s: swap columns s and pvt, build H(s) (dlarfg) in place, copy v
-__global__ __launch_bounds__(GT) void qg_refl(int s, int m, double* __restrict__ B, i64 ldb,
+__global__ __launch_bounds__(GT) void qg_pick(int s, int n, int kmax, int nb, int stop_after,
+                                              const double* vn1, const double* diag,
+                                              GState* st) {
+  (void)pick_body(s, n, kmax, nb, stop_after, vn1, diag, st);
+}
+
+// step s: the pivot (pick_body), then columns s and pvt swapped and H(s) built (dlarfg) in
+// place, v copied
+__global__ __launch_bounds__(GT) void qg_step(int s, int m, int n, int kmax, int nb,
+                                              int stop_after, double* __restrict__ B, i64 ldb,
                                               i64* jpvt, double* vn1, double* vn2, double* tau,
-                                              double* diag, double* v, const GState* st) {
+                                              double* diag, double* v, GState* st) {
   __shared__ double sw[GT / 64];
   __shared__ double s_alpha;
-  if (st->done) return;
-  const int pvt = st->pvt;
+  const int pvt = pick_body(s, n, kmax, nb, stop_after, vn1, diag, st);
+  if (pvt < 0) return;
   double* cs = B + (i64)s * ldb;
   if (pvt != s) {
     double* cp = B + (i64)pvt * ldb;
@@ -218,7 +230,11 @@ __global__ __launch_bounds__(GT) void qg_refl(int s, int m, double* __restrict__
   }
 }
 
-// step s: columns s+1..n-1 (one wave each): apply H(s), downdate the partial norms
+// step s: columns s+1..n-1 (one wave each): apply H(s), downdate the partial norms.
+// RPL > 0: the column's rows s..m-1 fit RPL values per lane, so they are read once into
+// registers, used for the dot product and updated there (one read and one write of the
+// trailing matrix per step); RPL == 0 streams the column twice.
+template <int RPL>
 __global__ __launch_bounds__(256) void qg_apply(int s, int m, int n, int lastrk,
                                                 double* __restrict__ B, i64 ldb,
                                                 const double* __restrict__ v,
@@ -227,21 +243,51 @@ __global__ __launch_bounds__(256) void qg_apply(int s, int m, int n, int lastrk,
   if (st->done) return;
   const int lane = threadIdx.x & 63;
   const int j = s + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= n) return;
   const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
-  double* c = B + (i64)j * ldb;
-  double w = 0.0;
-  for (int i = s + lane; i < m; i += 64) w += v[i] * c[i];
-  w = group_sum(w, 64);
-  const double tw = tau[s] * w;
-  // update; each lane also sums the squares of its updated rows below s (the recompute norm)
-  // and lane 0 keeps the new R(s, j), so nothing is read back from memory
   double ss = 0.0, x = 0.0;
-  for (int i = s + lane; i < m; i += 64) {
-    const double y = c[i] - tw * v[i];
-    c[i] = y;
-    if (i == s) x = y;
-    else ss += y * y;
+  if constexpr (RPL > 0) {
+    // v(s .. s + 64 RPL - 1) staged in LDS once per block (zero past m)
+    __shared__ double sv[64 * RPL];
+    for (int t = threadIdx.x; t < 64 * RPL; t += 256) sv[t] = s + t < m ? v[s + t] : 0.0;
+    __syncthreads();
+    if (j >= n) return;
+    double* c = B + (i64)j * ldb;
+    double cr[RPL];
+    double w = 0.0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int i = s + lane + 64 * r;
+      cr[r] = i < m ? c[i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) w += sv[lane + 64 * r] * cr[r];
+    w = group_sum(w, 64);
+    const double tw = tau[s] * w;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int i = s + lane + 64 * r;
+      const double y = cr[r] - tw * sv[lane + 64 * r];
+      if (i < m) {
+        c[i] = y;
+        if (i == s) x = y;
+        else ss += y * y;
+      }
+    }
+  } else {
+    if (j >= n) return;
+    double* c = B + (i64)j * ldb;
+    double w = 0.0;
+    for (int i = s + lane; i < m; i += 64) w += v[i] * c[i];
+    w = group_sum(w, 64);
+    const double tw = tau[s] * w;
+    // update; each lane also sums the squares of its updated rows below s (the recompute
+    // norm) and lane 0 keeps the new R(s, j), so nothing is read back from memory
+    for (int i = s + lane; i < m; i += 64) {
+      const double y = c[i] - tw * v[i];
+      c[i] = y;
+      if (i == s) x = y;
+      else ss += y * y;
+    }
   }
   x = __shfl(x, 0, 64);
   double a = vn1[j];
@@ -312,14 +358,16 @@ __global__ __launch_bounds__(256) void zq_norms(int m, int n, const zc* __restri
   }
 }
 
-// step s: swap columns s and pvt, zlarfg on column s in place, copy v
-__global__ __launch_bounds__(GT) void zq_refl(int s, int m, zc* __restrict__ B, i64 ldb,
+// step s: the pivot (pick_body), then columns s and pvt swapped, zlarfg on column s in
+// place, v copied
+__global__ __launch_bounds__(GT) void zq_step(int s, int m, int n, int kmax, int nb,
+                                              int stop_after, zc* __restrict__ B, i64 ldb,
                                               i64* jpvt, double* vn1, double* vn2, zc* tau,
-                                              double* diag, zc* v, const GState* st) {
+                                              double* diag, zc* v, GState* st) {
   __shared__ double sw[GT / 64];
   __shared__ zc s_alpha;
-  if (st->done) return;
-  const int pvt = st->pvt;
+  const int pvt = pick_body(s, n, kmax, nb, stop_after, vn1, diag, st);
+  if (pvt < 0) return;
   zc* cs = B + (i64)s * ldb;
   if (pvt != s) {
     zc* cp = B + (i64)pvt * ldb;
@@ -479,11 +527,13 @@ int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, in
   hipLaunchKernelGGL(qg_init, dim3(1), dim3(GT), 0, s0, n, vn1, atol, rtol, kmax, nb, st);
   LRA_HIP(hipGetLastError());
   for (int s = 0; s <= kmax; ++s) {
-    hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, kmax, nb, stop_after, vn1, diag,
-                       st);
-    if (s == kmax) break;
-    hipLaunchKernelGGL(zq_refl, dim3(1), dim3(GT), 0, s0, s, m, B, ldb, jpvt, vn1, vn2, zt,
-                       diag, v, st);
+    if (s == kmax) {
+      hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, kmax, nb, stop_after, vn1,
+                         diag, st);
+      break;
+    }
+    hipLaunchKernelGGL(zq_step, dim3(1), dim3(GT), 0, s0, s, m, n, kmax, nb, stop_after, B, ldb,
+                       jpvt, vn1, vn2, zt, diag, v, st);
     if (s + 1 < n)
       hipLaunchKernelGGL(zq_apply, dim3((unsigned)((n - s - 1 + 3) / 4)), dim3(256), 0, s0, s,
                          m, n, lastrk, B, ldb, v, zt, vn1, vn2, st);
@@ -530,14 +580,29 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
                      st);
   LRA_HIP(hipGetLastError());
   for (int s = 0; s <= q.kmax; ++s) {
-    hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, q.kmax, q.nb, q.stop_after, vn1,
-                       diag, st);
-    if (s == q.kmax) break;
-    hipLaunchKernelGGL(qg_refl, dim3(1), dim3(GT), 0, s0, s, m, q.A, q.lda, q.jpvt, vn1, vn2,
-                       q.tau, diag, v, st);
-    if (s + 1 < n)
-      hipLaunchKernelGGL(qg_apply, dim3((unsigned)((n - s - 1 + 3) / 4)), dim3(256), 0, s0, s,
-                         m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+    if (s == q.kmax) {  // the bookkeeping of the last step
+      hipLaunchKernelGGL(qg_pick, dim3(1), dim3(GT), 0, s0, s, n, q.kmax, q.nb, q.stop_after,
+                         vn1, diag, st);
+      break;
+    }
+    hipLaunchKernelGGL(qg_step, dim3(1), dim3(GT), 0, s0, s, m, n, q.kmax, q.nb, q.stop_after,
+                       q.A, q.lda, q.jpvt, vn1, vn2, q.tau, diag, v, st);
+    if (s + 1 < n) {
+      const dim3 grid((unsigned)((n - s - 1 + 3) / 4));
+      const int rows = m - s;  // rows s..m-1 of each trailing column
+      if (rows <= 64 * 8)
+        hipLaunchKernelGGL(qg_apply<8>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
+                           q.tau, vn1, vn2, st);
+      else if (rows <= 64 * 16)
+        hipLaunchKernelGGL(qg_apply<16>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
+                           q.tau, vn1, vn2, st);
+      else if (rows <= 64 * 32)
+        hipLaunchKernelGGL(qg_apply<32>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
+                           q.tau, vn1, vn2, st);
+      else
+        hipLaunchKernelGGL(qg_apply<0>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
+                           q.tau, vn1, vn2, st);
+    }
     LRA_HIP(hipGetLastError());
     // (every 64 steps, stop enqueueing once the device has stopped)
     if ((s & 63) == 63) {
