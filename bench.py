@@ -227,17 +227,20 @@ def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
 
 
 def serving_pool(torch, count: int, seed: int = 6):
-    """Serving pool matrices: m log-uniform in [256, 8192], square or tall (n log-uniform in
-    [256, m]) with probability 1/2, A = U diag(exp(-r (i-1))) V', r ~ U[0.01, 0.2], U and V
-    Q factors of Gaussian matrices (singular values below exp(-40) sigma_1 dropped)."""
+    """Serving pool matrices of the served workload's distribution: m log-uniform on
+    [256, 8192], square or tall (n log-uniform on [256, m]) with probability 1/2,
+    A = U diag(exp(-r (i-1))) V', r ~ U[0.01, 0.2], U and V Q factors of Gaussian matrices
+    (singular values below exp(-40) sigma_1 dropped); the last entry is 8192 x 8192."""
     rng = np.random.default_rng(seed)
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     pool = []
-    for _ in range(count):
+    for i in range(count):
         m = int(np.rint(256 * 32.0 ** rng.random()))
         n = m if rng.random() < 0.5 else int(np.rint(256 * (m / 256.0) ** rng.random()))
         rate = float(rng.uniform(0.01, 0.2))
+        if i == count - 1:  # the top of the size range is always in the pool
+            m = n = 8192
         r = min(m, n, int(40.0 / rate) + 1)
         U, _ = torch.linalg.qr(torch.randn(m, r, dtype=torch.float64, device="cuda",
                                            generator=g))
@@ -250,8 +253,8 @@ def serving_pool(torch, count: int, seed: int = 6):
     return pool
 
 
-def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 32, pool_size: int = 12,
-                  workers: int = 4, budget_gb: int = 96, seed: int = 6):
+def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 200,
+                  pool_size: int = 16, workers: int = 4, budget_gb: int = 96, seed: int = 6):
     """Serving: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
     1e-10 50/50) on resident matrices; per rate: requests/s completed and p50/p99 latency
     from submission to result (server clock)."""
@@ -291,13 +294,15 @@ def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 32
             out.append({"arrival_rate": lam, "requests": nreq,
                         "completed_per_s": round(nreq / span, 2),
                         "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
-                        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2)})
+                        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+                        "max_ms": round(1e3 * float(lat.max()), 2)})
             print(f"# serving rate {lam}/s: {out[-1]}", file=sys.stderr, flush=True)
     finally:
         srv.close()
     sizes = [tuple(A.shape) for A in pool]
     return {"workload": f"{pool_size} resident matrices {sizes}, idfact/psvdfact 50/50, "
-                        "rtol 1e-6/1e-10 50/50, sigma_i = exp(-r(i-1)), r ~ U[0.01,0.2]",
+                        "rtol 1e-6/1e-10 50/50, sigma_i = exp(-r(i-1)), r ~ U[0.01,0.2], "
+                        f"{nreq} Poisson arrivals per rate",
             "workers": workers, "hbm_budget_gb": budget_gb, "by_rate": out}
 
 

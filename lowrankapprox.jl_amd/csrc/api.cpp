@@ -135,6 +135,16 @@ extern "C" int lra_ctx_create(int device, size_t hbm_budget, lra_ctx** out) {
   return LRA_OK;
 }
 
+// Uncached exchange areas of destroyed contexts, per device.  Uncached memory that kernels
+// have read is never freed before the process ends; a new context takes a pooled area before
+// allocating one, so the areas held are bounded by the contexts alive at once.  A pooled area
+// is zeroed when it is returned and launch generations are process-wide, so no tag written by
+// its earlier owner can match a later reader.
+namespace {
+std::mutex g_uc_mu;
+std::vector<std::pair<void*, size_t>> g_uc_pool[64];
+}  // namespace
+
 extern "C" int lra_ctx_destroy(lra_ctx* c) {
   if (!c) return LRA_OK;
   hipSetDevice(c->device);
@@ -142,7 +152,15 @@ extern "C" int lra_ctx_destroy(lra_ctx* c) {
   lra_spin_forget(c);
   for (int i = 0; i < lra_ctx::kSlots; ++i)
     if (c->ws[i]) hipFree(c->ws[i]);
-  if (c->coop) hipFree(c->coop);  // the uncached area (coop_uc) stays allocated
+  if (c->coop) hipFree(c->coop);
+  if (c->coop_uc && c->device >= 0 && c->device < 64 &&
+      hipMemsetAsync(c->coop_uc, 0, c->coop_uc_size, c->stream) == hipSuccess &&
+      hipStreamSynchronize(c->stream) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_uc_mu);
+    g_uc_pool[c->device].push_back({c->coop_uc, c->coop_uc_size});
+  } else {
+    (void)hipGetLastError();  // (an area that could not be cleared stays unused)
+  }
   if (c->d_err) hipFree(c->d_err);
   if (c->host_pinned) hipHostFree(c->host_pinned);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -209,7 +227,18 @@ int lra_coop_area(lra_ctx* c, size_t bytes, void** out, bool* uncached) {
   if (!c->coop_uc_tried) {
     c->coop_uc_tried = true;
     void* p = nullptr;
-    if (hipExtMallocWithFlags(&p, kUncached, hipDeviceMallocUncached) == hipSuccess) {
+    if (c->device >= 0 && c->device < 64) {
+      std::lock_guard<std::mutex> lk(g_uc_mu);
+      auto& pool = g_uc_pool[c->device];
+      if (!pool.empty()) {
+        c->coop_uc = pool.back().first;
+        c->coop_uc_size = pool.back().second;
+        pool.pop_back();
+      }
+    }
+    if (c->coop_uc) {
+      // (taken from the pool, already zeroed)
+    } else if (hipExtMallocWithFlags(&p, kUncached, hipDeviceMallocUncached) == hipSuccess) {
       c->coop_uc = p;
       c->coop_uc_size = kUncached;
     } else {

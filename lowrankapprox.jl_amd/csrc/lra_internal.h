@@ -55,10 +55,9 @@ struct lra_ctx {
   // once, never freed before the process ends)
   void* coop = nullptr;       // ordinary memory, grown on demand
   size_t coop_size = 0;
-  void* coop_uc = nullptr;    // uncached, fixed size, never freed
+  void* coop_uc = nullptr;    // uncached, fixed size, never freed (pooled per device)
   size_t coop_uc_size = 0;
   bool coop_uc_tried = false;
-  uint32_t qrcp_gen = 0;      // launch generation of the cooperative QRCP (chunk tags)
   int* d_err = nullptr;  // device error word (timeout etc.)
   hipEvent_t spin_ev = nullptr;  // recorded after this context's grid-synchronising kernels
 };
@@ -144,6 +143,8 @@ static inline int lra_wsT(lra_ctx* ctx, int slot, size_t count, T** out) {
   return lra_ws(ctx, slot, count * sizeof(T), (void**)out);
 }
 int lra_coop_area(lra_ctx* ctx, size_t bytes, void** out, bool* uncached);
+// Next launch generation of the tagged exchanges (process-wide, never 0).
+uint32_t lra_next_qrcp_gen();
 int lra_check_device_error(lra_ctx* ctx);
 
 // ---------------------------------------------------------------------------------------

@@ -349,7 +349,7 @@ int factor_panel(lra_ctx* c, i64 m, i64 j0, int nb, double* A, i64 lda, double* 
   d.A = A; d.lda = lda; d.m = (int)m; d.j0 = (int)j0; d.nb = nb; d.G = (int)G; d.tau = tau;
   d.abort_flag = (int*)area;
   d.xchg = (u64x2*)((char*)area + 256);
-  d.gen = ++c->qrcp_gen;  // chunk tags of this launch (shared counter with the QRCP)
+  d.gen = lra_next_qrcp_gen();  // chunk tags of this launch (shared counter with the QRCP)
   d.uncached = uc ? 1 : 0;
   d.err = c->d_err;
   if (first) LRA_CHECK(lra_launch_zero_u32(c->stream, (unsigned*)area, 256 / 4));

@@ -28,6 +28,7 @@
 #include "lra_wave.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -782,6 +783,14 @@ bool lra_qrcp_pair_fits(lra_ctx* ctx, int m0, int m1, int n) {
   return pair_shapes(ctx, m0, m1, n, &s0, &s1);
 }
 
+// Launch generations are process-wide: an exchange area passed from a destroyed context to a
+// new one can then never hold a chunk tag that the new owner's launches would accept.
+uint32_t lra_next_qrcp_gen() {
+  static std::atomic<uint32_t> gen{0};
+  uint32_t g = gen.fetch_add(1, std::memory_order_relaxed) + 1;
+  return g ? g : gen.fetch_add(1, std::memory_order_relaxed) + 1;
+}
+
 int lra_qrcp_pair(lra_ctx* ctx, const QrcpCall& q0, const QrcpCall& q1) {
   QrcpShape s0, s1;
   if (q0.mode != 0 || q1.mode != 0 || q0.n != q1.n || q0.kmax <= 0 || q1.kmax <= 0 ||
@@ -806,7 +815,7 @@ int lra_qrcp_pair(lra_ctx* ctx, const QrcpCall& q0, const QrcpCall& q1) {
   d0.cand = (double*)(base + 256 + sb0 + sb1);
   d1.cand = (double*)(base + 256 + sb0 + sb1 + cb0);
   LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + sb0 + sb1) / 4));
-  d0.gen = d1.gen = ++ctx->qrcp_gen;
+  d0.gen = d1.gen = lra_next_qrcp_gen();
   hipError_t e;
   {
     SpinSection gate(ctx);
@@ -861,7 +870,7 @@ int lra_qrcp(lra_ctx* ctx, const QrcpCall& q) {
   // zero the header and every slot (a stale slot tag of an earlier launch must never match);
   // chunk tags carry the launch generation instead
   LRA_CHECK(lra_launch_zero_u32(ctx->stream, (unsigned*)base, (256 + slot_bytes) / 4));
-  d.gen = ++ctx->qrcp_gen;
+  d.gen = lra_next_qrcp_gen();
   hipError_t e;
   if (sh.G > 1) {
     SpinSection gate(ctx);

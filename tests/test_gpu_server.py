@@ -51,23 +51,25 @@ def test_server_mix_equals_alone(gpu):
         for h, A, (kind, m, n, rate, rtol) in zip(hs, mats, REQUESTS):
             Ah = host(A)
             got = srv.result(h)
+            # the same bits alone and in the mix: every kernel's reduction order is fixed
+            # and the GEMM configuration of a shape class is chosen once per process
             if kind == "idfact":
                 alone = gpu.idfact(A, rtol=rtol, seed=3)
                 assert got.k == alone.k, (m, n)
-                e, ea = id_err(Ah, got), id_err(Ah, alone)
-                assert e <= 10 * max(ea, 1e-14 * np.linalg.norm(Ah, 2))
-                if np.array_equal(host(got.sk), host(alone.sk)) and got.T.numel():
-                    Tg, Ta = host(got.T), host(alone.T)
-                    assert np.abs(Tg - Ta).max() <= 1e-6 * max(1.0, np.abs(Ta).max())
+                assert np.array_equal(host(got.sk), host(alone.sk))
+                assert np.array_equal(host(got.rd), host(alone.rd))
+                assert np.array_equal(host(got.T), host(alone.T)), (m, n)
+                e = id_err(Ah, got)
+                assert e <= 1e3 * rtol * np.linalg.norm(Ah, 2)
             else:
                 alone = gpu.psvdfact(A, rtol=rtol, seed=3)
                 assert got.k == alone.k, (m, n)
-                Sg, Sa = host(got.S), host(alone.S)
-                assert np.abs(Sg - Sa).max() <= 1e-9 * Sa[0]
+                assert np.array_equal(host(got.S), host(alone.S)), (m, n)
+                assert np.array_equal(host(got.U), host(alone.U))
+                assert np.array_equal(host(got.Vt), host(alone.Vt))
+                Sg = host(got.S)
                 R = host(got.U) @ np.diag(Sg) @ host(got.Vt)
-                Ra = host(alone.U) @ np.diag(Sa) @ host(alone.Vt)
-                ea = np.linalg.norm(Ah - Ra, 2)
-                assert np.linalg.norm(Ah - R, 2) <= 10 * max(ea, 1e-14 * Sa[0])
+                assert np.linalg.norm(Ah - R, 2) <= 1e3 * rtol * Sg[0]
             srv.release(h)
         assert srv.stats() == {"queued": 0, "running": 0, "bytes_held": 0}
     finally:
@@ -103,3 +105,32 @@ def test_server_refuses_oversized(gpu):
             srv.submit(A, "psvdfact")
     finally:
         srv.close()
+
+
+def test_server_cycles_reuse_exchange_areas(gpu):
+    """Servers created and closed one after another: each worker context's uncached exchange
+    area (64 MiB, never freed) passes to the next context on the device, so device memory
+    stays flat over the cycles, and every cycle returns the same bits."""
+    import torch
+    A = gpu.to_device(decaying(2000, 1000, 0.05, 5))
+    ref = gpu.idfact(A, rtol=1e-10, seed=3)
+    free = []
+    for cycle in range(6):
+        srv = gpu.Server(hbm_budget=8 << 30, workers=4)
+        try:
+            hs = [srv.submit(A, "idfact", rtol=1e-10, seed=3) for _ in range(4)]
+            for h in hs:
+                assert srv.wait(h, 60.0) == "done"
+                got = srv.result(h)
+                assert got.k == ref.k
+                assert np.array_equal(host(got.sk), host(ref.sk))
+                assert np.array_equal(host(got.T), host(ref.T))
+                srv.release(h)
+        finally:
+            srv.close()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        free.append(torch.cuda.mem_get_info()[0])
+        print(f"cycle {cycle}: free {free[-1] >> 20} MiB", flush=True)
+    # without reuse, five more cycles of four workers would strand 5 x 4 x 64 MiB
+    assert free[0] - min(free[1:]) <= 128 << 20, free
