@@ -19,6 +19,7 @@ import json
 import os
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -387,27 +388,42 @@ def profile_kernels(lra, torch, n: int):
     print(f"# profiled: copy 1 GiB, qrcp 520x{n} (k={k}), batched 4096 blocks", file=sys.stderr)
 
 
+FACTOR_NAMES = ("idfact", "psvdfact", "pqrfact", "qrcp_hbm")
+
+
 def profile_factor(lra, torch, n: int):
     """Launch sequence for a `rocprofv3 --kernel-trace` run of the dense factorizations on the
     n x n workload matrix: a warm-up of each (workspaces grow, GEMM shapes tune), then one
-    psvdfact and one pqrfact (rtol 1e-10), each preceded by a one-element torch add that
-    marks where it starts in the trace (`--summarize-trace` splits on these)."""
+    idfact, one psvdfact and one pqrfact (rtol 1e-10), and the pivoted QR in HBM of a
+    2056 x n Gaussian sketch of the matrix (rtol 1e-10; too tall for the register-resident
+    kernel), each preceded by a one-element torch add that marks where it starts in the trace
+    (`--summarize-trace` splits on these)."""
     A, _ = make_matrix(torch, n)
-    lra.psvdfact(A, rtol=1e-10)
-    lra.pqrfact(A, rtol=1e-10)
+    B = lra.sketch("left", "n", A, 2056, seed=3)
+
+    def qrcp_hbm(seed=None, rtol=1e-10):
+        jp, tau, k = lra.geqp3_adap(B.clone(), -1, 0.0, rtol, 32)
+        return types.SimpleNamespace(k=k)
+
+    def run(name, **kw):
+        return qrcp_hbm(**kw) if name == "qrcp_hbm" else getattr(lra, name)(A, **kw)
+
+    for name in FACTOR_NAMES:
+        for _ in range(2):
+            run(name, rtol=1e-10)
     mark = torch.ones(4, device="cuda")
     torch.cuda.synchronize()
-    for name in ("psvdfact", "pqrfact"):
+    for name in FACTOR_NAMES:
         mark.add_(1.0)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        F = getattr(lra, name)(A, rtol=1e-10, seed=5)
+        F = run(name, rtol=1e-10, seed=5)
         torch.cuda.synchronize()
         print(f"# profiled {name}: k={F.k} {1e3 * (time.perf_counter() - t):.2f} ms",
               file=sys.stderr, flush=True)
 
 
-def summarize_trace(path: str, names=("psvdfact", "pqrfact")):
+def summarize_trace(path: str, names=FACTOR_NAMES):
     """Per-kernel totals of the sections of a rocprofv3 kernel-trace CSV that follow the
     marker kernels of `--profile-factor` (torch's elementwise add), as a markdown table."""
     import csv
@@ -564,8 +580,9 @@ def main():
                          "(calibration), the final-round pivoted QR of the 8192^2 workload, "
                          "and one batched launch of the 4096-block workload")
     ap.add_argument("--profile-factor", action="store_true",
-                    help="only launch, for a rocprofv3 kernel-trace run: psvdfact and pqrfact "
-                         "of the workload matrix after a warm-up, each behind a marker kernel")
+                    help="only launch, for a rocprofv3 kernel-trace run: idfact, psvdfact, "
+                         "pqrfact of the workload matrix and the HBM pivoted QR of a 2056-row "
+                         "sketch of it, after a warm-up, each behind a marker kernel")
     ap.add_argument("--summarize-trace", metavar="CSV",
                     help="print per-kernel totals of a --profile-factor kernel trace and exit")
     args = ap.parse_args()

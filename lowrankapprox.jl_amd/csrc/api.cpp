@@ -356,6 +356,28 @@ static int sketch_gaussian_impl(lra_ctx* c, int side, int trans, i64 m, i64 n, c
   return lra_gemm(c, g);
 }
 
+// The Gaussian sketches of two rounds in one GEMM: C ((ell0 + ell1) x q, ld ell0 + ell1) =
+// [Omega_s0; Omega_s1] * op(A), rows 0..ell0-1 the round of Philox stream s0 and the rest
+// the round of stream s1 (each Omega is the one its round draws alone).  One tall product
+// instead of two short ones, whose M tiles leave the MFMA units part-idle.
+static int sketch_gaussian_left2(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 lda,
+                                 i64 ell0, i64 ell1, uint64_t seed, uint64_t s0, uint64_t s1,
+                                 double* C) {
+  const i64 p = trans == LRA_TRANS_N ? m : n;
+  const i64 q = trans == LRA_TRANS_N ? n : m;
+  const i64 L = ell0 + ell1, ldw = (p + 1) / 2 * 2;
+  double* w = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_OMEGA, (size_t)ldw * L, &w));
+  LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, seed, s0, ell0, p, w, ldw));
+  LRA_CHECK(lra_launch_gaussian_fill_t(c->stream, seed, s1, ell1, p, w + ell0 * ldw, ldw));
+  GemmArgs g;
+  g.alpha = 1.0; g.beta = 0.0; g.C = C; g.ldc = L; g.bgather = nullptr; g.prec = LRA_PREC_F64;
+  g.M = L; g.N = q; g.K = p;
+  g.A = w; g.lda = ldw; g.transA = 1;
+  g.B = A; g.ldb = lda; g.transB = trans == LRA_TRANS_N ? 0 : 1;
+  return lra_gemm(c, g);
+}
+
 extern "C" int lra_sketch_draw(lra_ctx* c, int sketch, uint64_t seed, uint64_t stream_id,
                                int64_t m, int64_t ell, double* d, int64_t* idx) {
   LRA_ARG(c != nullptr, 1);
@@ -512,10 +534,14 @@ static double prec_floor(int prec) {
 //     the first round that cannot is recomputed in FP64 with the same Omega, and every later
 //     round stays FP64, so the accepted factorization is exactly the FP64 drop-in's.
 // `make(ell, stream, prec, B)` writes the ell x N sketch of round `stream` into B (ld = ell).
+// `make2(ell0, ell1, stream0, stream1, C)`, when given, writes two rounds' FP64 sketches as
+// the row blocks of one (ell0 + ell1) x N matrix C (ld ell0 + ell1) for a pair of rounds.
 using SketchFn = std::function<int(i64, uint64_t, int, double*)>;
+using SketchFn2 = std::function<int(i64, i64, uint64_t, uint64_t, double*)>;
 
 static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opts* o,
-                          uint64_t stream_base, SketchFactOut* out, bool pair_ok = true) {
+                          uint64_t stream_base, SketchFactOut* out, bool pair_ok = true,
+                          const SketchFn2& make2 = nullptr) {
   const bool gauss = o->sketch == LRA_SKETCH_RANDN;
   int prec = gauss ? o->precision : LRA_PREC_F64;
   const bool adap = o->sketchfact_adap || o->rank < 0;
@@ -548,13 +574,23 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
           LRA_CHECK(lra_wsT(c, h ? WS_JPVT2 : WS_JPVT, (size_t)std::max<i64>(N, 1), &jp[h]));
           LRA_CHECK(lra_wsT(c, h ? WS_TAU2 : WS_TAU, (size_t)std::max<i64>(std::min(ells[h], N), 1), &tu[h]));
         }
+        // merged: both sketches as the row blocks of one matrix (one GEMM), factored in place
+        double* Cp = nullptr;
+        const i64 L = ell + ell2;
+        if (make2) {
+          LRA_CHECK(lra_wsT(c, WS_SKETCHP, (size_t)L * std::max<i64>(N, 1), &Cp));
+          LRA_CHECK(make2(ell, ell2, stream_base + (uint64_t)rnd, stream_base + (uint64_t)rnd + 1,
+                          Cp));
+        }
         for (int h = 0; h < 2; ++h) {
-          LRA_CHECK(make(ells[h], stream_base + (uint64_t)(rnd + h), prec, Bs[h]));
+          if (!Cp) LRA_CHECK(make(ells[h], stream_base + (uint64_t)(rnd + h), prec, Bs[h]));
           const i64 l = std::min(ells[h], N);
           const i64 kmax = (o->rank < 0 || o->rank > l) ? l : o->rank;
           QrcpCall& q = qs[h];
           q = QrcpCall{};
-          q.A = Bs[h]; q.lda = ells[h]; q.m = (int)ells[h]; q.n = (int)N; q.offset = 0;
+          q.A = Cp ? Cp + (h ? ell : 0) : Bs[h];
+          q.lda = Cp ? L : ells[h];
+          q.m = (int)ells[h]; q.n = (int)N; q.offset = 0;
           q.kmax = (int)kmax; q.nb = (int)std::min<i64>(o->nb, std::max<i64>(kmax, 1));
           q.mode = 0; q.atol = o->atol; q.rtol = o->rtol; q.jpvt = jp[h]; q.tau = tu[h];
           q.stop_after = (int)std::min<i64>(targets[h], 1 << 30);
@@ -566,6 +602,9 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
           LRA_CHECK(read_scalars(c, dk, hv, 2));
           for (int h = 0; h < 2; ++h) {
             if (hv[h] < targets[h]) {
+              if (Cp)  // the accepted round's factored rows, compact (ld = its order)
+                LRA_CHECK(lra_launch_copy2d(c->stream, ells[h], N, Cp + (h ? ell : 0), L, Bs[h],
+                                            ells[h]));
               out->B = Bs[h]; out->Bsave = nullptr; out->ell = ells[h]; out->N = N;
               out->jpvt = jp[h]; out->tau = tu[h]; out->k = hv[h]; out->rounds = rnd + h + 1;
               out->prec = prec; out->stream_id = stream_base + rnd + h;
@@ -767,7 +806,13 @@ static int sketchfact_left(lra_ctx* c, int trans, const LraSrc& s, const lra_opt
     if (niter > 0) LRA_CHECK(power_iterate(c, trans, s, ell, B, niter, herm));
     return LRA_OK;
   };
-  return sketchfact_run(c, N, make, o, stream_base, out, !s.op());
+  SketchFn2 make2 = nullptr;
+  if (!s.op() && o->sketch == LRA_SKETCH_RANDN && niter == 0)
+    make2 = [=](i64 ell0, i64 ell1, uint64_t s0, uint64_t s1, double* C) -> int {
+      return sketch_gaussian_left2(c, trans, s.m, s.n, s.A, s.lda, ell0, ell1, o->seed, s0, s1,
+                                   C);
+    };
+  return sketchfact_run(c, N, make, o, stream_base, out, !s.op(), make2);
 }
 
 int lra_src_getcols(lra_ctx* c, const LraSrc& s, int trans, const i64* idx, i64 k, double* C,

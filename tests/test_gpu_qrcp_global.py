@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("m,n,rtol", [(40, 300, 1e-12), (72, 3000, 1e-10), (264, 2500, 1e-13),
-                                      (12, 5, 0.0), (33, 700, 1e-6), (700, 400, 1e-12)])
+                                      (12, 5, 0.0), (33, 700, 1e-6), (700, 400, 1e-12),
+                                      (1032, 3000, 1e-10), (2056, 2500, 1e-12)])
 def test_global_qrcp_vs_oracle(gpu, orc, monkeypatch, m, n, rtol):
     import torch
     monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
@@ -71,3 +72,38 @@ def test_wide_matrix_beyond_register_kernel(gpu, orc):
     e_g = np.linalg.norm(A[:, host(V_g.rd) - 1] - A[:, host(V_g.sk) - 1] @ host(V_g.T), 2)
     e_o = np.linalg.norm(A[:, V_o.rd - 1] - A[:, V_o.sk - 1] @ V_o.T, 2)
     assert e_g <= 10 * max(e_o, 1e-15)
+
+
+def test_slow_decay_orders_1032_to_2600(gpu, orc):
+    """3000 x 2600 with sigma_i = exp(-0.011 (i-1)): at rtol 1e-10 the rank is about 2090, so
+    the adaptive rounds run to order 2056 (rejected, k reaches its target 2048) and then the
+    whole 2600 columns; the orders 1032, 2056 and 2600 run on the global path.  Checked against
+    the oracle with the same Omegas: rounds, rank (within one step: the diagonal falls by 1.1%
+    per step near the threshold), pivots up to the first near-tie, error within 10x."""
+    rng = np.random.default_rng(8)
+    m, n = 3000, 2600
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.exp(-0.011 * np.arange(n))
+    A = np.asfortranarray((U * s) @ V.T)
+    del U, V
+
+    class DeviceOmega:
+        def gaussian(self, rnd, rows, cols):
+            return np.asfortranarray(gpu.gaussian(rows, cols, 2, rnd).cpu().numpy())
+
+    print("slow decay 3000x2600: gpu idfact", flush=True)
+    V_g = gpu.idfact(A, rtol=1e-10, seed=2)
+    print(f"gpu k={V_g.k} rounds={V_g.rounds}; oracle", flush=True)
+    V_o = orc.idfact("n", A, orc.LRAOptions().copy(rtol=1e-10), rand=DeviceOmega())
+    print(f"oracle k={V_o.k} rounds={V_o.rounds}", flush=True)
+    assert V_g.rounds == V_o.rounds == 8
+    assert abs(V_g.k - V_o.k) <= 1 and 2048 <= V_o.k < n
+    sk_g = host(V_g.sk)
+    first_bad = next((i for i, (a, b) in enumerate(zip(sk_g, V_o.sk)) if a != b), None)
+    if first_bad is not None:
+        assert V_o.gap[first_bad] < 1e-10, (first_bad, V_o.gap[first_bad])
+    e_g = np.linalg.norm(A[:, host(V_g.rd) - 1] - A[:, sk_g - 1] @ host(V_g.T), 2)
+    e_o = np.linalg.norm(A[:, V_o.rd - 1] - A[:, V_o.sk - 1] @ V_o.T, 2)
+    assert e_g <= 10 * max(e_o, 1e-15)
+    assert e_g <= 1e3 * 1e-10
