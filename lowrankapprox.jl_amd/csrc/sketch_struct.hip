@@ -24,6 +24,7 @@ namespace {
 struct cplx {
   double re, im;
 };
+typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
@@ -197,15 +198,16 @@ __device__ __forceinline__ void dft_reg(cplx* x) {
 // p2, k = k1 + L1 k2: pass 1 (a thread per (sequence, p2)) loads its L1 inputs straight from
 // HBM, transforms them in registers and applies w_l^(k1 p2); pass 2 (a thread per (sequence,
 // k1)) transforms the L2 values from LDS; every output slot then accumulates its frequency as
-// in srft_kernel.  The next slab's inputs are loaded while the current slab is transformed.
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations, not for its
-// outstanding global loads (a __syncthreads() fence waits for both), so a slab's prefetch
-// stays in flight across the barriers of the slab before it.  The kernel below exchanges
-// data between threads through LDS only.
+// in srft_kernel.  Two workgroups share a CU (at most 255 VGPRs per lane, 69.9 KB of LDS at
+// l = 256: pass 2 transforms each thread's Y values in place, so frequency k1 + L1 k2 of a
+// sequence sits where pass 1 put row p2 = k2 of block k1), so one workgroup transforms while
+// the other waits for its slab's loads.  MAXS output slots per thread (256 MAXS >= (ell+1)/2).
+// Workgroup barrier for LDS traffic only (waits for this wave's LDS operations, not for its
+// global loads).  The kernel below exchanges data between threads through LDS only.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <int L1, int L2>
-__global__ __launch_bounds__(256) void srft_fft_kernel(
+template <int L1, int L2, int MAXS>
+__global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
     i64 m, const double* __restrict__ A, i64 lda, i64 mm, const double* __restrict__ d,
     const i64* __restrict__ idx, int ell, double* __restrict__ B, i64 ldb) {
   constexpr int L = L1 * L2;
@@ -213,11 +215,10 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   // LDS padding: each k1 block of Y and each row of Z is one complex longer than its data, so
   // the lanes of a wave that read different k1 blocks (pass 2) or different frequency rows
   // (the accumulation) start on different banks instead of all on one
-  constexpr int YB = L2 * QC + 1, ZS = QC + 1;
+  constexpr int YB = L2 * QC + 1;
   extern __shared__ double lds[];
-  cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2][c], k1 blocks of YB
-  cplx* Z = Y + (size_t)L1 * YB;          // [k][c], rows of ZS
-  cplx* tw = Z + (size_t)L * ZS;          // w_l^j, j < l
+  cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2 or k2][c], k1 blocks of YB
+  cplx* tw = Y + L1 * YB;                 // w_l^j, j < l
   const int tid = threadIdx.x;
   const i64 col = blockIdx.x;
   const double* x = A + col * lda;
@@ -227,7 +228,6 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     tw[j] = {cs, sn};
   }
   const int nslots = (ell + 1) / 2;
-  constexpr int MAXS = 4;
   cplx acc[MAXS], wstep[MAXS], wstep2[MAXS];
   int fcol[MAXS], fconj[MAXS];
   i64 e0[MAXS], estep[MAXS];  // e0 = f q0 mod m, advanced by f 2 QC mod m per slab
@@ -241,8 +241,10 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
     if (s < nslots) {
       const i64 f = idx[2 * s] - 1;
       estep[u] = (f * (2 * QC)) % m;
-      fcol[u] = (int)(f % L);
-      fconj[u] = (int)((L - f % L) % L);
+      // Y offsets of frequency f mod l and of its conjugate partner (-f) mod l
+      const int kc = (int)(f % L), kn = (L - kc) % L;
+      fcol[u] = (kc % L1) * YB + (kc / L1) * QC;
+      fconj[u] = (kn % L1) * YB + (kn / L1) * QC;
       double sn, cs;
       sincospi(-2.0 * (double)f / (double)m, &sn, &cs);
       wstep[u] = {cs, sn};
@@ -257,35 +259,44 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   const bool p2act = tid < QC * L1;
   const int c2 = tid % QC, k1 = tid / QC;
   const i64 nq = mm;
-  // raw x and d of the next slab, multiplied only in its pass 1, so the wait for the loads
-  // falls there and not where they are issued.  (Every load is unconditional, from a clamped
-  // address, with the mask applied in pass 1: loads under branches would each wait for the
-  // previous one.)
-  double rx[L1][2], rd[L1][2];
-  auto load = [&](i64 q0) {
-    const i64 q = q0 + 2 * c1;
-    const i64 qa = min(q, nq - 1), qb = min(q + 1, nq - 1);
-    const int pr = p1act ? p2 : 0;
-#pragma unroll
-    for (int a = 0; a < L1; ++a) {
-      const i64 base = (i64)(a * L2 + pr) * mm;
-      rx[a][0] = x[base + qa];
-      rx[a][1] = x[base + qb];
-      rd[a][0] = d[base + qa];
-      rd[a][1] = d[base + qb];
-    }
-  };
-  load(0);
+  // 16-byte loads of column pairs when every pair is 16-byte aligned in both x and d
+  const bool vec = (mm & 1) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(d) & 15) == 0;
   lds_barrier();  // the twiddles are written by other waves than the ones reading them
   for (i64 q0 = 0; q0 < nq; q0 += 2 * QC) {
-    // pass 1
+    // pass 1.  Every load is unconditional, from a clamped address, with the mask applied
+    // afterwards: loads under branches would each wait for the previous one.
     if (p1act) {
       const i64 q = q0 + 2 * c1;
       const bool ma = q < nq, mb = q + 1 < nq;
       cplx v[L1];
+      if (vec) {
+        // columns q, q + 1 as one 16-byte load (q even, mm even; q clamped to nq - 2)
+        const i64 qv = min(q, nq - 2);
+        d2v rx[L1];
 #pragma unroll
-      for (int a = 0; a < L1; ++a)
-        v[a] = {ma ? rd[a][0] * rx[a][0] : 0.0, mb ? rd[a][1] * rx[a][1] : 0.0};
+        for (int a = 0; a < L1; ++a)
+          rx[a] = *reinterpret_cast<const d2v*>(x + (i64)(a * L2 + p2) * mm + qv);
+#pragma unroll
+        for (int a = 0; a < L1; ++a) {
+          const d2v dv = *reinterpret_cast<const d2v*>(d + (i64)(a * L2 + p2) * mm + qv);
+          v[a] = {ma ? dv.x * rx[a].x : 0.0, mb ? dv.y * rx[a].y : 0.0};
+        }
+      } else {
+        const i64 qa = min(q, nq - 1), qb = min(q + 1, nq - 1);
+        double rx[L1][2];
+#pragma unroll
+        for (int a = 0; a < L1; ++a) {
+          const i64 base = (i64)(a * L2 + p2) * mm;
+          rx[a][0] = x[base + qa];
+          rx[a][1] = x[base + qb];
+        }
+#pragma unroll
+        for (int a = 0; a < L1; ++a) {
+          const i64 base = (i64)(a * L2 + p2) * mm;
+          v[a] = {ma ? d[base + qa] * rx[a][0] : 0.0, mb ? d[base + qb] * rx[a][1] : 0.0};
+        }
+      }
       dft_reg<L1>(v);
 #pragma unroll
       for (int a = 0; a < L1; ++a) {
@@ -293,16 +304,15 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
         Y[(size_t)a * YB + p2 * QC + c1] = t;
       }
     }
-    if (q0 + 2 * QC < nq) load(q0 + 2 * QC);  // next slab, in flight during this one
     lds_barrier();
-    // pass 2
+    // pass 2, in place: frequency k1 + L1 k2 goes where row p2 = k2 was
     if (p2act) {
       cplx v[L2];
 #pragma unroll
       for (int b = 0; b < L2; ++b) v[b] = Y[(size_t)k1 * YB + b * QC + c2];
       dft_reg<L2>(v);
 #pragma unroll
-      for (int b = 0; b < L2; ++b) Z[((size_t)k1 + L1 * b) * ZS + c2] = v[b];
+      for (int b = 0; b < L2; ++b) Y[(size_t)k1 * YB + b * QC + c2] = v[b];
     }
     lds_barrier();
     // accumulate every output frequency (unpacking the two real spectra of a sequence)
@@ -316,11 +326,12 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
       if (e0[u] >= m) e0[u] -= m;
       cplx wq = {cs, sn};
       cplx a = acc[u];
+#pragma unroll 2
       for (int c = 0; c < QC; ++c) {
         const i64 q = q0 + 2 * c;
         if (q >= nq) break;
-        const cplx zc = Z[(size_t)fcol[u] * ZS + c];
-        const cplx zn = Z[(size_t)fconj[u] * ZS + c];
+        const cplx zc = Y[fcol[u] + c];
+        const cplx zn = Y[fconj[u] + c];
         // twice the two real spectra: 2 X1 = zc + conj(zn), 2 X2 = (zc - conj(zn)) / i; the
         // factor 1/2 is applied once at the end.  Columns q and q + 1 enter as
         // w^q (X1 + w X2), and w^q advances by w^2.
@@ -336,8 +347,7 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
       }
       acc[u] = a;
     }
-    // (the next slab's pass 2 overwrites Z only after its first barrier, which every thread
-    // reaches after finishing this accumulation)
+    lds_barrier();  // Y read by every thread before the next slab's pass 1 writes it
   }
 #pragma unroll
   for (int u = 0; u < MAXS; ++u) {
@@ -349,25 +359,33 @@ __global__ __launch_bounds__(256) void srft_fft_kernel(
   }
 }
 
-template <int L1, int L2>
-int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
-                    const double* d, const i64* idx, int ell, double* B, i64 ldb) {
+template <int L1, int L2, int MAXS>
+int launch_srft_fft_s(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
+                      const double* d, const i64* idx, int ell, double* B, i64 ldb) {
   constexpr int L = L1 * L2;
   constexpr int QC = 256 / (L1 > L2 ? L1 : L2);
-  // Y (L1 blocks of L2 QC + 1), Z (L rows of QC + 1) and the L twiddles, 16 bytes each
-  const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + (size_t)L * (QC + 1) + L) * 16;
+  // Y (L1 blocks of L2 QC + 1) and the L twiddles, 16 bytes each
+  const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + L) * 16;
   if (bytes > 64 * 1024) {
     static std::atomic<uint64_t> lds_set{0};  // per device, set once
-    lra_lds_limit_once(lds_set, (const void*)srft_fft_kernel<L1, L2>, (int)bytes);
+    lra_lds_limit_once(lds_set, (const void*)srft_fft_kernel<L1, L2, MAXS>, (int)bytes);
   }
-  hipLaunchKernelGGL((srft_fft_kernel<L1, L2>), dim3((unsigned)Q), dim3(256), bytes, st, P, X,
-                     ldx, mm, d, idx, ell, B, ldb);
+  hipLaunchKernelGGL((srft_fft_kernel<L1, L2, MAXS>), dim3((unsigned)Q), dim3(256), bytes, st,
+                     P, X, ldx, mm, d, idx, ell, B, ldb);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     lra_set_error("srft launch: %s", hipGetErrorString(e));
     return LRA_E_HIP;
   }
   return LRA_OK;
+}
+
+template <int L1, int L2>
+int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
+                    const double* d, const i64* idx, int ell, double* B, i64 ldb) {
+  // output slots per thread: (ell + 1) / 2 slots over 256 threads
+  if (ell <= 512) return launch_srft_fft_s<L1, L2, 1>(st, Q, P, X, ldx, mm, d, idx, ell, B, ldb);
+  return launch_srft_fft_s<L1, L2, 4>(st, Q, P, X, ldx, mm, d, idx, ell, B, ldb);
 }
 
 // B[i, j] = A[r[i]-1, j]
