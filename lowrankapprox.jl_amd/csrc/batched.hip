@@ -174,7 +174,9 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
   __shared__ double sVal[2][BW];
   __shared__ int sPos[2][BW], sLane[2][BW], sFlag[2][BW];
   __shared__ double sMax[BW];
-  __shared__ double sPc[CL == 2 ? BW : 1][LM];   // the partner's candidate column, per wave
+  __shared__ double sPc[LM];   // the partner's candidate column (pairs; wave 0 fetches it)
+  __shared__ double sXv;       // ... its norm, position and recompute flag
+  __shared__ int sXp, sXf;
   __shared__ int sBlk, sAbort;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -351,15 +353,23 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
 #pragma unroll
     for (int w = 1; w < BW; ++w) maxnrm = fmax(maxnrm, sMax[w]);
     if constexpr (CL == 2) {
-      // both halves need the largest column norm of the whole block (exchange 0, parity 1)
-      if (wave == 0)
+      // both halves need the largest column norm of the whole block (exchange 0, parity 1).
+      // Only wave 0 exchanges (one poller per workgroup keeps the uncached traffic of the
+      // waits low); the others take the result from LDS after the barrier.
+      if (wave == 0) {
         xpublish<LM>(myslot + (LM + 2), xtag(gen, qi, 0), maxnrm, 0, 0, nullptr, p.uncached);
-      double pv;
-      int pp, pf;
-      if (!xpoll<LM>(peerslot + (LM + 2), xtag(gen, qi, 0), p.uncached, p.err, &pv, &pp, &pf,
-                     sPc[wave]))
-        sAbort = 1;
-      maxnrm = fmax(maxnrm, pv);
+        double pv;
+        int pp, pf;
+        const bool ok = xpoll<LM>(peerslot + (LM + 2), xtag(gen, qi, 0), p.uncached, p.err, &pv,
+                                  &pp, &pf, sPc);
+        if (lane == 0) {
+          sXv = pv;
+          if (!ok) sAbort = 1;
+        }
+      }
+      __syncthreads();
+      if (sAbort) return;
+      maxnrm = fmax(maxnrm, sXv);
     }
     const double ptol = fmax(p.atol, p.rtol * maxnrm);
     int kcap = min(ell, n);
@@ -404,15 +414,26 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
       }
       myflag = 0;
       if constexpr (CL == 2) {
-        // swap candidates with the other half: the larger norm wins, then the lower position
-        if (wave == 0)
+        // swap candidates with the other half (wave 0 exchanges, the others read the
+        // partner's candidate from LDS): the larger norm wins, then the lower position.
+        // sPc is rewritten only after the next step's first barrier, which every wave
+        // reaches after its reflector build has read it.
+        if (wave == 0) {
           xpublish<LM>(myslot + par * (LM + 2), xtag(gen, qi, s + 1), cv, cp, cflag,
                        sLane[par][cw] >= 0 ? sCand[par][cw] : nullptr, p.uncached);
-        double pv;
-        int pp, pf;
-        if (!xpoll<LM>(peerslot + par * (LM + 2), xtag(gen, qi, s + 1), p.uncached, p.err, &pv,
-                       &pp, &pf, sPc[wave]))
-          sAbort = 1;
+          double pv;
+          int pp, pf;
+          const bool ok = xpoll<LM>(peerslot + par * (LM + 2), xtag(gen, qi, s + 1),
+                                    p.uncached, p.err, &pv, &pp, &pf, sPc);
+          if (lane == 0) {
+            sXv = pv; sXp = pp; sXf = pf;
+            if (!ok) sAbort = 1;
+          }
+        }
+        __syncthreads();
+        if (sAbort) return;
+        const double pv = sXv;
+        const int pp = sXp, pf = sXf;
         cflag |= pf;
         if (pv > cv || (pv == cv && pp < cp)) { cv = pv; cp = pp; cw = -1; }
       }
@@ -442,7 +463,7 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
       double tau = 0.0;
       // waves without an active column skip it (wave 0 always builds it: it writes R11)
       if (wave == 0 || __any(active)) {
-        const double* xc = cw >= 0 ? sCand[par][cw] : sPc[CL == 2 ? wave : 0];
+        const double* xc = cw >= 0 ? sCand[par][cw] : sPc;
         double xr[NR];
         double ss = 0.0;
 #pragma unroll

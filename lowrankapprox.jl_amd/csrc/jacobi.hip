@@ -110,9 +110,10 @@ LRA_DEV double rsqrt_12(double x) {
 // for W >= 4, so the W pairs of a step fill one wave per SIMD), each holding MR rows in
 // registers; Xs columns are zero-padded to mp = L * MR >= m rows (stride mp + 16).  Dynamic
 // LDS: Xs (2W columns), Qs (2W x 2W, row-major with a padded stride, so a column of Q is read
-// bank-conflict free and a row contiguously).
-template <int W, int MR>
-__global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB, int r, int first,
+// bank-conflict free and a row contiguously).  BT threads: 512, or 256 (one wave per SIMD,
+// room for 2 MR > 72 registers) for the tall blocks of W <= 2 with MR > 36.
+template <int W, int MR, int BT>
+__global__ __launch_bounds__(BT) void jacobi_block_kernel(int m, int n, int NB, int r, int first,
                                                            double* __restrict__ X, i64 ldx,
                                                            double* __restrict__ V, i64 ldv,
                                                            double tol, int* __restrict__ rotated,
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
   if (prev && *(const volatile int*)prev == 0) return;
   constexpr int C = 2 * W;
   constexpr int QS = C + 1;                 // row stride of Qs
-  constexpr int NW = 8;                     // waves
+  constexpr int NW = BT / 64;               // waves
   constexpr int L = W >= 16 ? 16 : W == 8 ? 32 : 64;  // lanes per pair
   constexpr int G = 64 / L;                 // pairs per wave
   constexpr int mp = L * MR;                // padded column length in LDS
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
   auto gcol = [&](int c) { return c < W ? P * W + c : Q * W + (c - W); };
   // Xs(i, c) = X(i, gcol(c)), zero for padding rows and columns; every load of a thread is
   // issued before the first store
-  for (int i0 = 0; i0 < mp; i0 += 512) {
+  for (int i0 = 0; i0 < mp; i0 += BT) {
     const int i = i0 + tid;
     double tmp[C];
 #pragma unroll
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
       for (int c = 0; c < C; ++c) Xs[c * mps + i] = tmp[c];
     }
   }
-  for (int e = tid; e < C * QS; e += 512) Qs[e] = (e / QS) == (e % QS) ? 1.0 : 0.0;
+  for (int e = tid; e < C * QS; e += BT) Qs[e] = (e / QS) == (e % QS) ? 1.0 : 0.0;
   if (tid == 0) any = 0;
   __syncthreads();
   const double tol2 = tol * tol;
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
   if (!any) return;
   // V(:, block cols) <- V(:, block cols) * Q on MFMA, as V' <- Q' V': the block of V (n x 2W)
   // is staged in LDS over Xs; lane l of a 16 x 16 output tile then holds 16 consecutive rows of
-  // one column of V, so the stores are coalesced.  The first 512 rows of the block are loaded
+  // one column of V, so the stores are coalesced.  The first BT rows of the block are loaded
   // before X is written back, so the write-back covers their latency.
   double vt[C];
 #pragma unroll
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
     const int g = gcol(c);
     if (g >= n) continue;
     double* x = X + (i64)g * ldx;
-    for (int i = tid; i < m; i += 512) x[i] = Xs[c * mps + i];
+    for (int i = tid; i < m; i += BT) x[i] = Xs[c * mps + i];
   }
   __syncthreads();  // Xs has been written out
   double* Vs = Xs;  // Vs[t * mps + i] = V(i, gcol(t))
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(512) void jacobi_block_kernel(int m, int n, int NB,
 #pragma unroll
     for (int c = 0; c < C; ++c) Vs[c * mps + tid] = vt[c];
   }
-  for (int i0 = 512; i0 < n; i0 += 512) {
+  for (int i0 = BT; i0 < n; i0 += BT) {
     const int i = i0 + tid;
     double tmp[C];
 #pragma unroll
@@ -409,8 +410,9 @@ struct BlockCfg {
 constexpr int lanes_per_pair(int W) { return W >= 16 ? 16 : W == 8 ? 32 : 64; }
 
 // Widest block (8, 4, 2, 1; 16 on request) whose 2W zero-padded columns of X fit in LDS with
-// at most 32 rows per lane in registers, narrower when the matrix has few columns (at least 4
-// blocks); W = 0 (one launch per round of single pairs) when none does.  At k = 426 a sweep
+// at most 32 rows per lane in registers (36 for W = 4: up to 2304 rows; 64 for W <= 2, in
+// 256-thread workgroups beyond 36: up to 4096 rows), narrower when the matrix has few columns
+// (at least 4 blocks); W = 0 (one launch per round of single pairs) when none does.  At k = 426 a sweep
 // takes 0.95 ms at W = 8 and 1.06 ms at W = 16: the step cost grows with the pairs per wave
 // faster than the round count falls.  LRA_JACOBI_W=<w> caps the width (tests and tuning).
 BlockCfg block_cfg(i64 m, i64 n) {
@@ -422,8 +424,12 @@ BlockCfg block_cfg(i64 m, i64 n) {
     if (W > 1 && (i64)4 * W > n) continue;
     const i64 L = lanes_per_pair(W);
     int MR = 0;
-    for (int b : {4, 8, 16, 24, 28, 32})
-      if (L * b >= m && !((W <= 2) && (b == 24 || b == 28))) { MR = b; break; }
+    for (int b : {4, 8, 16, 24, 28, 32, 36, 40, 48, 64})
+      if (L * b >= m && !((W <= 2) && (b == 24 || b == 28)) &&
+          (W <= 2 || b <= 32 || (W == 4 && b == 36))) {
+        MR = b;
+        break;
+      }
     if (!MR) continue;
     const size_t lds = ((size_t)2 * W * (L * MR + 16) + (size_t)2 * W * (2 * W + 1)) * sizeof(double);
     if (lds > kJacobiLds) continue;
@@ -439,10 +445,11 @@ template <int W, int MR>
 void launch_block_wm(int NB, size_t lds, hipStream_t st, int m, int n, int r, int first,
                      double* X, i64 ldx, double* V, i64 ldv, double tol, int* rot,
                      const int* prev) {
+  constexpr int BT = MR > 36 ? 256 : 512;
   static std::atomic<uint64_t> lds_set{0};
-  lra_lds_limit_once(lds_set, (const void*)jacobi_block_kernel<W, MR>, (int)kJacobiLds);
-  hipLaunchKernelGGL((jacobi_block_kernel<W, MR>), dim3((unsigned)(NB / 2)), dim3(512), lds, st,
-                     m, n, NB, r, first, X, ldx, V, ldv, tol, rot, prev);
+  lra_lds_limit_once(lds_set, (const void*)jacobi_block_kernel<W, MR, BT>, (int)kJacobiLds);
+  hipLaunchKernelGGL((jacobi_block_kernel<W, MR, BT>), dim3((unsigned)(NB / 2)), dim3(BT), lds,
+                     st, m, n, NB, r, first, X, ldx, V, ldv, tol, rot, prev);
 }
 
 template <int W>
@@ -459,7 +466,19 @@ void launch_block_w(int MR, int NB, size_t lds, hipStream_t st, int m, int n, in
     case 28:
       if constexpr (W > 2) launch_block_wm<W, 28>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
       break;
-    default: launch_block_wm<W, 32>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 32: launch_block_wm<W, 32>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev); break;
+    case 36:
+      if constexpr (W <= 4) launch_block_wm<W, 36>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
+      break;
+    case 40:
+      if constexpr (W <= 2) launch_block_wm<W, 40>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
+      break;
+    case 48:
+      if constexpr (W <= 2) launch_block_wm<W, 48>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
+      break;
+    default:
+      if constexpr (W <= 2) launch_block_wm<W, 64>(NB, lds, st, m, n, r, first, X, ldx, V, ldv, tol, rot, prev);
+      break;
   }
 }
 

@@ -173,6 +173,16 @@ def test_gesvj_vs_numpy(gpu):
         _check_svd(X, *_gesvj(gpu, X))
 
 
+@pytest.mark.parametrize("m,k", [(2091, 2091), (2600, 300), (4000, 120)])
+def test_gesvj_tall_blocks(gpu, m, k):
+    """Matrices taller than 2048 rows: blocks of two columns with 40, 48 or 64 rows per lane
+    (up to 4096 rows) instead of one launch per round of single pairs; graded columns."""
+    rng = np.random.default_rng(m + k)
+    X = rng.standard_normal((m, k)) @ np.diag(10.0 ** -np.linspace(0, 10, k))
+    print(f"gesvj tall {m}x{k}", flush=True)
+    _check_svd(X, *_gesvj(gpu, X))
+
+
 @pytest.mark.parametrize("w", ["0", "1", "2", "4", "8", "16"])
 def test_gesvj_block_widths(gpu, w, monkeypatch):
     """Every block width of the block-cyclic order (0: one launch per round of single pairs)
