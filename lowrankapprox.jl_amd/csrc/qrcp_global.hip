@@ -8,8 +8,9 @@
 //             finished, the first position of the largest partial norm (idamax), LAPACK's
 //             column swap (data, jpvt, vn1, vn2), dlarfg on the pivot column (R's diagonal in
 //             place, v below it, a contiguous copy of v);
-//   qg_apply  one wave per trailing column: H applied, LAWN 176 downdate; a column whose
-//             downdate fails the tol3z test gets its norm recomputed and ends the block.
+//   qg_apply  one wave per trailing column (two for 1025..4096 rows, qg_apply2): H applied,
+//             LAWN 176 downdate; a column whose downdate fails the tol3z test gets its norm
+//             recomputed and ends the block.
 // Once the stop rule fires, the remaining launches return at once.  The arithmetic and the
 // control follow qrcp_kernel, so both give the same pivots up to rounding.
 #include "lra_internal.h"
@@ -309,6 +310,73 @@ __global__ __launch_bounds__(256) void qg_apply(int s, int m, int n, int lastrk,
   }
 }
 
+// qg_apply for taller columns: two waves per column (rows s + 64 (2r + h) + lane in wave h),
+// RPL values per lane, so rows up to 128 RPL are read once and written once per step; the
+// dot product and the recompute norm meet in LDS.
+template <int RPL>
+__global__ __launch_bounds__(256) void qg_apply2(int s, int m, int n, int lastrk,
+                                                 double* __restrict__ B, i64 ldb,
+                                                 const double* __restrict__ v,
+                                                 const double* __restrict__ tau, double* vn1,
+                                                 double* vn2, GState* st) {
+  if (st->done) return;
+  __shared__ double sv[128 * RPL];  // v(s .. s + 128 RPL - 1), zero past m
+  __shared__ double sw[2][2], sss[2][2], sx[2];
+  const int lane = threadIdx.x & 63, h = (threadIdx.x >> 6) & 1, cl = threadIdx.x >> 7;
+  const int j = s + 1 + blockIdx.x * 2 + cl;
+  const bool have = j < n;  // (no early exit: the barriers below involve every wave)
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
+  for (int t = threadIdx.x; t < 128 * RPL; t += 256) sv[t] = s + t < m ? v[s + t] : 0.0;
+  __syncthreads();
+  double* c = B + (i64)(have ? j : 0) * ldb;
+  double cr[RPL];
+  double w = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (2 * r + h) + lane;
+    cr[r] = (have && s + o < m) ? c[s + o] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) w += sv[64 * (2 * r + h) + lane] * cr[r];
+  w = group_sum(w, 64);
+  if (lane == 0) sw[cl][h] = w;
+  __syncthreads();
+  const double tw = tau[s] * (sw[cl][0] + sw[cl][1]);
+  double ss = 0.0, x = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (2 * r + h) + lane;
+    const double y = cr[r] - tw * sv[o];
+    if (have && s + o < m) {
+      c[s + o] = y;
+      if (o == 0) x = y;
+      else ss += y * y;
+    }
+  }
+  ss = group_sum(ss, 64);
+  if (lane == 0) {
+    sss[cl][h] = ss;
+    if (h == 0) sx[cl] = x;  // row s is offset 0: lane 0 of wave 0
+  }
+  __syncthreads();
+  if (!have || h != 0 || lane != 0) return;
+  x = sx[cl];
+  const double a = vn1[j];
+  if (s + 1 < lastrk && a != 0.0) {
+    double temp = fabs(x) / a;
+    temp = (1.0 + temp) * (1.0 - temp);
+    temp = temp < 0.0 ? 0.0 : temp;
+    const double q = a / vn2[j];
+    const double temp2 = temp * q * q;
+    if (temp2 <= tol3z) {
+      vn1[j] = vn2[j] = sqrt(sss[cl][0] + sss[cl][1]);
+      __hip_atomic_store(&st->flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      vn1[j] = a * sqrt(temp);
+    }
+  }
+}
+
 __global__ void qg_finish(const GState* st, i64* k_out, int* early_out) {
   *k_out = st->k;
   if (early_out) *early_out = st->early;
@@ -596,9 +664,15 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
       else if (rows <= 64 * 16)
         hipLaunchKernelGGL(qg_apply<16>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
                            q.tau, vn1, vn2, st);
-      else if (rows <= 64 * 32)
-        hipLaunchKernelGGL(qg_apply<32>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
-                           q.tau, vn1, vn2, st);
+      else if (rows <= 128 * 16)
+        hipLaunchKernelGGL(qg_apply2<16>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
+                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+      else if (rows <= 128 * 24)
+        hipLaunchKernelGGL(qg_apply2<24>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
+                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+      else if (rows <= 128 * 32)
+        hipLaunchKernelGGL(qg_apply2<32>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
+                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
       else
         hipLaunchKernelGGL(qg_apply<0>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
                            q.tau, vn1, vn2, st);
