@@ -8,9 +8,9 @@
 //             finished, the first position of the largest partial norm (idamax), LAPACK's
 //             column swap (data, jpvt, vn1, vn2), dlarfg on the pivot column (R's diagonal in
 //             place, v below it, a contiguous copy of v);
-//   qg_apply  one wave per trailing column (two for 1025..4096 rows, qg_apply2): H applied,
-//             LAWN 176 downdate; a column whose downdate fails the tol3z test gets its norm
-//             recomputed and ends the block.
+//   qg_apply  one wave per trailing column (qg_applyw: two for 1025..4096 rows, four up to
+//             8192): H applied, LAWN 176 downdate; a column whose downdate fails the tol3z
+//             test gets its norm recomputed and ends the block.
 // Once the stop rule fires, the remaining launches return at once.  The arithmetic and the
 // control follow qrcp_kernel, so both give the same pivots up to rounding.
 #include "lra_internal.h"
@@ -310,42 +310,48 @@ __global__ __launch_bounds__(256) void qg_apply(int s, int m, int n, int lastrk,
   }
 }
 
-// qg_apply for taller columns: two waves per column (rows s + 64 (2r + h) + lane in wave h),
-// RPL values per lane, so rows up to 128 RPL are read once and written once per step; the
+// qg_apply for taller columns: NWC waves per column (rows s + 64 (NWC r + h) + lane in wave h),
+// RPL values per lane, so rows up to 64 NWC RPL are read once and written once per step; the
 // dot product and the recompute norm meet in LDS.
-template <int RPL>
-__global__ __launch_bounds__(256) void qg_apply2(int s, int m, int n, int lastrk,
+template <int NWC, int RPL>
+__global__ __launch_bounds__(256) void qg_applyw(int s, int m, int n, int lastrk,
                                                  double* __restrict__ B, i64 ldb,
                                                  const double* __restrict__ v,
                                                  const double* __restrict__ tau, double* vn1,
                                                  double* vn2, GState* st) {
   if (st->done) return;
-  __shared__ double sv[128 * RPL];  // v(s .. s + 128 RPL - 1), zero past m
-  __shared__ double sw[2][2], sss[2][2], sx[2];
-  const int lane = threadIdx.x & 63, h = (threadIdx.x >> 6) & 1, cl = threadIdx.x >> 7;
-  const int j = s + 1 + blockIdx.x * 2 + cl;
+  constexpr int CPB = 4 / NWC;  // columns per block
+  constexpr int ROWS = 64 * NWC * RPL;
+  __shared__ double sv[ROWS];  // v(s .. s + ROWS - 1), zero past m
+  __shared__ double sw[CPB][NWC], sss[CPB][NWC], sx[CPB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = wave % NWC, cl = wave / NWC;
+  const int j = s + 1 + blockIdx.x * CPB + cl;
   const bool have = j < n;  // (no early exit: the barriers below involve every wave)
   const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
-  for (int t = threadIdx.x; t < 128 * RPL; t += 256) sv[t] = s + t < m ? v[s + t] : 0.0;
+  for (int t = threadIdx.x; t < ROWS; t += 256) sv[t] = s + t < m ? v[s + t] : 0.0;
   __syncthreads();
   double* c = B + (i64)(have ? j : 0) * ldb;
   double cr[RPL];
   double w = 0.0;
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
-    const int o = 64 * (2 * r + h) + lane;
+    const int o = 64 * (NWC * r + h) + lane;
     cr[r] = (have && s + o < m) ? c[s + o] : 0.0;
   }
 #pragma unroll
-  for (int r = 0; r < RPL; ++r) w += sv[64 * (2 * r + h) + lane] * cr[r];
+  for (int r = 0; r < RPL; ++r) w += sv[64 * (NWC * r + h) + lane] * cr[r];
   w = group_sum(w, 64);
   if (lane == 0) sw[cl][h] = w;
   __syncthreads();
-  const double tw = tau[s] * (sw[cl][0] + sw[cl][1]);
+  double wt = 0.0;
+#pragma unroll
+  for (int q = 0; q < NWC; ++q) wt += sw[cl][q];
+  const double tw = tau[s] * wt;
   double ss = 0.0, x = 0.0;
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
-    const int o = 64 * (2 * r + h) + lane;
+    const int o = 64 * (NWC * r + h) + lane;
     const double y = cr[r] - tw * sv[o];
     if (have && s + o < m) {
       c[s + o] = y;
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(256) void qg_apply2(int s, int m, int n, int lastrk
   ss = group_sum(ss, 64);
   if (lane == 0) {
     sss[cl][h] = ss;
-    if (h == 0) sx[cl] = x;  // row s is offset 0: lane 0 of wave 0
+    if (h == 0) sx[cl] = x;  // row s is offset 0: lane 0 of the column's first wave
   }
   __syncthreads();
   if (!have || h != 0 || lane != 0) return;
@@ -369,7 +375,10 @@ __global__ __launch_bounds__(256) void qg_apply2(int s, int m, int n, int lastrk
     const double q = a / vn2[j];
     const double temp2 = temp * q * q;
     if (temp2 <= tol3z) {
-      vn1[j] = vn2[j] = sqrt(sss[cl][0] + sss[cl][1]);
+      double ssum = 0.0;
+#pragma unroll
+      for (int q2 = 0; q2 < NWC; ++q2) ssum += sss[cl][q2];
+      vn1[j] = vn2[j] = sqrt(ssum);
       __hip_atomic_store(&st->flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       vn1[j] = a * sqrt(temp);
@@ -665,14 +674,17 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
         hipLaunchKernelGGL(qg_apply<16>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
                            q.tau, vn1, vn2, st);
       else if (rows <= 128 * 16)
-        hipLaunchKernelGGL(qg_apply2<16>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
-                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+        hipLaunchKernelGGL((qg_applyw<2, 16>), dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256),
+                           0, s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
       else if (rows <= 128 * 24)
-        hipLaunchKernelGGL(qg_apply2<24>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
-                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+        hipLaunchKernelGGL((qg_applyw<2, 24>), dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256),
+                           0, s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
       else if (rows <= 128 * 32)
-        hipLaunchKernelGGL(qg_apply2<32>, dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256), 0,
-                           s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+        hipLaunchKernelGGL((qg_applyw<2, 32>), dim3((unsigned)((n - s - 1 + 1) / 2)), dim3(256),
+                           0, s0, s, m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
+      else if (rows <= 256 * 32)
+        hipLaunchKernelGGL((qg_applyw<4, 32>), dim3((unsigned)(n - s - 1)), dim3(256), 0, s0, s,
+                           m, n, lastrk, q.A, q.lda, v, q.tau, vn1, vn2, st);
       else
         hipLaunchKernelGGL(qg_apply<0>, grid, dim3(256), 0, s0, s, m, n, lastrk, q.A, q.lda, v,
                            q.tau, vn1, vn2, st);

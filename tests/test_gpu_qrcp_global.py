@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("m,n,rtol", [(40, 300, 1e-12), (72, 3000, 1e-10), (264, 2500, 1e-13),
                                       (12, 5, 0.0), (33, 700, 1e-6), (700, 400, 1e-12),
                                       (1032, 3000, 1e-10), (2056, 2500, 1e-12),
-                                      (3500, 1200, 1e-12)])
+                                      (3500, 1200, 1e-12), (6000, 300, 1e-12)])
 def test_global_qrcp_vs_oracle(gpu, orc, monkeypatch, m, n, rtol):
     import torch
     monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
