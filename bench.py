@@ -388,25 +388,32 @@ def profile_kernels(lra, torch, n: int):
     print(f"# profiled: copy 1 GiB, qrcp 520x{n} (k={k}), batched 4096 blocks", file=sys.stderr)
 
 
-FACTOR_NAMES = ("idfact", "psvdfact", "pqrfact", "qrcp_hbm")
+FACTOR_NAMES = ("idfact", "psvdfact", "pqrfact", "qrcp_hbm", "idfact_slow", "psvdfact_slow")
 
 
 def profile_factor(lra, torch, n: int):
     """Launch sequence for a `rocprofv3 --kernel-trace` run of the dense factorizations on the
     n x n workload matrix: a warm-up of each (workspaces grow, GEMM shapes tune), then one
-    idfact, one psvdfact and one pqrfact (rtol 1e-10), and the pivoted QR in HBM of a
-    2056 x n Gaussian sketch of the matrix (rtol 1e-10; too tall for the register-resident
-    kernel), each preceded by a one-element torch add that marks where it starts in the trace
-    (`--summarize-trace` splits on these)."""
+    idfact, one psvdfact and one pqrfact (rtol 1e-10), the pivoted QR in HBM of a 2056 x n
+    Gaussian sketch of the matrix (rtol 1e-10; too tall for the register-resident kernel),
+    and idfact and psvdfact of the slowest matrix of the serving pool (3758 x 3758,
+    sigma_i = exp(-0.011 (i-1)), k ~ 2100-2250 at rtol 1e-10), each preceded by a one-element
+    torch add that marks where it starts in the trace (`--summarize-trace` splits on these)."""
     A, _ = make_matrix(torch, n)
     B = lra.sketch("left", "n", A, 2056, seed=3)
+    S = serving_pool(torch, 16)[14]  # (its last entry, 8192 x 8192, is forced)
+    torch.cuda.synchronize()
 
     def qrcp_hbm(seed=None, rtol=1e-10):
         jp, tau, k = lra.geqp3_adap(B.clone(), -1, 0.0, rtol, 32)
         return types.SimpleNamespace(k=k)
 
     def run(name, **kw):
-        return qrcp_hbm(**kw) if name == "qrcp_hbm" else getattr(lra, name)(A, **kw)
+        if name == "qrcp_hbm":
+            return qrcp_hbm(**kw)
+        if name.endswith("_slow"):
+            return getattr(lra, name[:-5])(S, **kw)
+        return getattr(lra, name)(A, **kw)
 
     for name in FACTOR_NAMES:
         for _ in range(2):
@@ -546,9 +553,12 @@ def bench_config1(lra, torch, steps: int = 50):
     return out
 
 
-def bench_psvd(lra, torch, A, steps: int = 5):
-    """psvdfact of the throughput workload matrix at rtol = 1e-10 (src/psvd.jl:238-272)."""
-    lra.psvdfact(A, rtol=1e-10, seed=100)
+def bench_psvd(lra, torch, A, steps: int = 10, warmup: int = 3):
+    """psvdfact of the throughput workload matrix at rtol = 1e-10 (src/psvd.jl:238-272), after
+    `warmup` calls with other seeds (their ranks differ by one or two, so workspaces reach
+    their size and every GEMM shape class is tuned before the timed calls)."""
+    for w in range(warmup):
+        lra.psvdfact(A, rtol=1e-10, seed=100 + w)
     torch.cuda.synchronize()
     lat = []
     for s in range(steps):
@@ -581,8 +591,9 @@ def main():
                          "and one batched launch of the 4096-block workload")
     ap.add_argument("--profile-factor", action="store_true",
                     help="only launch, for a rocprofv3 kernel-trace run: idfact, psvdfact, "
-                         "pqrfact of the workload matrix and the HBM pivoted QR of a 2056-row "
-                         "sketch of it, after a warm-up, each behind a marker kernel")
+                         "pqrfact of the workload matrix, the HBM pivoted QR of a 2056-row "
+                         "sketch of it, idfact and psvdfact of the slowest serving-pool "
+                         "matrix, after a warm-up, each behind a marker kernel")
     ap.add_argument("--summarize-trace", metavar="CSV",
                     help="print per-kernel totals of a --profile-factor kernel trace and exit")
     args = ap.parse_args()
