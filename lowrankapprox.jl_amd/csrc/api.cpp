@@ -552,14 +552,17 @@ static int sketchfact_run(lra_ctx* c, i64 N, const SketchFn& make, const lra_opt
   for (i64 rnd = 0;;) {
     const i64 ell = samp_of(o, target);
     if (ell <= 0) { lra_set_error("sketch order %lld", (long long)ell); return LRA_E_UNSUPPORTED; }
-    // Rounds 1+2, 3+4, ... of an FP64 adaptive loop run as pairs: both sketches are formed
-    // and factored in one co-resident launch, round r+1 on the larger share of the grid, so
-    // round r's pivoted QR costs no time of its own (dense matrices only: a caller's
-    // operator is applied exactly once per round, as the reference does).  Each round's
-    // factorization is exactly what it is alone (its own sketch, stop rule and stop_after);
-    // the first accepted round is taken, as the sequential loop would.  A pair that does not
-    // fit one grid runs round by round.
-    if (pair_ok && adap && prec == LRA_PREC_F64 && (rnd & 1) == 1) {
+    // Rounds r and r+1 of an FP64 adaptive loop run as a pair whenever both fit one
+    // co-resident grid (0+1, 2+3, ...; a round that does not pair runs alone and the next
+    // one tries again): both sketches are formed by one GEMM and factored in one launch,
+    // round r+1 on the larger share of the grid, so round r's pivoted QR costs no time of its
+    // own (dense matrices only: a caller's operator is applied exactly once per round, as
+    // the reference does).  Each round's factorization is exactly what it is alone (its own
+    // sketch, stop rule and stop_after); the first accepted round is taken, as the
+    // sequential loop would.  Round 0 pairs only for wide sketches (N >= 4096): on small
+    // problems the pair's larger launch and GEMM cost more than round 0's QR saves, and
+    // small ranks are accepted in round 0.
+    if (pair_ok && adap && prec == LRA_PREC_F64 && (rnd >= 1 || N >= 4096)) {
       const i64 ell2 = samp_of(o, 2 * target);
       if (ell2 > ell && ell2 < (1 << 20) && N < (1 << 30) &&
           lra_qrcp_pair_fits(c, (int)std::min(ell, N), (int)std::min(ell2, N), (int)N) &&

@@ -761,19 +761,58 @@ static QrcpDev dev_of(lra_ctx* ctx, const QrcpCall& q, const QrcpShape& sh) {
 // get alone; the smaller one takes the same rows per lane and block size with the fewest
 // lanes per column that hold it.  Both must be cooperative, each within its slot exchange, and
 // together co-resident.
+// The smaller round's shape for a pair launched with the larger round's shape s1 (same rows
+// per lane and block size: one kernel instance runs both), when both grids are co-resident.
+static bool pair_fit(lra_ctx* ctx, int m0, int n, const QrcpShape& s1, QrcpShape* s0) {
+  int tpc = 1;
+  while (tpc * s1.rpt < m0) tpc <<= 1;
+  if (tpc > 64) return false;
+  const int cpw = s1.block / tpc;
+  const int G = (n + cpw - 1) / cpw;
+  if (G <= 1 || G > s1.block || s1.G <= 1 || s1.G > s1.block) return false;
+  int per_cu = 0;
+  if (occupancy_pair(s1.rpt, s1.block, &per_cu) != 0 || per_cu < 1) return false;
+  if (G + s1.G > per_cu * ctx->num_cu) return false;
+  *s0 = {tpc, s1.rpt, s1.block, G, cpw};
+  return true;
+}
+
+// Shapes of a pair: the larger round's own best shape when the pair fits with it, else the
+// cheapest shape of the larger round (by the step-cost model) with which it does.
 static bool pair_shapes(lra_ctx* ctx, int m0, int m1, int n, QrcpShape* s0, QrcpShape* s1) {
   if (m0 > m1 || n <= 0) return false;
   if (!choose_shape(ctx, m1, n, s1) || s1->G <= 1) return false;
-  int tpc = 1;
-  while (tpc * s1->rpt < m0) tpc <<= 1;
-  if (tpc > 64) return false;
-  const int cpw = s1->block / tpc;
-  const int G = (n + cpw - 1) / cpw;
-  if (G <= 1 || G > s1->block || s1->G > s1->block) return false;
-  *s0 = {tpc, s1->rpt, s1->block, G, cpw};
-  int per_cu = 0;
-  if (occupancy_pair(s1->rpt, s1->block, &per_cu) != 0 || per_cu < 1) return false;
-  return s0->G + s1->G <= per_cu * ctx->num_cu;
+  if (pair_fit(ctx, m0, n, *s1, s0)) return true;
+  bool found = false;
+  double best = 1e30;
+  QrcpShape b0{}, b1{};
+  for (int block : {256, 512}) {
+    const int rpt_max = block == 256 ? 80 : 40;
+    for (int tpc = 1; tpc <= 64; tpc <<= 1) {
+      int rpt = (m1 + tpc - 1) / tpc;
+      rpt = (rpt + 3) / 4 * 4;
+      if (rpt < 4) rpt = 4;
+      if (rpt > rpt_max) continue;
+      const int cpw = block / tpc;
+      const int G = (n + cpw - 1) / cpw;
+      if (G > 256 || G <= 1) continue;
+      const QrcpShape t1 = {tpc, rpt, block, G, cpw};
+      QrcpShape t0;
+      if (!pair_fit(ctx, m0, n, t1, &t0)) continue;
+      const double cost = coop_step_cost(rpt, block, G);
+      if (!found || cost < best) {
+        best = cost;
+        b0 = t0;
+        b1 = t1;
+        found = true;
+      }
+    }
+  }
+  if (found) {
+    *s0 = b0;
+    *s1 = b1;
+  }
+  return found;
 }
 
 bool lra_qrcp_pair_fits(lra_ctx* ctx, int m0, int m1, int n) {
