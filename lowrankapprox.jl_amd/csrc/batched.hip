@@ -38,8 +38,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
-constexpr int BT = 1024;     // threads per workgroup
-constexpr int BW = BT / 64;  // waves per workgroup
 constexpr int SLAB = 40 * 1024;  // scratch doubles per workgroup (>= LM * columns, every variant)
 constexpr int NSTAGE = 4;
 
@@ -153,14 +151,17 @@ LRA_DEV bool xpoll(const u64x2* slot, uint64_t tag, int uncached, int* err, doub
 }
 
 // LM: largest sketch order; TPC: threads per column (RPT = LM / TPC rows each); CL: workgroups
-// per block (2: each takes half the columns and the pair exchanges its pivot candidates).
-template <int LM, int TPC, int CL>
-__global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
+// per block (2: each takes half the columns and the pair exchanges its pivot candidates);
+// BT: threads per workgroup (512: two workgroups per CU, so one's sketch overlaps the
+// other's pivot steps; 1024: one).
+template <int LM, int TPC, int CL, int BT>
+__global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
+  constexpr int BW = BT / 64;                // waves per workgroup
   constexpr int RPT = LM / TPC;
   constexpr int CPW = BT / TPC;              // largest n
   constexpr int NTL = (LM + 15) / 16;        // 16-row MFMA tiles of the sketch
   constexpr int SPW = NTL <= 3 ? 2 : 1;      // strips per wave per pass
-  constexpr int MC = NTL <= 3 ? 256 : 128;   // rows of A per Omega chunk
+  constexpr int MC = NTL <= 3 && BT == 1024 ? 256 : 128;  // rows of A per Omega chunk
   constexpr int OLD = MC + 2;                // LDS row stride of the chunk (16-byte rows)
   constexpr int KM = LM < 64 ? LM : 64;      // R11 capacity (k < the round's target <= 64)
   constexpr int NR = (LM + 63) / 64;         // rows per lane in the reflector build
@@ -618,13 +619,15 @@ __global__ __launch_bounds__(BT) void id_batched_kernel(BStage p) {
   }
 }
 
-// the kernel variants: sketch orders up to 40 with one thread per column (n <= 1024); up to
-// 72 with two threads per column, one workgroup for n <= 512 and a workgroup pair for
-// n <= 1024
+// the kernel variants.  Order 40 (one thread per column): 512-thread workgroups, two per CU,
+// for n <= 512 and one 1024-thread workgroup for 512 < n <= 1024.  Order 72 (two threads per
+// column): one workgroup for n <= 512 and a workgroup pair for 512 < n <= 1024.
 struct Variant {
-  int lmax, maxn, pair_maxn;
+  int lmax, maxn, maxn1;  // one-workgroup kernel up to maxn, second kernel up to maxn1
+  bool pairs;             // the second kernel is the workgroup pair (else one big workgroup)
+  int per_cu;             // co-resident workgroups per CU of the first kernel
 };
-constexpr Variant kVariants[] = {{40, 1024, 0}, {72, 512, 1024}};
+constexpr Variant kVariants[] = {{40, 512, 1024, false, 2}, {72, 512, 1024, true, 1}};
 
 int variant_of(i64 ell) {
   for (int v = 0; v < 2; ++v)
@@ -632,10 +635,15 @@ int variant_of(i64 ell) {
   return -1;
 }
 
-hipError_t launch_variant(int v, bool pairs, const BStage& s, int grid, hipStream_t st) {
-  if (v == 0) hipLaunchKernelGGL((id_batched_kernel<40, 1, 1>), dim3(grid), dim3(BT), 0, st, s);
-  else if (!pairs) hipLaunchKernelGGL((id_batched_kernel<72, 2, 1>), dim3(grid), dim3(BT), 0, st, s);
-  else hipLaunchKernelGGL((id_batched_kernel<72, 2, 2>), dim3(grid), dim3(BT), 0, st, s);
+hipError_t launch_variant(int v, int kind, const BStage& s, int grid, hipStream_t st) {
+  if (v == 0 && kind == 0)
+    hipLaunchKernelGGL((id_batched_kernel<40, 1, 1, 512>), dim3(grid), dim3(512), 0, st, s);
+  else if (v == 0)
+    hipLaunchKernelGGL((id_batched_kernel<40, 1, 1, 1024>), dim3(grid), dim3(1024), 0, st, s);
+  else if (kind == 0)
+    hipLaunchKernelGGL((id_batched_kernel<72, 2, 1, 1024>), dim3(grid), dim3(1024), 0, st, s);
+  else
+    hipLaunchKernelGGL((id_batched_kernel<72, 2, 2, 1024>), dim3(grid), dim3(1024), 0, st, s);
   return hipGetLastError();
 }
 
@@ -654,7 +662,7 @@ struct lra_batch_plan {
   std::vector<double*> T;
   std::vector<int> direct;  // blocks that go to lra_idfact without the batched kernels
   int nq[2] = {0, 0};       // stage-0 blocks for the one-workgroup / pair kernels
-  int grid = 0, npairs = 0, nstage = 0;
+  int grid = 0, npairs = 0, nstage = 0;  // grid: workgroups the scratch slabs cover
   int ell[NSTAGE] = {}, target[NSTAGE] = {}, variant[NSTAGE] = {};
   BDesc* d_desc = nullptr;
   int* d_lists = nullptr;        // [stage][kind] lists of nblk entries (kind 1: pairs)
@@ -692,7 +700,7 @@ static int enqueue(lra_batch_plan* P) {
   for (int r = 0; r < P->nstage; ++r) {
     const Variant& V = kVariants[P->variant[r]];
     for (int kind = 0; kind < 2; ++kind) {
-      if (kind == 1 && V.pair_maxn == 0) continue;
+      if (kind == 1 && V.maxn1 == 0) continue;
       if (r == 0 && P->nq[kind] == 0) continue;
       BStage s;
       s.desc = P->d_desc;
@@ -709,9 +717,9 @@ static int enqueue(lra_batch_plan* P) {
       s.next_list = more ? list(r + 1, 0) : nullptr;
       s.next_n = more ? P->d_counts + 2 * NSTAGE + 2 * (r + 1) : nullptr;
       s.next_maxn = more ? N->maxn : 0;
-      s.next_list2 = more && N->pair_maxn ? list(r + 1, 1) : nullptr;
-      s.next_n2 = more && N->pair_maxn ? P->d_counts + 2 * NSTAGE + 2 * (r + 1) + 1 : nullptr;
-      s.next_maxn2 = more ? N->pair_maxn : 0;
+      s.next_list2 = more && N->maxn1 ? list(r + 1, 1) : nullptr;
+      s.next_n2 = more && N->maxn1 ? P->d_counts + 2 * NSTAGE + 2 * (r + 1) + 1 : nullptr;
+      s.next_maxn2 = more ? N->maxn1 : 0;
       s.ell = P->ell[r];
       s.rank = (int)std::min<i64>(P->o.rank, INT_MAX);
       s.nb = (int)P->o.nb;
@@ -722,13 +730,17 @@ static int enqueue(lra_batch_plan* P) {
       s.uncached = P->uncached ? 1 : 0;
       s.gen = P->d_counts + 4 * NSTAGE;
       s.err = c->d_err;
-      const int grid = kind == 0 ? P->grid : 2 * P->npairs;
+      const int work = r == 0 ? P->nq[kind] : (int)P->nblk;
+      const bool pair_kernel = kind == 1 && V.pairs;
+      const int grid = pair_kernel ? 2 * P->npairs
+                                   : std::max(1, std::min(work, (kind == 0 ? V.per_cu : 1) *
+                                                                    c->num_cu));
       hipError_t e;
-      if (kind == 1) {
+      if (pair_kernel) {
         SpinSection gate(c);
-        e = launch_variant(P->variant[r], true, s, grid, c->stream);
+        e = launch_variant(P->variant[r], kind, s, grid, c->stream);
       } else {
-        e = launch_variant(P->variant[r], false, s, grid, c->stream);
+        e = launch_variant(P->variant[r], kind, s, grid, c->stream);
       }
       if (e != hipSuccess) {
         lra_set_error("batched launch: %s", hipGetErrorString(e));
@@ -806,7 +818,7 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
   for (i64 b = 0; b < nblk; ++b) {
     // stage 0 needs the block's width within its variant, and T rows for its largest rank
     const int kind = n[b] <= V0.maxn ? 0 : 1;
-    const bool ok = P->nstage > 0 && n[b] >= 1 && n[b] <= std::max(V0.maxn, V0.pair_maxn) &&
+    const bool ok = P->nstage > 0 && n[b] >= 1 && n[b] <= std::max(V0.maxn, V0.maxn1) &&
                     m[b] <= INT_MAX &&
                     ldt[b] >= std::max<i64>(1, std::min<i64>(P->ell[0], n[b]));
     if (ok) {
@@ -824,15 +836,16 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
       for (int b : q[kind]) all = all && ldt[b] >= std::min<i64>(P->ell[r], n[b]);
     if (!all) { P->nstage = r; break; }
   }
-  bool pairs = V0.pair_maxn > 0 && !q[1].empty();
-  for (int r = 1; r < P->nstage; ++r) pairs = pairs || kVariants[P->variant[r]].pair_maxn > 0;
+  bool pairs = V0.pairs && !q[1].empty();
+  for (int r = 1; r < P->nstage; ++r) pairs = pairs || kVariants[P->variant[r]].pairs;
   for (int kind = 0; kind < 2; ++kind) {
     std::stable_sort(q[kind].begin(), q[kind].end(),
                      [&](int x, int y) { return m[x] * n[x] > m[y] * n[y]; });
     P->nq[kind] = (int)q[kind].size();
   }
-  P->grid = std::max(1, std::min(std::max(P->nq[0], P->nstage > 1 ? (int)nblk : 0),
-                                 c->num_cu));
+  // scratch slabs: the most workgroups any launch has (two per CU for the 512-thread kernel)
+  P->grid = std::max(1, std::min(std::max({P->nq[0], P->nq[1], P->nstage > 1 ? (int)nblk : 0}),
+                                 2 * c->num_cu));
   P->npairs = std::max(1, c->num_cu / 2);
   int st = LRA_OK;
   if (pairs) {
