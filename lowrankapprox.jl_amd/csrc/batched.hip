@@ -40,6 +40,7 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int SLAB = 40 * 1024;  // scratch doubles per workgroup (>= LM * columns, every variant)
 constexpr int NSTAGE = 4;
+constexpr int NKIND = 3;  // kernels per stage: small / one workgroup / workgroup pair
 
 struct BDesc {
   const double* A;
@@ -61,12 +62,9 @@ struct BStage {
   i64 ldo;
   double* scratch;          // SLAB doubles per workgroup
   i64* k_out;               // per block: rank, or -1 (to lra_idfact)
-  int* next_list;           // rejected blocks for the next stage's one-workgroup kernel
-  unsigned* next_n;
-  int next_maxn;
-  int* next_list2;          // ... and for its workgroup-pair kernel (wider blocks)
-  unsigned* next_n2;
-  int next_maxn2;
+  int* next_list[NKIND];    // rejected blocks for the next stage's kernels (by width)
+  unsigned* next_n[NKIND];
+  int next_maxn[NKIND];
   int ell, rank, nb, target;
   double atol, rtol;
   // workgroup pairs: the exchange slots, launch generation (tags) and error word
@@ -161,7 +159,8 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
   constexpr int CPW = BT / TPC;              // largest n
   constexpr int NTL = (LM + 15) / 16;        // 16-row MFMA tiles of the sketch
   constexpr int SPW = NTL <= 3 ? 2 : 1;      // strips per wave per pass
-  constexpr int MC = NTL <= 3 && BT == 1024 ? 256 : 128;  // rows of A per Omega chunk
+  // rows of A per Omega chunk (512-thread workgroups: small enough for two per CU)
+  constexpr int MC = BT == 1024 ? (NTL <= 3 ? 256 : 128) : (NTL <= 3 ? 128 : 32);
   constexpr int OLD = MC + 2;                // LDS row stride of the chunk (16-byte rows)
   constexpr int KM = LM < 64 ? LM : 64;      // R11 capacity (k < the round's target <= 64)
   constexpr int NR = (LM + 63) / 64;         // rows per lane in the reflector build
@@ -575,14 +574,13 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
     if (reject) {
       if (tid == 0 && part == 0) {
         p.k_out[b] = -1;
-        if (p.next_list && n <= p.next_maxn) {
-          const unsigned slot =
-              __hip_atomic_fetch_add(p.next_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          p.next_list[slot] = b;
-        } else if (p.next_list2 && n <= p.next_maxn2) {
-          const unsigned slot =
-              __hip_atomic_fetch_add(p.next_n2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          p.next_list2[slot] = b;
+        for (int kk = 0; kk < NKIND; ++kk) {
+          if (p.next_list[kk] && n <= p.next_maxn[kk]) {
+            const unsigned slot = __hip_atomic_fetch_add(p.next_n[kk], 1u, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            p.next_list[kk][slot] = b;
+            break;
+          }
         }
       }
       continue;
@@ -619,15 +617,18 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
   }
 }
 
-// the kernel variants.  Order 40 (one thread per column): 512-thread workgroups, two per CU,
-// for n <= 512 and one 1024-thread workgroup for 512 < n <= 1024.  Order 72 (two threads per
-// column): one workgroup for n <= 512 and a workgroup pair for 512 < n <= 1024.
+// the kernel variants: per order, up to three kernels by block width.  Order 40 (one thread
+// per column): 512-thread workgroups, two per CU, for n <= 512 and one 1024-thread workgroup
+// for 512 < n <= 1024.  Order 72 (two threads per column): 512-thread workgroups for
+// n <= 256, 1024-thread ones for n <= 512 and workgroup pairs for 512 < n <= 1024.
 struct Variant {
-  int lmax, maxn, maxn1;  // one-workgroup kernel up to maxn, second kernel up to maxn1
-  bool pairs;             // the second kernel is the workgroup pair (else one big workgroup)
-  int per_cu;             // co-resident workgroups per CU of the first kernel
+  int lmax;
+  int maxn[NKIND];  // widest block of each kernel (0: none)
 };
-constexpr Variant kVariants[] = {{40, 512, 1024, false, 2}, {72, 512, 1024, true, 1}};
+constexpr Variant kVariants[] = {{40, {512, 1024, 0}}, {72, {256, 512, 1024}}};
+// kernel kind: 0 = 512-thread workgroups (two per CU), 1 = one 1024-thread workgroup,
+// 2 = workgroup pair of 1024-thread workgroups
+constexpr int kPerCu[NKIND] = {2, 1, 0};
 
 int variant_of(i64 ell) {
   for (int v = 0; v < 2; ++v)
@@ -641,6 +642,8 @@ hipError_t launch_variant(int v, int kind, const BStage& s, int grid, hipStream_
   else if (v == 0)
     hipLaunchKernelGGL((id_batched_kernel<40, 1, 1, 1024>), dim3(grid), dim3(1024), 0, st, s);
   else if (kind == 0)
+    hipLaunchKernelGGL((id_batched_kernel<72, 2, 1, 512>), dim3(grid), dim3(512), 0, st, s);
+  else if (kind == 1)
     hipLaunchKernelGGL((id_batched_kernel<72, 2, 1, 1024>), dim3(grid), dim3(1024), 0, st, s);
   else
     hipLaunchKernelGGL((id_batched_kernel<72, 2, 2, 1024>), dim3(grid), dim3(1024), 0, st, s);
@@ -661,13 +664,13 @@ struct lra_batch_plan {
   std::vector<i64*> sk, rd;
   std::vector<double*> T;
   std::vector<int> direct;  // blocks that go to lra_idfact without the batched kernels
-  int nq[2] = {0, 0};       // stage-0 blocks for the one-workgroup / pair kernels
+  int nq[NKIND] = {};       // stage-0 blocks of each kernel
   int grid = 0, npairs = 0, nstage = 0;  // grid: workgroups the scratch slabs cover
   int ell[NSTAGE] = {}, target[NSTAGE] = {}, variant[NSTAGE] = {};
   BDesc* d_desc = nullptr;
-  int* d_lists = nullptr;        // [stage][kind] lists of nblk entries (kind 1: pairs)
-  unsigned* d_counts = nullptr;  // [0, 2 NSTAGE): queue heads, [2 NSTAGE, 4 NSTAGE): list
-                                 // sizes, [4 NSTAGE]: launch generation
+  int* d_lists = nullptr;        // [stage][kind] lists of nblk entries
+  unsigned* d_counts = nullptr;  // [0, NK NSTAGE): queue heads, [NK NSTAGE, 2 NK NSTAGE): list
+                                 // sizes, [2 NK NSTAGE]: launch generation
   double* d_omega[NSTAGE] = {};
   i64 ldo = 0;
   double* d_scratch = nullptr;
@@ -692,34 +695,37 @@ static void plan_free(lra_batch_plan* P) {
 
 static int enqueue(lra_batch_plan* P) {
   lra_ctx* c = P->c;
-  if (P->nq[0] + P->nq[1] == 0) return LRA_OK;
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, c->stream, P->d_counts, 4 * NSTAGE,
-                     P->d_counts + 4 * NSTAGE);
+  if (P->nq[0] + P->nq[1] + P->nq[2] == 0) return LRA_OK;
+  constexpr int NC = NKIND * NSTAGE;  // queue heads, then list sizes
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, c->stream, P->d_counts, 2 * NC,
+                     P->d_counts + 2 * NC);
   LRA_HIP(hipGetLastError());
-  auto list = [&](int r, int kind) { return P->d_lists + (size_t)(2 * r + kind) * P->nblk; };
+  auto list = [&](int r, int kind) {
+    return P->d_lists + (size_t)(NKIND * r + kind) * P->nblk;
+  };
   for (int r = 0; r < P->nstage; ++r) {
     const Variant& V = kVariants[P->variant[r]];
-    for (int kind = 0; kind < 2; ++kind) {
-      if (kind == 1 && V.maxn1 == 0) continue;
+    for (int kind = 0; kind < NKIND; ++kind) {
+      if (V.maxn[kind] == 0) continue;
       if (r == 0 && P->nq[kind] == 0) continue;
       BStage s;
       s.desc = P->d_desc;
       s.list = list(r, kind);
-      s.list_n = r == 0 ? nullptr : P->d_counts + 2 * NSTAGE + 2 * r + kind;
+      s.list_n = r == 0 ? nullptr : P->d_counts + NC + NKIND * r + kind;
       s.nq = r == 0 ? P->nq[kind] : (int)P->nblk;
-      s.head = P->d_counts + 2 * r + kind;
+      s.head = P->d_counts + NKIND * r + kind;
       s.omega = P->d_omega[r];
       s.ldo = P->ldo;
       s.scratch = P->d_scratch;
       s.k_out = P->d_k;
       const bool more = r + 1 < P->nstage;
       const Variant* N = more ? &kVariants[P->variant[r + 1]] : nullptr;
-      s.next_list = more ? list(r + 1, 0) : nullptr;
-      s.next_n = more ? P->d_counts + 2 * NSTAGE + 2 * (r + 1) : nullptr;
-      s.next_maxn = more ? N->maxn : 0;
-      s.next_list2 = more && N->maxn1 ? list(r + 1, 1) : nullptr;
-      s.next_n2 = more && N->maxn1 ? P->d_counts + 2 * NSTAGE + 2 * (r + 1) + 1 : nullptr;
-      s.next_maxn2 = more ? N->maxn1 : 0;
+      for (int kk = 0; kk < NKIND; ++kk) {
+        const bool has = more && N->maxn[kk] > 0;
+        s.next_list[kk] = has ? list(r + 1, kk) : nullptr;
+        s.next_n[kk] = has ? P->d_counts + NC + NKIND * (r + 1) + kk : nullptr;
+        s.next_maxn[kk] = has ? N->maxn[kk] : 0;
+      }
       s.ell = P->ell[r];
       s.rank = (int)std::min<i64>(P->o.rank, INT_MAX);
       s.nb = (int)P->o.nb;
@@ -728,13 +734,12 @@ static int enqueue(lra_batch_plan* P) {
       s.rtol = P->o.rtol;
       s.xchg = P->xchg;
       s.uncached = P->uncached ? 1 : 0;
-      s.gen = P->d_counts + 4 * NSTAGE;
+      s.gen = P->d_counts + 2 * NC;
       s.err = c->d_err;
       const int work = r == 0 ? P->nq[kind] : (int)P->nblk;
-      const bool pair_kernel = kind == 1 && V.pairs;
+      const bool pair_kernel = kind == 2;
       const int grid = pair_kernel ? 2 * P->npairs
-                                   : std::max(1, std::min(work, (kind == 0 ? V.per_cu : 1) *
-                                                                    c->num_cu));
+                                   : std::max(1, std::min(work, kPerCu[kind] * c->num_cu));
       hipError_t e;
       if (pair_kernel) {
         SpinSection gate(c);
@@ -812,14 +817,14 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     if (!adap) break;
     tg *= 2;
   }
-  std::vector<int> q[2];
+  std::vector<int> q[NKIND];
   i64 mmax = 0;
   const Variant& V0 = kVariants[P->variant[0]];
   for (i64 b = 0; b < nblk; ++b) {
     // stage 0 needs the block's width within its variant, and T rows for its largest rank
-    const int kind = n[b] <= V0.maxn ? 0 : 1;
-    const bool ok = P->nstage > 0 && n[b] >= 1 && n[b] <= std::max(V0.maxn, V0.maxn1) &&
-                    m[b] <= INT_MAX &&
+    int kind = 0;
+    while (kind < NKIND && (V0.maxn[kind] == 0 || n[b] > V0.maxn[kind])) ++kind;
+    const bool ok = P->nstage > 0 && n[b] >= 1 && kind < NKIND && m[b] <= INT_MAX &&
                     ldt[b] >= std::max<i64>(1, std::min<i64>(P->ell[0], n[b]));
     if (ok) {
       q[kind].push_back((int)b);
@@ -832,19 +837,20 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
   // block's ldt allows are dropped (forwarding is decided on the device from n alone)
   for (int r = 1; r < P->nstage; ++r) {
     bool all = true;
-    for (int kind = 0; kind < 2; ++kind)
+    for (int kind = 0; kind < NKIND; ++kind)
       for (int b : q[kind]) all = all && ldt[b] >= std::min<i64>(P->ell[r], n[b]);
     if (!all) { P->nstage = r; break; }
   }
-  bool pairs = V0.pairs && !q[1].empty();
-  for (int r = 1; r < P->nstage; ++r) pairs = pairs || kVariants[P->variant[r]].pairs;
-  for (int kind = 0; kind < 2; ++kind) {
+  bool pairs = V0.maxn[2] > 0 && !q[2].empty();
+  for (int r = 1; r < P->nstage; ++r) pairs = pairs || kVariants[P->variant[r]].maxn[2] > 0;
+  for (int kind = 0; kind < NKIND; ++kind) {
     std::stable_sort(q[kind].begin(), q[kind].end(),
                      [&](int x, int y) { return m[x] * n[x] > m[y] * n[y]; });
     P->nq[kind] = (int)q[kind].size();
   }
   // scratch slabs: the most workgroups any launch has (two per CU for the 512-thread kernel)
-  P->grid = std::max(1, std::min(std::max({P->nq[0], P->nq[1], P->nstage > 1 ? (int)nblk : 0}),
+  P->grid = std::max(1, std::min(std::max({P->nq[0], P->nq[1], P->nq[2],
+                                           P->nstage > 1 ? (int)nblk : 0}),
                                  2 * c->num_cu));
   P->npairs = std::max(1, c->num_cu / 2);
   int st = LRA_OK;
@@ -853,7 +859,7 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     st = lra_coop_area(c, sizeof(u64x2) * 4 * 74 * 2 * (size_t)P->npairs, &xp, &P->uncached);
     P->xchg = (u64x2*)xp;
   }
-  if (st == LRA_OK && P->nq[0] + P->nq[1] > 0) {
+  if (st == LRA_OK && P->nq[0] + P->nq[1] + P->nq[2] > 0) {
     std::vector<BDesc> hd(nblk);
     for (i64 b = 0; b < nblk; ++b) {
       hd[b].A = A[b]; hd[b].lda = lda[b]; hd[b].sk = sk[b]; hd[b].rd = rd[b]; hd[b].T = T[b];
@@ -863,10 +869,12 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     P->ldo = std::max<i64>(2, (mmax + 1) & ~(i64)1);
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMalloc((void**)&P->d_desc, sizeof(BDesc) * nblk);
-    if (e == hipSuccess) e = hipMalloc((void**)&P->d_lists, sizeof(int) * nblk * 2 * NSTAGE);
-    if (e == hipSuccess) e = hipMalloc((void**)&P->d_counts, sizeof(unsigned) * (4 * NSTAGE + 1));
     if (e == hipSuccess)
-      e = hipMemsetAsync(P->d_counts, 0, sizeof(unsigned) * (4 * NSTAGE + 1), c->stream);
+      e = hipMalloc((void**)&P->d_lists, sizeof(int) * nblk * NKIND * NSTAGE);
+    if (e == hipSuccess)
+      e = hipMalloc((void**)&P->d_counts, sizeof(unsigned) * (2 * NKIND * NSTAGE + 1));
+    if (e == hipSuccess)
+      e = hipMemsetAsync(P->d_counts, 0, sizeof(unsigned) * (2 * NKIND * NSTAGE + 1), c->stream);
     for (int r = 0; r < P->nstage && e == hipSuccess; ++r)
       e = hipMalloc((void**)&P->d_omega[r], sizeof(double) * P->ell[r] * P->ldo);
     if (e == hipSuccess)
@@ -876,7 +884,7 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     if (e == hipSuccess)
       e = hipMemcpyAsync(P->d_desc, hd.data(), sizeof(BDesc) * nblk, hipMemcpyHostToDevice,
                          c->stream);
-    for (int kind = 0; kind < 2 && e == hipSuccess; ++kind)
+    for (int kind = 0; kind < NKIND && e == hipSuccess; ++kind)
       if (P->nq[kind] > 0)
         e = hipMemcpyAsync(P->d_lists + (size_t)kind * nblk, q[kind].data(),
                            sizeof(int) * P->nq[kind], hipMemcpyHostToDevice, c->stream);
@@ -947,7 +955,7 @@ extern "C" int lra_idfact_batched_finish(lra_batch_plan* P, int64_t* k_out,
   lra_ctx* c = P->c;
   LRA_HIP(hipSetDevice(c->device));
   std::vector<i64> hk(P->nblk, -1);
-  if (P->nq[0] + P->nq[1] > 0) {
+  if (P->nq[0] + P->nq[1] + P->nq[2] > 0) {
     LRA_HIP(hipMemcpyAsync(hk.data(), P->d_k, sizeof(i64) * P->nblk, hipMemcpyDeviceToHost,
                            c->stream));
   }
