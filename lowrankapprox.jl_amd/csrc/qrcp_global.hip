@@ -162,31 +162,55 @@ __global__ __launch_bounds__(GT) void qg_step(int s, int m, int n, int kmax, int
   const int pvt = pick_body(s, n, kmax, nb, stop_after, vn1, diag, st);
   if (pvt < 0) return;
   double* cs = B + (i64)s * ldb;
-  if (pvt != s) {
-    double* cp = B + (i64)pvt * ldb;
-    for (int i = threadIdx.x; i < m; i += GT) {
-      const double t = cs[i];
-      cs[i] = cp[i];
-      cp[i] = t;
+  double* cp = B + (i64)pvt * ldb;
+  // LAPACK's swap of columns s and pvt, fused with the first dlarfg pass: each thread keeps
+  // its rows of the new column s (up to MAXR per thread) and sums the squares below row s
+  constexpr int MAXR = 8;
+  const bool regs = m <= GT * MAXR;
+  double xr[MAXR];
+  double ss = 0.0;
+  if (regs) {
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int i = (int)threadIdx.x + r * GT;
+      double x = 0.0;
+      if (i < m) {
+        x = cp[i];
+        if (pvt != s) {
+          const double t = cs[i];
+          cs[i] = x;
+          cp[i] = t;
+        }
+      }
+      xr[r] = x;
+      if (i > s && i < m) ss += x * x;
+      if (i == s) s_alpha = x;
     }
-    if (threadIdx.x == 0) {
-      const i64 t = jpvt[pvt];
-      jpvt[pvt] = jpvt[s];
-      jpvt[s] = t;
-      vn1[pvt] = vn1[s];
-      vn2[pvt] = vn2[s];
+  } else {
+    for (int i = threadIdx.x; i < m; i += GT) {
+      const double x = cp[i];
+      if (pvt != s) {
+        const double t = cs[i];
+        cs[i] = x;
+        cp[i] = t;
+      }
+      if (i > s) ss += x * x;
+      if (i == s) s_alpha = x;
     }
   }
-  __syncthreads();
-  // dlarfg on cs[s..m-1]
-  double ss = 0.0;
-  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) ss += cs[i] * cs[i];
+  if (pvt != s && threadIdx.x == 0) {
+    const i64 t = jpvt[pvt];
+    jpvt[pvt] = jpvt[s];
+    jpvt[s] = t;
+    vn1[pvt] = vn1[s];
+    vn2[pvt] = vn2[s];
+  }
   ss = group_sum(ss, 64);
   if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = ss;
-  if (threadIdx.x == 0) s_alpha = cs[s];
-  __syncthreads();
+  __syncthreads();  // (also makes every thread's swapped rows visible to the others)
   double tot = 0.0;
   for (int w = 0; w < GT / 64; ++w) tot += sw[w];
+  // dlarfg on cs[s..m-1]
   double alpha = s_alpha, xnorm = sqrt(tot), beta = alpha, t = 0.0, scal = 1.0, fac = 1.0;
   if (m - s > 1 && xnorm != 0.0) {
     beta = -copysign(lapy2(alpha, xnorm), alpha);
@@ -218,10 +242,22 @@ __global__ __launch_bounds__(GT) void qg_step(int s, int m, int n, int kmax, int
     scal = 1.0 / (alpha - beta);
     for (int j = 0; j < knt; ++j) beta *= safmin;
   }
-  for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
-    const double x = (cs[i] * fac) * scal;
-    cs[i] = x;
-    v[i] = x;
+  if (regs) {
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      const int i = (int)threadIdx.x + r * GT;
+      if (i > s && i < m) {
+        const double x = (xr[r] * fac) * scal;
+        cs[i] = x;
+        v[i] = x;
+      }
+    }
+  } else {
+    for (int i = s + 1 + (int)threadIdx.x; i < m; i += GT) {
+      const double x = (cs[i] * fac) * scal;
+      cs[i] = x;
+      v[i] = x;
+    }
   }
   if (threadIdx.x == 0) {
     cs[s] = beta;
