@@ -636,9 +636,10 @@ def main():
     if mixed:
         mixed[0]["note"] = ("rtol=1e-10 is below the low-precision noise floor (1e-6): rounds "
                             "whose target the low-precision sketch certifies above the floor "
-                            "run in low precision, the later ones in FP64 (bit-for-bit the "
-                            "FP64 drop-in's result); the pivoted QR, precision-independent, "
-                            "is about two thirds of the time")
+                            "run in low precision, the later ones in FP64 (the FP64 drop-in's "
+                            "rounds, rank and skeleton); the pivoted QR, precision-independent, "
+                            "is about two thirds of the time, and the low-precision rounds "
+                            "run one by one where the FP64 drop-in pairs them")
     err64 = id_error(lra, A, V64)
     roof, roof_gemm = kernel_roofline(lra, torch, A)
     psvd = bench_psvd(lra, torch, A)

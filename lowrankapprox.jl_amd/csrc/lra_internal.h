@@ -46,7 +46,7 @@ struct lra_ctx {
   int num_cu = 0;
   int lds_bytes = 0;
   // workspace slots: grown on demand, never shrunk until destroy
-  static const int kSlots = 64;
+  static const int kSlots = 80;
   void* ws[kSlots] = {};
   size_t ws_size[kSlots] = {};
   // small pinned host staging area for scalars read back synchronously
@@ -115,7 +115,7 @@ void lra_opts_for_operator(lra_opts* o);
 // named workspace slots (each call path uses fixed slots so they can overlap safely)
 enum WsSlot {
   WS_OMEGA = 0, WS_SKETCH, WS_SKETCH2, WS_SPLITK, WS_JPVT, WS_TAU, WS_SCAL, WS_R,
-  WS_T, WS_TMP0, WS_TMP1, WS_TMP2, WS_TMP3, WS_LP0, WS_LP1, WS_BATCH0, WS_BATCH1,
+  WS_T, WS_TMP0, WS_TMP1, WS_TMP2, WS_TMP3, WS_LP0, WS_LP1, WS_LP2, WS_BATCH0, WS_BATCH1,
   WS_BATCH2, WS_QR0, WS_QR1, WS_QR2, WS_X0, WS_MD0, WS_MD1, WS_OP0, WS_OP1, WS_CUR0,
   WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
   WS_PI0, WS_PI1, WS_PI2, WS_QG, WS_PR0, WS_PR1, WS_PR2, WS_PR3, WS_PR4, WS_PR5, WS_PR6,

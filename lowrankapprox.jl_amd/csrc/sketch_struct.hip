@@ -206,10 +206,14 @@ __device__ __forceinline__ void dft_reg(cplx* x) {
 // global loads).  The kernel below exchanges data between threads through LDS only.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <int L1, int L2, int MAXS>
+// SIGNS: d is +-1 (drawn by the library, as srft_rand draws it for real types) and given as a
+// bit mask (bit i set: d[i] = -1), staged in LDS once per workgroup; d x is then x with its
+// sign flipped where the bit is set, exactly the product, without loading d per slab.
+template <int L1, int L2, int MAXS, bool SIGNS>
 __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
     i64 m, const double* __restrict__ A, i64 lda, i64 mm, const double* __restrict__ d,
-    const i64* __restrict__ idx, int ell, double* __restrict__ B, i64 ldb) {
+    const uint32_t* __restrict__ dbits, const i64* __restrict__ idx, int ell,
+    double* __restrict__ B, i64 ldb) {
   constexpr int L = L1 * L2;
   constexpr int QC = 256 / (L1 > L2 ? L1 : L2);  // sequences per slab
   // LDS padding: each k1 block of Y and each row of Z is one complex longer than its data, so
@@ -219,6 +223,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
   extern __shared__ double lds[];
   cplx* Y = reinterpret_cast<cplx*>(lds);  // [k1][p2 or k2][c], k1 blocks of YB
   cplx* tw = Y + L1 * YB;                 // w_l^j, j < l
+  uint32_t* sb = reinterpret_cast<uint32_t*>(tw + L);  // SIGNS: the m-bit sign mask
   const int tid = threadIdx.x;
   const i64 col = blockIdx.x;
   const double* x = A + col * lda;
@@ -226,6 +231,9 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
     double sn, cs;
     sincospi(-2.0 * (double)j / (double)L, &sn, &cs);
     tw[j] = {cs, sn};
+  }
+  if constexpr (SIGNS) {
+    for (int t = tid; t < (int)((m + 31) / 32); t += 256) sb[t] = dbits[t];
   }
   const int nslots = (ell + 1) / 2;
   cplx acc[MAXS], wstep[MAXS], wstep2[MAXS];
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
   const i64 nq = mm;
   // 16-byte loads of column pairs when every pair is 16-byte aligned in both x and d
   const bool vec = (mm & 1) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(d) & 15) == 0;
+                   (SIGNS || (reinterpret_cast<uintptr_t>(d) & 15) == 0);
   lds_barrier();  // the twiddles are written by other waves than the ones reading them
   for (i64 q0 = 0; q0 < nq; q0 += 2 * QC) {
     // pass 1.  Every load is unconditional, from a clamped address, with the mask applied
@@ -277,10 +285,22 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
 #pragma unroll
         for (int a = 0; a < L1; ++a)
           rx[a] = *reinterpret_cast<const d2v*>(x + (i64)(a * L2 + p2) * mm + qv);
+        if constexpr (SIGNS) {
+          // rows pos, pos + 1 (pos even) share a mask word
 #pragma unroll
-        for (int a = 0; a < L1; ++a) {
-          const d2v dv = *reinterpret_cast<const d2v*>(d + (i64)(a * L2 + p2) * mm + qv);
-          v[a] = {ma ? dv.x * rx[a].x : 0.0, mb ? dv.y * rx[a].y : 0.0};
+          for (int a = 0; a < L1; ++a) {
+            const int pos = (int)((i64)(a * L2 + p2) * mm + qv);
+            const uint32_t w = sb[pos >> 5] >> (pos & 31);
+            const double x0 = (w & 1u) ? -rx[a].x : rx[a].x;
+            const double x1 = (w & 2u) ? -rx[a].y : rx[a].y;
+            v[a] = {ma ? x0 : 0.0, mb ? x1 : 0.0};
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < L1; ++a) {
+            const d2v dv = *reinterpret_cast<const d2v*>(d + (i64)(a * L2 + p2) * mm + qv);
+            v[a] = {ma ? dv.x * rx[a].x : 0.0, mb ? dv.y * rx[a].y : 0.0};
+          }
         }
       } else {
         const i64 qa = min(q, nq - 1), qb = min(q + 1, nq - 1);
@@ -294,7 +314,14 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
 #pragma unroll
         for (int a = 0; a < L1; ++a) {
           const i64 base = (i64)(a * L2 + p2) * mm;
-          v[a] = {ma ? d[base + qa] * rx[a][0] : 0.0, mb ? d[base + qb] * rx[a][1] : 0.0};
+          if constexpr (SIGNS) {
+            const int pa = (int)(base + qa), pb = (int)(base + qb);
+            const double x0 = ((sb[pa >> 5] >> (pa & 31)) & 1u) ? -rx[a][0] : rx[a][0];
+            const double x1 = ((sb[pb >> 5] >> (pb & 31)) & 1u) ? -rx[a][1] : rx[a][1];
+            v[a] = {ma ? x0 : 0.0, mb ? x1 : 0.0};
+          } else {
+            v[a] = {ma ? d[base + qa] * rx[a][0] : 0.0, mb ? d[base + qb] * rx[a][1] : 0.0};
+          }
         }
       }
       dft_reg<L1>(v);
@@ -359,19 +386,22 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 2 : 1) void srft_fft_kernel(
   }
 }
 
-template <int L1, int L2, int MAXS>
+template <int L1, int L2, int MAXS, bool SIGNS>
 int launch_srft_fft_s(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
-                      const double* d, const i64* idx, int ell, double* B, i64 ldb) {
+                      const double* d, const uint32_t* dbits, const i64* idx, int ell,
+                      double* B, i64 ldb) {
   constexpr int L = L1 * L2;
   constexpr int QC = 256 / (L1 > L2 ? L1 : L2);
-  // Y (L1 blocks of L2 QC + 1) and the L twiddles, 16 bytes each
-  const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + L) * 16;
+  // Y (L1 blocks of L2 QC + 1) and the L twiddles, 16 bytes each, then the sign mask
+  const size_t bytes = ((size_t)L1 * (L2 * QC + 1) + L) * 16 +
+                       (SIGNS ? (size_t)((P + 31) / 32) * 4 : 0);
   if (bytes > 64 * 1024) {
     static std::atomic<uint64_t> lds_set{0};  // per device, set once
-    lra_lds_limit_once(lds_set, (const void*)srft_fft_kernel<L1, L2, MAXS>, (int)bytes);
+    lra_lds_limit_once(lds_set, (const void*)srft_fft_kernel<L1, L2, MAXS, SIGNS>,
+                       (int)std::max<size_t>(bytes, 96 * 1024));
   }
-  hipLaunchKernelGGL((srft_fft_kernel<L1, L2, MAXS>), dim3((unsigned)Q), dim3(256), bytes, st,
-                     P, X, ldx, mm, d, idx, ell, B, ldb);
+  hipLaunchKernelGGL((srft_fft_kernel<L1, L2, MAXS, SIGNS>), dim3((unsigned)Q), dim3(256),
+                     bytes, st, P, X, ldx, mm, d, dbits, idx, ell, B, ldb);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     lra_set_error("srft launch: %s", hipGetErrorString(e));
@@ -380,13 +410,33 @@ int launch_srft_fft_s(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i6
   return LRA_OK;
 }
 
+// dbits: the sign mask of a library-drawn d (P <= kSignRows), or null (d as given)
 template <int L1, int L2>
 int launch_srft_fft(hipStream_t st, i64 Q, i64 P, const double* X, i64 ldx, i64 mm,
-                    const double* d, const i64* idx, int ell, double* B, i64 ldb) {
+                    const double* d, const uint32_t* dbits, const i64* idx, int ell, double* B,
+                    i64 ldb) {
   // output slots per thread: (ell + 1) / 2 slots over 256 threads
-  if (ell <= 512) return launch_srft_fft_s<L1, L2, 1>(st, Q, P, X, ldx, mm, d, idx, ell, B, ldb);
-  return launch_srft_fft_s<L1, L2, 4>(st, Q, P, X, ldx, mm, d, idx, ell, B, ldb);
+  if (ell <= 512 && dbits)
+    return launch_srft_fft_s<L1, L2, 1, true>(st, Q, P, X, ldx, mm, d, dbits, idx, ell, B, ldb);
+  if (ell <= 512)
+    return launch_srft_fft_s<L1, L2, 1, false>(st, Q, P, X, ldx, mm, d, dbits, idx, ell, B, ldb);
+  return launch_srft_fft_s<L1, L2, 4, false>(st, Q, P, X, ldx, mm, d, dbits, idx, ell, B, ldb);
 }
+
+// bits[w] bit j set when d[32 w + j] < 0
+__global__ void srft_sign_bits_kernel(i64 P, const double* __restrict__ d, uint32_t* bits) {
+  const i64 w = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+  if (w >= (P + 31) / 32) return;
+  uint32_t b = 0;
+  for (int j = 0; j < 32; ++j) {
+    const i64 i = 32 * w + j;
+    if (i < P && d[i] < 0.0) b |= 1u << j;
+  }
+  bits[w] = b;
+}
+
+// sign masks up to 65536 rows (8 KB): the kernel's LDS stays within two workgroups per CU
+constexpr i64 kSignRows = 65536;
 
 // B[i, j] = A[r[i]-1, j]
 __global__ void subset_kernel(i64 n, const double* A, i64 lda, int ell, const i64* r, double* B,
@@ -454,6 +504,7 @@ int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
   if (P <= 0) { lra_set_error("srft: empty operand"); return LRA_E_UNSUPPORTED; }
   if (ell > 2048) { lra_set_error("srft: ell=%lld > 2048", (long long)ell); return LRA_E_UNSUPPORTED; }
   if (Q > 2147483647ll) { lra_set_error("srft: too many columns"); return LRA_E_UNSUPPORTED; }
+  const uint32_t* dbits = nullptr;
   if (!d) {
     double* dd = nullptr;
     i64* ii = nullptr;
@@ -462,6 +513,15 @@ int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
     LRA_CHECK(lra_launch_srft_rand(c->stream, seed, stream_id, P, ell, dd, ii));
     d = dd;
     idx = ii;
+    if (P <= kSignRows) {  // d is +-1 by construction: pass it as a sign mask
+      uint32_t* bits = nullptr;
+      const i64 nw = (P + 31) / 32;
+      LRA_CHECK(lra_wsT(c, WS_LP2, (size_t)nw, &bits));
+      hipLaunchKernelGGL(srft_sign_bits_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0,
+                         c->stream, P, d, bits);
+      LRA_LAUNCH_CHECK("srft signs");
+      dbits = bits;
+    }
   }
   i64 l = std::min<i64>(ell, P);
   while (P % l) --l;
@@ -470,11 +530,11 @@ int lra_sketch_srft_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
   if (pow2 && ell <= 2048) {
     // two register passes for power-of-two l <= 256
     switch (l) {
-      case 256: return launch_srft_fft<16, 16>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
-      case 128: return launch_srft_fft<16, 8>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
-      case 64: return launch_srft_fft<8, 8>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
-      case 32: return launch_srft_fft<8, 4>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
-      case 16: return launch_srft_fft<4, 4>(c->stream, Q, P, X, ldx, mm, d, idx, (int)ell, B, ldb);
+      case 256: return launch_srft_fft<16, 16>(c->stream, Q, P, X, ldx, mm, d, dbits, idx, (int)ell, B, ldb);
+      case 128: return launch_srft_fft<16, 8>(c->stream, Q, P, X, ldx, mm, d, dbits, idx, (int)ell, B, ldb);
+      case 64: return launch_srft_fft<8, 8>(c->stream, Q, P, X, ldx, mm, d, dbits, idx, (int)ell, B, ldb);
+      case 32: return launch_srft_fft<8, 4>(c->stream, Q, P, X, ldx, mm, d, dbits, idx, (int)ell, B, ldb);
+      case 16: return launch_srft_fft<4, 4>(c->stream, Q, P, X, ldx, mm, d, dbits, idx, (int)ell, B, ldb);
       default: break;
     }
   }

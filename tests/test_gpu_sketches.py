@@ -123,14 +123,18 @@ def decaying(m, n, r, sig, seed):
 @pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16"])
 def test_mixed_below_floor_equals_fp64(gpu, prec):
     """rtol below the precision's noise floor: the decisive round is FP64 with the same Omega,
-    so the mixed result is the FP64 drop-in's."""
+    so the mixed result is the FP64 drop-in's factorization: the same rounds, rank and
+    skeleton.  Its sketch may come from a different GEMM (the drop-in forms two paired rounds'
+    sketches in one product), so T agrees to rounding and the ID errors match."""
     r = 300
     A = decaying(2000, 1500, r, 10.0 ** (-12 * np.arange(r) / (r - 1)), 1)
     V = gpu.idfact(A, seed=4, rtol=1e-10)
     Vm = gpu.idfact(A, seed=4, rtol=1e-10, precision=prec)
     assert V.k == Vm.k and V.rounds == Vm.rounds
     assert np.array_equal(host(V.sk), host(Vm.sk))
-    assert np.allclose(host(V.T), host(Vm.T), rtol=1e-12, atol=1e-12)
+    e = np.linalg.norm(A[:, host(V.rd) - 1] - A[:, host(V.sk) - 1] @ host(V.T), 2)
+    em = np.linalg.norm(A[:, host(Vm.rd) - 1] - A[:, host(Vm.sk) - 1] @ host(Vm.T), 2)
+    assert em <= 2 * e + 1e-14 * np.linalg.norm(A, 2), (e, em)
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16x3"])
