@@ -15,13 +15,15 @@
 //      matrix of Philox stream r shared by every block: its first m columns are exactly the
 //      Omega that idfact draws for an m-row block, so a block's result is the result of idfact
 //      on that block alone.  Omega is staged through LDS in row chunks; each wave computes
-//      16-column strips of B and stages them in a per-workgroup scratch slab;
+//      16-column strips of B, with the next 16 rows of A in flight while the current ones are
+//      multiplied, and stages them in a per-workgroup scratch slab;
 //   2. pivoted QR of B with TPC threads per column (ell/TPC rows each in registers): per step
-//      one barrier -- every wave publishes its best column (largest partial norm, then lowest
-//      position, as idamax) together with that column's values, then every wave builds the
-//      Householder reflector itself from the winner's values.  Norm downdating, the recompute
-//      criterion, the block boundaries of dlaqps and the stopping rule of geqp3_adap! are those
-//      of the single-matrix kernel in qrcp.hip;
+//      every wave publishes its best column (largest partial norm, then lowest position, as
+//      idamax) together with that column's values; after a barrier every wave picks the pivot
+//      (a DPP argmax over the wave candidates) and runs the block bookkeeping, wave 0 builds the
+//      Householder reflector into LDS, and after a second barrier every wave applies it to its
+//      columns.  Norm downdating, the recompute criterion, the block boundaries of dlaqps and
+//      the stopping rule of geqp3_adap! are those of the single-matrix kernel in qrcp.hip;
 //   3. sk / rd from the final column positions and T(:, j) = R11 \ R(1:k, j) by back
 //      substitution in each column's registers.
 #include "lra_internal.h"
@@ -30,12 +32,19 @@
 #include <algorithm>
 #include <climits>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));  // 8-byte aligned pair
+// global-memory views of A (the block pointers are read from the descriptor table, so the
+// compiler cannot infer their address space): global loads count against vmcnt only, and an
+// LDS wait does not wait for them
+typedef __attribute__((address_space(1))) d2u gd2u;
+typedef __attribute__((address_space(1))) double gdouble;
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int SLAB = 40 * 1024;  // scratch doubles per workgroup (>= LM * columns, every variant)
@@ -79,6 +88,9 @@ __global__ void reset_kernel(unsigned* c, int n, unsigned* gen) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
   if (threadIdx.x == 0) *gen = *gen + 1u;
 }
+
+// barrier for LDS hand-offs only: outstanding global loads into registers stay in flight
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // ---- exchange between the two workgroups of a pair (one column block split in halves) ----
 // A slot holds LM + 2 tagged 16-byte chunks: {norm}, {position | flag << 32}, then the
@@ -148,6 +160,59 @@ LRA_DEV bool xpoll(const u64x2* slot, uint64_t tag, int uncached, int* err, doub
   return true;
 }
 
+// rows k0 + 4 lq .. +3 of the columns of the wave's first NV strips: A is the block, voff[i]
+// the byte offset of row 4 lq of this lane's column in strip i; FULL: the whole group
+// k0 .. k0+15 lies within the m rows, else rows past m read as zero
+template <int SPW, int NV, bool FULL>
+LRA_DEV void batched_load_rows(const double* A, const i64 (&voff)[SPW], int k0, int m,
+                               d2 (&x)[SPW][2]) {
+  const char* base = (const char*)(A + k0);
+  if constexpr (FULL) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const gd2u* src = (const gd2u*)(base + voff[i]);
+      x[i][0] = src[0];
+      x[i][1] = src[1];
+    }
+  } else {
+    const int kr = k0 + 4 * (((int)threadIdx.x & 63) >> 4);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const gdouble* src = (const gdouble*)(base + voff[i]);
+      x[i][0].x = kr < m ? src[0] : 0.0;
+      x[i][0].y = kr + 1 < m ? src[1] : 0.0;
+      x[i][1].x = kr + 2 < m ? src[2] : 0.0;
+      x[i][1].y = kr + 3 < m ? src[3] : 0.0;
+    }
+  }
+}
+
+// acc[i] += Omega(:, kg .. kg+15 of the staged chunk) * x[i] for the wave's first NV strips
+// (rows ell .. LM-1 of the staged chunk are zero, so every row is read unconditionally)
+template <int LM, int SPW, int NV, int NTL, int OLD>
+LRA_DEV void batched_mfma_group(const double* sO, int kg, int lq, int l16,
+                                const d2 (&x)[SPW][2], d4 (&acc)[SPW][NTL]) {
+  const int kr = kg + 4 * lq;
+  double av[NTL][4];
+#pragma unroll
+  for (int t = 0; t < NTL; ++t) {
+    // (rows past LM, in the last tile, read row LM-1: they only reach output rows past LM)
+    const int row = 16 * t + l16 < LM ? 16 * t + l16 : LM - 1;
+    const d2 y0 = *reinterpret_cast<const d2*>(sO + row * OLD + kr);
+    const d2 y1 = *reinterpret_cast<const d2*>(sO + row * OLD + kr + 2);
+    av[t][0] = y0.x; av[t][1] = y0.y; av[t][2] = y1.x; av[t][3] = y1.y;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double bv[4] = {x[i][0].x, x[i][0].y, x[i][1].x, x[i][1].y};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < NTL; ++t)
+        acc[i][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][u], bv[u], acc[i][t], 0, 0, 0);
+  }
+}
+
 // LM: largest sketch order; TPC: threads per column (RPT = LM / TPC rows each); CL: workgroups
 // per block (2: each takes half the columns and the pair exchanges its pivot candidates);
 // BT: threads per workgroup (512: two workgroups per CU, so one's sketch overlaps the
@@ -167,19 +232,22 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
   static_assert(RPT * TPC == LM && LM * CPW <= SLAB && (RPT % 2) == 0, "shape");
 
   __shared__ __align__(16) double sO[LM * OLD];  // Omega chunk, row-major
-  __shared__ double sCand[2][BW][LM];            // each wave's best column, by step parity
-  __shared__ double sV[BW][LM];                  // each wave's copy of the reflector
+  __shared__ double sCand[BW][LM];               // each wave's best column
+  __shared__ double sX[LM];                      // the step's reflector, by absolute row
   __shared__ double sR[KM * KM];                 // R11, column-major
   __shared__ double sDiag[LM];
-  __shared__ double sVal[2][BW];
-  __shared__ int sPos[2][BW], sLane[2][BW], sFlag[2][BW];
+  __shared__ double sVal[BW];
+  __shared__ int sPos[BW], sLane[BW], sFlag[BW];
+  __shared__ double sTau;                        // the step's tau (written by wave 0)
   __shared__ double sMax[BW];
   __shared__ double sPc[LM];   // the partner's candidate column (pairs; wave 0 fetches it)
   __shared__ double sXv;       // ... its norm, position and recompute flag
   __shared__ int sXp, sXf;
   __shared__ int sBlk, sAbort;
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wave-uniform: the LDS addresses derived from it are scalar
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l16 = lane & 15, lq = lane >> 4;
   const int lc = tid / TPC, lic = tid % TPC;   // column, row part
   const int rbase = lic * RPT;
@@ -236,11 +304,14 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
     asm volatile("" : "+s"(ell));
 
     // ---------------- 1. sketch: Bs(0:ell, 0:n) = Omega(:, 0:m) * A ----------------------
+    // Each wave owns up to SPW 16-column strips per pass.  The rows of A are read in groups
+    // of 16 (lane: rows kr .. kr+3 of its column l16), one group ahead of the MFMAs that use
+    // them, so a group's loads are in flight while the previous group is multiplied; the
+    // Omega chunk (MC rows) is staged in LDS and shared by every strip.
     {
       const int nstrips = (max(nloc, 0) + 15) >> 4;
       const int spw = nstrips > BW ? SPW : 1;
       const int npass = (nstrips + BW * spw - 1) / (BW * spw);
-      const bool vec = ((d.lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(d.A) & 15) == 0);
       for (int pass = 0; pass < npass; ++pass) {
         d4 acc[SPW][NTL];
 #pragma unroll
@@ -248,75 +319,75 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
 #pragma unroll
           for (int t = 0; t < NTL; ++t) acc[i][t] = d4{0.0, 0.0, 0.0, 0.0};
         const int q0 = pass * BW * spw + wave * spw;
-        for (int c0 = 0; c0 < m; c0 += MC) {
-          const int mc = min(MC, m - c0);
-          __syncthreads();  // every wave is done with the previous chunk
-          for (int e = tid; e < LM * (MC / 2); e += BT) {
-            const int r = e / (MC / 2), cc = (e % (MC / 2)) * 2;
-            d2 v = {0.0, 0.0};
-            if (r < ell) {
-              const double* src = p.omega + (i64)r * p.ldo + c0 + cc;
-              if (cc + 1 < mc) v = *reinterpret_cast<const d2*>(src);
-              else if (cc < mc) v.x = src[0];
-            }
-            *reinterpret_cast<d2*>(sO + r * OLD + cc) = v;
-          }
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < SPW; ++i) {
-            const int q = q0 + i;
-            if (i >= spw || q >= nstrips) continue;  // wave-uniform
-            const int col = q * 16 + l16;
-            const bool cok = col < nloc;
-            const double* acol = d.A + (i64)(cok ? c0col + col : 0) * d.lda + c0;
-            for (int kg = 0; kg < mc; kg += 16) {
-              // this lane's four k values of the 16: kr .. kr+3 (both operands use the same
-              // k order, so the contraction is unchanged)
-              const int kr = kg + 4 * lq;
-              double bv[4];
-              if (vec && cok && kr + 3 < mc) {
-                const d2 x0 = *reinterpret_cast<const d2*>(acol + kr);
-                const d2 x1 = *reinterpret_cast<const d2*>(acol + kr + 2);
-                bv[0] = x0.x; bv[1] = x0.y; bv[2] = x1.x; bv[3] = x1.y;
-              } else {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) bv[u] = (cok && kr + u < mc) ? acol[kr + u] : 0.0;
-              }
-              double av[NTL][4];
-#pragma unroll
-              for (int t = 0; t < NTL; ++t) {
-                const int row = 16 * t + l16;
-                if (row < ell) {
-                  const d2 y0 = *reinterpret_cast<const d2*>(sO + row * OLD + kr);
-                  const d2 y1 = *reinterpret_cast<const d2*>(sO + row * OLD + kr + 2);
-                  av[t][0] = y0.x; av[t][1] = y0.y; av[t][2] = y1.x; av[t][3] = y1.y;
-                } else {
-                  av[t][0] = av[t][1] = av[t][2] = av[t][3] = 0.0;
-                }
-              }
-#pragma unroll
-              for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int t = 0; t < NTL; ++t)
-                  acc[i][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][u], bv[u], acc[i][t],
-                                                                   0, 0, 0);
-            }
-          }
-        }
-        // output element (row 16t + lq + 4r, column l16) of each 16 x 16 tile
+        int nvs = min(spw, nstrips - q0);  // this wave's strips in this pass (wave-uniform)
+        nvs = nvs < 0 ? 0 : nvs;
+        // column offsets (row 0); columns past nloc read column 0 of the half: their products
+        // land in output columns that are never stored
+        i64 voff[SPW];
 #pragma unroll
         for (int i = 0; i < SPW; ++i) {
-          const int q = q0 + i;
-          if (i >= spw || q >= nstrips) continue;
-          const int col = q * 16 + l16;
-          if (col < nloc) {
+          const int col = (q0 + i) * 16 + l16;
+          voff[i] = ((i64)(c0col + (col < nloc ? col : 0)) * d.lda + 4 * lq) * 8;
+        }
+        // NV: the wave's strips in this pass (a compile-time count, so every load below is
+        // unconditional and the wait for a group's rows leaves the next group's loads in flight)
+        auto run = [&](auto nv_c) {
+          constexpr int NV = decltype(nv_c)::value;
+          const int mfull = m & ~15;  // rows in whole 16-row groups
+          d2 xa[SPW][2], xb[SPW][2];  // two row groups in flight (ping-pong)
+          if (NV > 0 && mfull > 0) batched_load_rows<SPW, NV, true>(d.A, voff, 0, m, xa);
+          for (int c0 = 0; c0 < m; c0 += MC) {
+            const int mc = min(MC, m - c0);
+            lds_barrier();  // every wave is done with the previous chunk
+            for (int e = tid; e < LM * (MC / 2); e += BT) {
+              const int r = e / (MC / 2), cc = (e % (MC / 2)) * 2;
+              d2 v = {0.0, 0.0};
+              if (r < ell) {
+                const double* src = p.omega + (i64)r * p.ldo + c0 + cc;
+                if (cc + 1 < mc) v = *reinterpret_cast<const d2*>(src);
+                else if (cc < mc) v.x = src[0];
+              }
+              *reinterpret_cast<d2*>(sO + r * OLD + cc) = v;
+            }
+            lds_barrier();
+            if constexpr (NV > 0) {
+              // whole groups of this chunk, two per iteration, every load and MFMA
+              // unconditional, so waiting for one group leaves the next one's loads in flight.
+              // Every chunk but the last holds an even number of whole groups and starts on
+              // xa; the last iteration's xa load is the next chunk's first group, or the last
+              // whole group again (clamped, not used).
+              const int gend = min(mc, mfull - c0);
+              int kg = 0;
+              for (; kg + 32 <= gend; kg += 32) {
+                batched_load_rows<SPW, NV, true>(d.A, voff, c0 + kg + 16, m, xb);
+                batched_mfma_group<LM, SPW, NV, NTL, OLD>(sO, kg, lq, l16, xa, acc);
+                batched_load_rows<SPW, NV, true>(d.A, voff, min(c0 + kg + 32, mfull - 16), m, xa);
+                batched_mfma_group<LM, SPW, NV, NTL, OLD>(sO, kg + 16, lq, l16, xb, acc);
+              }
+              if (kg < gend)  // an odd last whole group (last chunk only), already in xa
+                batched_mfma_group<LM, SPW, NV, NTL, OLD>(sO, kg, lq, l16, xa, acc);
+              if (c0 + mc == m && mfull < m) {  // the last, partial group
+                batched_load_rows<SPW, NV, false>(d.A, voff, mfull, m, xb);
+                batched_mfma_group<LM, SPW, NV, NTL, OLD>(sO, mfull - c0, lq, l16, xb, acc);
+              }
+            }
+          }
+        };
+        if (nvs >= SPW) run(std::integral_constant<int, SPW>{});
+        else if (nvs == 1) run(std::integral_constant<int, 1>{});
+        else run(std::integral_constant<int, 0>{});
+        // output element (row 16t + lq + 4r, column l16) of each 16 x 16 tile; rows ell .. LM-1
+        // are zero (so are the staged Omega rows), rows past LM are not stored
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) {
+          const int col = (q0 + i) * 16 + l16;
+          if (i < nvs && col < nloc) {
+            gdouble* dst = (gdouble*)Bs + (i64)col * LM + lq;
 #pragma unroll
             for (int t = 0; t < NTL; ++t)
 #pragma unroll
-              for (int r = 0; r < 4; ++r) {
-                const int row = 16 * t + lq + 4 * r;
-                if (row < ell) Bs[row + (i64)col * LM] = acc[i][t][r];
-              }
+              for (int r = 0; r < 4; ++r)
+                if (16 * t + 4 * r < LM) dst[16 * t + 4 * r] = acc[i][t][r];
           }
         }
       }
@@ -379,11 +450,17 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
     int pos = c0col + lc;
     bool active = have;
     int myflag = 0;
-    int j0 = 0, jb = min(nbe, kcap);
+    int j0 = 0, jb = min(nbe, kcap);  // dlaqps block bookkeeping (wave 0's copy is the one used)
     int k_res = kcap;
     int reject = 0;
     for (int s = 0;; ++s) {
       const int par = s & 1;
+      // lane-derived values recomputed each step from an opaque copy of the thread index: kept
+      // live across the loop, the addresses built from them were spilled to scratch memory and
+      // reloaded (one memory round trip each) on the step's critical path
+      int tid_o = tid;
+      asm volatile("" : "+v"(tid_o));
+      const int lane = tid_o & 63, lic = tid_o % TPC, rbase = lic * RPT;
       // A: this wave's best column publishes its values and its (norm, position)
       {
         double wv;
@@ -391,36 +468,40 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
         const int wf = __any(myflag) ? 1 : 0;
         if (wl >= 0 && (lane & ~(TPC - 1)) == wl) {
 #pragma unroll
-          for (int i = 0; i < RPT; ++i) sCand[par][wave][rbase + i] = a[i];
+          for (int i = 0; i < RPT; ++i) sCand[wave][rbase + i] = a[i];
         }
         const int wpos = __builtin_amdgcn_readlane(pos, wl < 0 ? 0 : wl);
         if (lane == 0) {
-          sVal[par][wave] = wv;
-          sPos[par][wave] = wl < 0 ? INT_MAX : wpos;
-          sLane[par][wave] = wl;
-          sFlag[par][wave] = wf;
+          sVal[wave] = wv;
+          sPos[wave] = wl < 0 ? INT_MAX : wpos;
+          sLane[wave] = wl;
+          sFlag[wave] = wf;
         }
       }
+      myflag = 0;
       __syncthreads();
       if (CL == 2 && sAbort) return;
-      double cv = sVal[par][0];
-      int cp = sPos[par][0], cw = 0, cflag = sFlag[par][0];
-#pragma unroll
-      for (int w = 1; w < BW; ++w) {
-        const double ov = sVal[par][w];
-        const int op = sPos[par][w];
-        if (ov > cv || (ov == cv && op < cp)) { cv = ov; cp = op; cw = w; }
-        cflag |= sFlag[par][w];
+
+      // B: the pivot (largest norm, then lowest position; lane w holds wave w's candidate),
+      // computed by every wave; pairs swap it with the other half (wave 0 exchanges, the
+      // others read the outcome from LDS): the larger norm wins, then the lower position
+      int cw, cp, cwl, cflag;
+      double cv;
+      {
+        const bool lw = lane < BW;
+        const double ov = lw ? sVal[lane] : -1.0;
+        const int op = lw ? sPos[lane] : INT_MAX;
+        const int wl = wave_argmax(ov, op, 1, &cv);
+        cw = wl < 0 ? 0 : wl;
+        cp = wl < 0 ? INT_MAX : __builtin_amdgcn_readlane(op, wl);
+        cwl = __builtin_amdgcn_readlane(lw ? sLane[lane] : -1, cw);
+        cflag = __any(lw && sFlag[lane] != 0) ? 1 : 0;
       }
-      myflag = 0;
       if constexpr (CL == 2) {
-        // swap candidates with the other half (wave 0 exchanges, the others read the
-        // partner's candidate from LDS): the larger norm wins, then the lower position.
-        // sPc is rewritten only after the next step's first barrier, which every wave
-        // reaches after its reflector build has read it.
+        // (sPc, the partner's column, is written and read by wave 0 only)
         if (wave == 0) {
           xpublish<LM>(myslot + par * (LM + 2), xtag(gen, qi, s + 1), cv, cp, cflag,
-                       sLane[par][cw] >= 0 ? sCand[par][cw] : nullptr, p.uncached);
+                       cwl >= 0 ? sCand[cw] : nullptr, p.uncached);
           double pv;
           int pp, pf;
           const bool ok = xpoll<LM>(peerslot + par * (LM + 2), xtag(gen, qi, s + 1),
@@ -433,12 +514,12 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
         __syncthreads();
         if (sAbort) return;
         const double pv = sXv;
-        const int pp = sXp, pf = sXf;
-        cflag |= pf;
-        if (pv > cv || (pv == cv && pp < cp)) { cv = pv; cp = pp; cw = -1; }
+        const int pp = __builtin_amdgcn_readfirstlane(sXp);
+        cflag |= __builtin_amdgcn_readfirstlane(sXf);
+        if (pv > cv || (pv == cv && pp < cp)) { cv = pv; cp = pp; cw = -1; cwl = -1; }
       }
-      const bool done_steps = (s == kcap);
       // D: dlaqps block bookkeeping for the step just finished (s-1) and the stop rule
+      const bool done_steps = (s == kcap);
       if (s > 0) {
         const int last = s - 1;
         const bool blk_end = (last - j0 + 1 == jb) || (cflag != 0) || done_steps;
@@ -457,69 +538,72 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
       }
       if (done_steps) { k_res = s; break; }
 
-      // E: every wave builds the reflector (dlarfg) from the winner's column; lane l holds
-      // rows l and l + 64
-      const int rk = s;
-      double tau = 0.0;
-      // waves without an active column skip it (wave 0 always builds it: it writes R11)
-      if (wave == 0 || __any(active)) {
-        const double* xc = cw >= 0 ? sCand[par][cw] : sPc;
-        double xr[NR];
-        double ss = 0.0;
+      // E (wave 0): the reflector (dlarfg) of the pivot column into sX by absolute row (zero
+      // above row rk, 1 at rk) and R11's column rk; the other waves wait at the barrier
+      {
+        if (wave == 0) {
+          const int rk = s;
+          const double* xc = cw >= 0 ? sCand[cw] : sPc;
+          double xr[NR];
+          double ss = 0.0;
 #pragma unroll
-        for (int h = 0; h < NR; ++h) {
-          const int r = lane + 64 * h;
-          xr[h] = r < LM ? xc[r] : 0.0;
-          ss += (r > rk && r < ell) ? xr[h] * xr[h] : 0.0;
-        }
-        ss = group_sum(ss, 64);
-        const double alpha0 = xc[rk];
-        double alpha = alpha0, xnorm = sqrt(ss), beta = alpha0, scal = 1.0, fac = 1.0;
-        if (ell - rk > 1 && xnorm != 0.0) {
-          beta = -copysign(lapy2(alpha, xnorm), alpha);
-          const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
-          int knt = 0;
-          if (fabs(beta) < safmin) {
-            const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
-            do {
-              ++knt;
-              fac *= rsafmn;
-              beta *= rsafmn;
-              alpha *= rsafmn;
-            } while (fabs(beta) < safmin && knt < 20);
-            double s2 = 0.0;
-#pragma unroll
-            for (int h = 0; h < NR; ++h) {
-              const int r = lane + 64 * h;
-              const double xs = xr[h] * fac;
-              s2 += (r > rk && r < ell) ? xs * xs : 0.0;
-            }
-            xnorm = sqrt(group_sum(s2, 64));
+          for (int h = 0; h < NR; ++h) {
+            const int r = lane + 64 * h;
+            xr[h] = r < LM ? xc[r] : 0.0;
+            ss += (r > rk && r < ell) ? xr[h] * xr[h] : 0.0;
+          }
+          ss = group_sum(ss, 64);
+          const double alpha0 = xc[rk];
+          double alpha = alpha0, xnorm = sqrt(ss), beta = alpha0, scal = 1.0, fac = 1.0;
+          double tau = 0.0;
+          if (ell - rk > 1 && xnorm != 0.0) {
             beta = -copysign(lapy2(alpha, xnorm), alpha);
-          }
-          tau = (beta - alpha) / beta;
-          scal = 1.0 / (alpha - beta);
-          for (int j = 0; j < knt; ++j) beta *= safmin;
-        }
+            const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+            int knt = 0;
+            if (fabs(beta) < safmin) {
+              const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+              do {
+                ++knt;
+                fac *= rsafmn;
+                beta *= rsafmn;
+                alpha *= rsafmn;
+              } while (fabs(beta) < safmin && knt < 20);
+              double s2 = 0.0;
 #pragma unroll
-        for (int h = 0; h < NR; ++h) {
-          const int r = lane + 64 * h;
-          const double vr = (r > rk && r < ell) ? (xr[h] * fac) * scal : (r == rk ? 1.0 : 0.0);
-          if (r < LM) sV[wave][r] = vr;
-          if (wave == 0 && rk < KM) {
-            if (r < rk) sR[r + rk * KM] = xr[h];
-            if (r == rk) sR[rk + rk * KM] = beta;
+              for (int h = 0; h < NR; ++h) {
+                const int r = lane + 64 * h;
+                const double xs = xr[h] * fac;
+                s2 += (r > rk && r < ell) ? xs * xs : 0.0;
+              }
+              xnorm = sqrt(group_sum(s2, 64));
+              beta = -copysign(lapy2(alpha, xnorm), alpha);
+            }
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+            for (int j = 0; j < knt; ++j) beta *= safmin;
+          }
+#pragma unroll
+          for (int h = 0; h < NR; ++h) {
+            const int r = lane + 64 * h;
+            const double vr =
+                (r > rk && r < ell) ? (xr[h] * fac) * scal : (r == rk ? 1.0 : 0.0);
+            if (r < LM) sX[r] = vr;
+            if (rk < KM) {
+              if (r < rk) sR[r + rk * KM] = xr[h];
+              if (r == rk) sR[rk + rk * KM] = beta;
+            }
+          }
+          if (lane == 0) {
+            sDiag[rk] = beta;
+            sTau = tau;
           }
         }
-        if (wave == 0 && lane == 0) sDiag[rk] = beta;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
+      __syncthreads();
 
       // F: position swap (LAPACK swaps columns rk and pvt), apply H, downdate the norms
-      const bool is_win =
-          active && cw >= 0 && wave == cw && (lane & ~(TPC - 1)) == sLane[par][cw];
+      const int rk = s;
+      const bool is_win = active && cw >= 0 && wave == cw && (lane & ~(TPC - 1)) == cwl;
       if (is_win) {
         active = false;
         pos = s;
@@ -529,14 +613,14 @@ __global__ __launch_bounds__(BT, 4) void id_batched_kernel(BStage p) {
       // (the TPC lanes of a column share `active`, so the group sums stay within active lanes)
       int flag = 0;
       if (active) {
-        const double* v = sV[wave] + rbase;
+        const double* v = sX + rbase;
         double w0 = 0.0, w1 = 0.0;
 #pragma unroll
         for (int i = 0; i < RPT; i += 2) {
           w0 += v[i] * a[i];
           w1 += v[i + 1] * a[i + 1];
         }
-        const double tw = tau * group_sum(w0 + w1, TPC);
+        const double tw = sTau * group_sum(w0 + w1, TPC);
         // re-read v from LDS below rather than keeping its values live beside the column
         asm volatile("" ::: "memory");
         double xs = 0.0;
