@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (`bench.py
 # --profile-kernels`), calibrated on a 1 GiB copy (fetch x2.000, write x1.000); the commands
 # and raw counters are in profiles/hbm_traffic_counters_qrcp_and_batched.md.
-TRAFFIC_QRCP_FINAL = 0.698e9      # qrcp_kernel<36, 512, true>, 520 x 8192 sketch, k = 436
-TRAFFIC_BATCHED = 12.48e9         # the three id_batched_kernel launches of one batch
+TRAFFIC_QRCP_FINAL = 0.702e9      # qrcp_kernel<36, 512, true>, 520 x 8192 sketch, k = 436
+TRAFFIC_BATCHED = 13.10e9         # the five id_batched_kernel launches of one batch
 TRAFFIC_SOURCE = "profiles/hbm_traffic_counters_qrcp_and_batched.md"
 
 
