@@ -513,12 +513,14 @@ static const Cfg kCfgs[] = {{0, 16, 256}, {1, 32, 256}, {2, 48, 256}, {3, 64, 25
                             {5, 16, 256}, {6, 32, 256}, {7, 48, 256}, {8, 64, 256}, {9, 128, 128},
                             {10, 64, 128}, {11, 128, 256}, {12, 128, 128}, {13, 64, 256},
                             {14, 256, 128}, {15, 80, 256}, {16, 96, 256}, {17, 160, 256},
-                            {18, 176, 128}};
-constexpr int kNumCfg = 19;
-// ids 5..18: the k-contiguous kernel (gemm_kc_f64_kernel, FP64 or F32 MFMA; 11, 13, 14, 16-18
+                            {18, 176, 128}, {19, 208, 128}, {20, 112, 128}, {21, 112, 256}};
+constexpr int kNumCfg = 22;
+// ids 5..21: the k-contiguous kernel (gemm_kc_f64_kernel, FP64 or F32 MFMA; 11, 13, 14, 16-21
 // with 512 threads, 12 with 32-wide k-tiles).  15-17 hold a short sketch (ell = 72, 136) in one
 // M-tile, so each k-tile of A is read from L2 once; 18 (11 MFMA rows per wave) covers
-// ell = 520 in three M-tiles with 1.5% padding.
+// ell = 520 in three M-tiles with 1.5% padding; 19 (13 MFMA rows per wave) covers two paired
+// rounds' 400 rows (orders 136 + 264) in two M-tiles, 20 and 21 the 112 rows of orders 40 + 72
+// in one.
 bool kc_cfg(int id) { return id >= 5; }
 bool kc_ok(const GemmArgs& g) {
   return (g.prec == LRA_PREC_F64 || g.prec == LRA_PREC_F32) && g.transA == 1 && g.transB == 0 &&
@@ -605,6 +607,21 @@ hipError_t launch_cfg(int id, const GemmDev& d, int blocks, hipStream_t st) {
       else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<176, 128, 1, 8, 16, true>), dim3(blocks), dim3(512), 0, st, d);
       else return hipErrorInvalidValue;
       break;
+    case 19:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<208, 128, 1, 8, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<208, 128, 1, 8, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 20:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<112, 128, 1, 8, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<112, 128, 1, 8, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
+    case 21:
+      if constexpr (PREC == LRA_PREC_F64) hipLaunchKernelGGL((gemm_kc_f64_kernel<112, 256, 1, 8, 16, false>), dim3(blocks), dim3(512), 0, st, d);
+      else if constexpr (PREC == LRA_PREC_F32) hipLaunchKernelGGL((gemm_kc_f64_kernel<112, 256, 1, 8, 16, true>), dim3(blocks), dim3(512), 0, st, d);
+      else return hipErrorInvalidValue;
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -648,6 +665,10 @@ int bucket(i64 x) {  // power-of-two bucket
   while ((1ll << b) < x) ++b;
   return b;
 }
+// M's class: its 16-row MFMA tile count up to 1024 rows (a sketch order; the best M tile
+// depends on how the order divides into tiles, e.g. 400 rows fill two 208-row tiles but
+// 264 rows fit a 160 x 256 tile better), the power-of-two bucket beyond
+int bucket_m(i64 m) { return m <= 1024 ? (int)((m + 15) / 16) : 64 + bucket(m); }
 
 int prepare(lra_ctx* ctx, const GemmArgs& g, int cfg, int splits, GemmDev* d, int* blocks) {
   const Cfg& c = kCfgs[cfg];
@@ -820,7 +841,7 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
   }
   if (tn_skinny_ok(g)) return run_tn_skinny(ctx, g);
   if (ct_ok(g)) return run_ct(ctx, g);
-  TuneKey key{g.prec, g.transA, g.transB, bucket(g.M), bucket(g.N), bucket(g.K)};
+  TuneKey key{g.prec, g.transA, g.transB, bucket_m(g.M), bucket(g.N), bucket(g.K)};
   TuneVal choice;
   bool have = false;
   {
@@ -901,7 +922,7 @@ int lra_gemm(lra_ctx* ctx, const GemmArgs& g) {
 // Report the configuration chosen for a shape (for the bench's roofline bookkeeping).
 extern "C" int lra_gemm_choice(int prec, int transA, int transB, int64_t M, int64_t N,
                                int64_t K, int* cfg, int* splits, int* bm, int* bn) {
-  TuneKey key{prec, transA, transB, bucket(M), bucket(N), bucket(K)};
+  TuneKey key{prec, transA, transB, bucket_m(M), bucket(N), bucket(K)};
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(key);
   if (it == g_tune.end()) return LRA_E_STATE;

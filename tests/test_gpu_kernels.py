@@ -42,10 +42,11 @@ def test_sketch_gemm_f64_variants(gpu, side, trans):
 
 
 @pytest.mark.parametrize("cfg", ["5:1", "6:1", "7:2", "8:1", "9:1", "10:4", "11:1", "12:2", "13:3",
-                                 "14:1", "15:2", "16:1", "17:4", "18:3", "18:6", "0:1", "4:2"])
+                                 "14:1", "15:2", "16:1", "17:4", "18:3", "18:6", "19:1", "19:4",
+                                 "20:2", "21:1", "21:3", "0:1", "4:2"])
 @pytest.mark.parametrize("M,N,K", [(40, 1000, 1031), (136, 513, 800), (7, 300, 2)])
 def test_sketch_gemm_forced_configs(gpu, monkeypatch, cfg, M, N, K):
-    """Every GEMM configuration (the k-contiguous FP64 kernel 5..18 and the general one 0..4),
+    """Every GEMM configuration (the k-contiguous FP64 kernel 5..21 and the general one 0..4),
     forced through LRA_GEMM_CFG, on a generated-Omega sketch with ragged M, N, K (odd K
     exercises the single-element tail of the 16-byte loads)."""
     rng = np.random.default_rng(M * N + K)
@@ -234,7 +235,8 @@ def test_dgemm_short_k_tall_c(gpu, ta, tb, K, beta):
 
 
 @pytest.mark.parametrize("cfg", ["5:1", "6:2", "7:1", "8:1", "9:1", "10:4", "11:1", "12:2", "13:3",
-                                 "14:1", "15:2", "16:1", "17:4", "18:4", "18:6"])
+                                 "14:1", "15:2", "16:1", "17:4", "18:4", "18:6", "19:2", "20:1",
+                                 "21:2"])
 @pytest.mark.parametrize("M,N,K", [(40, 1000, 1031), (136, 513, 800)])
 def test_sketch_gemm_f32_kc_configs(gpu, monkeypatch, cfg, M, N, K):
     """The k-contiguous kernel in its F32 form (FP64 loads rounded to FP32 into LDS,
