@@ -179,6 +179,7 @@ def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         plan = lra.BatchedID(blocks, rtol=1e-8)
+        widths = plan.widths
         plan.launch()
         plan.finish()
         ks = plan.k.copy()
@@ -212,6 +213,8 @@ def bench_batched(lra, torch, reps: int = 5, nblk: int = 4096):
     return {"workload": f"{nblk} kernel blocks K=1/|x-y|, m,n=round(64*16^U) (sum m*n = "
                         f"{int(mnp.sum())}), rtol=1e-8, FP64",
             "value": round(ids_s, 1), "unit": "IDs/s", "ms_per_batch": round(ms, 3),
+            "kernel_widths": {"per_round_512_thread_max_n": widths,
+                              "note": "picked by the plan's autotuner on these blocks"},
             "ranks": {"min": int(ks.min()), "max": int(ks.max()),
                       "second_round": int((ks >= 32).sum())},
             "blocks_left_to_idfact": int(nfb),
@@ -381,6 +384,7 @@ def profile_kernels(lra, torch, n: int):
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         plan = lra.BatchedID(blocks, rtol=1e-8)
+        widths = plan.widths
         plan.launch()
         plan.finish()
         plan.close()

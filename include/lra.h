@@ -464,6 +464,13 @@ int lra_idfact_batched_capture(lra_batch_plan* plan);
 int lra_idfact_batched_launch(lra_batch_plan* plan);
 int lra_idfact_batched_finish(lra_batch_plan* plan, int64_t* k_out, int64_t* n_fallback);
 int lra_idfact_batched_plan_destroy(lra_batch_plan* plan);
+/* The plan's kernel split (MI355X extension, no counterpart in the reference): widths[r] is
+ * the widest block of round r that runs in a 512-thread workgroup (two per CU) rather than a
+ * 1024-thread one or a workgroup pair, 0 for none, -1 past the plan's rounds; count entries.
+ * _plan picks them by timing the candidates on the blocks when LRA_AUTOTUNE is not 0 and the
+ * batch has at least 256 blocks (LRA_BATCHED_W0=w0,w1 forces them); results do not depend on
+ * them. */
+int lra_idfact_batched_plan_widths(const lra_batch_plan* plan, int* widths, int count);
 
 /*
  * Serving (MI355X extension; the reference has no server, each call factors one matrix).

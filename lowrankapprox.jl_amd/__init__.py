@@ -802,6 +802,15 @@ class BatchedID:
             C.cast(self._rd, C.c_void_p), C.cast(self._T, C.c_void_p), self.ldt.ctypes.data,
             C.byref(self.h)), "lra_idfact_batched_plan")
 
+    @property
+    def widths(self) -> list:
+        """Per adaptive round, the widest block run by a 512-thread workgroup (0: none); the
+        plan's autotuner picks them on the blocks (they change time, not results)."""
+        w = (C.c_int * 4)()
+        check(_lib.load().lra_idfact_batched_plan_widths(self.h, w, 4),
+              "lra_idfact_batched_plan_widths")
+        return [int(x) for x in w if x >= 0]
+
     def capture(self) -> None:
         """Capture the launch in a hipGraph (needs a non-default current stream)."""
         check(_lib.load().lra_idfact_batched_capture(self.h), "lra_idfact_batched_capture")

@@ -182,6 +182,7 @@ SIGNATURES = [
     ("lra_idfact_batched_launch", C.c_int, [C.c_void_p]),
     ("lra_idfact_batched_finish", C.c_int, [C.c_void_p, P_I, C.POINTER(i64)]),
     ("lra_idfact_batched_plan_destroy", C.c_int, [C.c_void_p]),
+    ("lra_idfact_batched_plan_widths", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
     ("lra_server_create", C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
     ("lra_server_submit", C.c_int, [C.c_void_p, C.c_int, i64, i64, P_D, i64,
                                     C.POINTER(lra_opts), C.POINTER(i64)]),

@@ -30,6 +30,8 @@
 #include "lra_wave.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <climits>
 #include <numeric>
 #include <type_traits>
@@ -749,6 +751,8 @@ struct lra_batch_plan {
   std::vector<double*> T;
   std::vector<int> direct;  // blocks that go to lra_idfact without the batched kernels
   int nq[NKIND] = {};       // stage-0 blocks of each kernel
+  int w0[NSTAGE] = {};      // per stage: widest block of the 512-thread kernel (0: none)
+  std::vector<int> queued;  // blocks the kernels take (stage 0), in block order
   int grid = 0, npairs = 0, nstage = 0;  // grid: workgroups the scratch slabs cover
   int ell[NSTAGE] = {}, target[NSTAGE] = {}, variant[NSTAGE] = {};
   BDesc* d_desc = nullptr;
@@ -777,6 +781,38 @@ static void plan_free(lra_batch_plan* P) {
   delete P;
 }
 
+// widest block of stage r's kernel `kind` (0: the kernel is not launched)
+static int maxn_of(const lra_batch_plan* P, int r, int kind) {
+  return kind == 0 ? P->w0[r] : kVariants[P->variant[r]].maxn[kind];
+}
+
+// Stage 0's work lists: each queued block goes to the first kernel whose width limit holds
+// it, each list by decreasing m*n (the large blocks start first); uploaded to the device and
+// finished (the host vectors are temporaries).
+static int upload_stage0(lra_batch_plan* P) {
+  std::vector<int> q[NKIND];
+  for (int b : P->queued) {
+    int kind = 0;
+    while (kind < NKIND && (maxn_of(P, 0, kind) == 0 || P->n[b] > maxn_of(P, 0, kind))) ++kind;
+    if (kind < NKIND) q[kind].push_back(b);  // (queued blocks fit stage 0's widest kernel)
+  }
+  hipError_t e = hipSuccess;
+  for (int kind = 0; kind < NKIND; ++kind) {
+    std::stable_sort(q[kind].begin(), q[kind].end(),
+                     [&](int x, int y) { return P->m[x] * P->n[x] > P->m[y] * P->n[y]; });
+    P->nq[kind] = (int)q[kind].size();
+    if (P->nq[kind] > 0 && e == hipSuccess)
+      e = hipMemcpyAsync(P->d_lists + (size_t)kind * P->nblk, q[kind].data(),
+                         sizeof(int) * P->nq[kind], hipMemcpyHostToDevice, P->c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(P->c->stream);
+  if (e != hipSuccess) {
+    lra_set_error("batched plan: %s", hipGetErrorString(e));
+    return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
 static int enqueue(lra_batch_plan* P) {
   lra_ctx* c = P->c;
   if (P->nq[0] + P->nq[1] + P->nq[2] == 0) return LRA_OK;
@@ -788,9 +824,8 @@ static int enqueue(lra_batch_plan* P) {
     return P->d_lists + (size_t)(NKIND * r + kind) * P->nblk;
   };
   for (int r = 0; r < P->nstage; ++r) {
-    const Variant& V = kVariants[P->variant[r]];
     for (int kind = 0; kind < NKIND; ++kind) {
-      if (V.maxn[kind] == 0) continue;
+      if (maxn_of(P, r, kind) == 0) continue;
       if (r == 0 && P->nq[kind] == 0) continue;
       BStage s;
       s.desc = P->d_desc;
@@ -803,12 +838,11 @@ static int enqueue(lra_batch_plan* P) {
       s.scratch = P->d_scratch;
       s.k_out = P->d_k;
       const bool more = r + 1 < P->nstage;
-      const Variant* N = more ? &kVariants[P->variant[r + 1]] : nullptr;
       for (int kk = 0; kk < NKIND; ++kk) {
-        const bool has = more && N->maxn[kk] > 0;
+        const bool has = more && maxn_of(P, r + 1, kk) > 0;
         s.next_list[kk] = has ? list(r + 1, kk) : nullptr;
         s.next_n[kk] = has ? P->d_counts + NC + NKIND * (r + 1) + kk : nullptr;
-        s.next_maxn[kk] = has ? N->maxn[kk] : 0;
+        s.next_maxn[kk] = has ? maxn_of(P, r + 1, kk) : 0;
       }
       s.ell = P->ell[r];
       s.rank = (int)std::min<i64>(P->o.rank, INT_MAX);
@@ -838,6 +872,68 @@ static int enqueue(lra_batch_plan* P) {
     }
   }
   return LRA_OK;
+}
+
+// The batch's autotuner: which blocks of each round run in the 512-thread kernel (two
+// workgroups per CU) and which in the 1024-thread one is a width limit per round.  The plan
+// times the candidate limits on its own blocks -- the compiled limit, half of it, none, per
+// round -- with one warm launch and the best of two timed launches each, and keeps the
+// fastest.  Every block's result is the same under any limit (each block is factored by one
+// workgroup or pair exactly as idfact would), so the choice changes time only.
+// LRA_BATCHED_W0=w0,w1 forces the limits of rounds 0 and 1; LRA_AUTOTUNE=0, or fewer than 256
+// blocks, keeps the compiled ones.
+static int tune_widths(lra_batch_plan* P) {
+  const int lim0 = kVariants[P->variant[0]].maxn[0];
+  const int lim1 = P->nstage > 1 ? kVariants[P->variant[1]].maxn[0] : 0;
+  if (const char* f = getenv("LRA_BATCHED_W0")) {
+    int a = -1, b = -1;
+    const int got = sscanf(f, "%d,%d", &a, &b);
+    if (got >= 1) P->w0[0] = std::max(0, std::min(a, lim0));
+    if (got >= 2 && P->nstage > 1) P->w0[1] = std::max(0, std::min(b, lim1));
+    return upload_stage0(P);
+  }
+  const char* at = getenv("LRA_AUTOTUNE");
+  if ((at && at[0] == '0') || P->queued.size() < 256) return LRA_OK;
+  lra_ctx* c = P->c;
+  std::vector<std::pair<int, int>> cands;
+  for (int a : {lim0, lim0 / 2, 0})
+    for (int b : {lim1, lim1 / 2, 0}) {
+      cands.push_back({a, b});
+      if (P->nstage < 2) break;
+    }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LRA_HIP(hipEventCreate(&e0));
+  LRA_HIP(hipEventCreate(&e1));
+  int st = LRA_OK;
+  float best = 0.0f;
+  std::pair<int, int> pick = cands[0];
+  for (size_t i = 0; i < cands.size() && st == LRA_OK; ++i) {
+    P->w0[0] = cands[i].first;
+    if (P->nstage > 1) P->w0[1] = cands[i].second;
+    st = upload_stage0(P);
+    if (st == LRA_OK && i == 0) st = enqueue(P);  // warm: first launch of each kernel
+    float t = 0.0f;
+    for (int rep = 0; rep < 2 && st == LRA_OK; ++rep) {
+      float ms = 0.0f;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) st = LRA_E_HIP;
+      if (st == LRA_OK) st = enqueue(P);
+      if (st == LRA_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
+                           hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+        lra_set_error("batched autotune: event timing failed");
+        st = LRA_E_HIP;
+      }
+      if (rep == 0 || ms < t) t = ms;
+    }
+    if (st == LRA_OK && (i == 0 || t < best)) { best = t; pick = cands[i]; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st == LRA_OK && P->xchg) st = lra_check_device_error(c);  // a pair's wait expired
+  if (st != LRA_OK) return st;
+  P->w0[0] = pick.first;
+  if (P->nstage > 1) P->w0[1] = pick.second;
+  return upload_stage0(P);
 }
 
 int lra_chkopts_public(const lra_opts* o);
@@ -901,8 +997,9 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     if (!adap) break;
     tg *= 2;
   }
-  std::vector<int> q[NKIND];
+  for (int r = 0; r < P->nstage; ++r) P->w0[r] = kVariants[P->variant[r]].maxn[0];
   i64 mmax = 0;
+  int nwide = 0;  // stage-0 blocks of the workgroup-pair kernel
   const Variant& V0 = kVariants[P->variant[0]];
   for (i64 b = 0; b < nblk; ++b) {
     // stage 0 needs the block's width within its variant, and T rows for its largest rank
@@ -911,7 +1008,8 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     const bool ok = P->nstage > 0 && n[b] >= 1 && kind < NKIND && m[b] <= INT_MAX &&
                     ldt[b] >= std::max<i64>(1, std::min<i64>(P->ell[0], n[b]));
     if (ok) {
-      q[kind].push_back((int)b);
+      P->queued.push_back((int)b);
+      nwide += kind == 2;
       mmax = std::max(mmax, m[b]);
     } else {
       P->direct.push_back((int)b);
@@ -921,20 +1019,14 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
   // block's ldt allows are dropped (forwarding is decided on the device from n alone)
   for (int r = 1; r < P->nstage; ++r) {
     bool all = true;
-    for (int kind = 0; kind < NKIND; ++kind)
-      for (int b : q[kind]) all = all && ldt[b] >= std::min<i64>(P->ell[r], n[b]);
+    for (int b : P->queued) all = all && ldt[b] >= std::min<i64>(P->ell[r], n[b]);
     if (!all) { P->nstage = r; break; }
   }
-  bool pairs = V0.maxn[2] > 0 && !q[2].empty();
+  bool pairs = V0.maxn[2] > 0 && nwide > 0;
   for (int r = 1; r < P->nstage; ++r) pairs = pairs || kVariants[P->variant[r]].maxn[2] > 0;
-  for (int kind = 0; kind < NKIND; ++kind) {
-    std::stable_sort(q[kind].begin(), q[kind].end(),
-                     [&](int x, int y) { return m[x] * n[x] > m[y] * n[y]; });
-    P->nq[kind] = (int)q[kind].size();
-  }
+  const int nqueued = (int)P->queued.size();
   // scratch slabs: the most workgroups any launch has (two per CU for the 512-thread kernel)
-  P->grid = std::max(1, std::min(std::max({P->nq[0], P->nq[1], P->nq[2],
-                                           P->nstage > 1 ? (int)nblk : 0}),
+  P->grid = std::max(1, std::min(std::max(nqueued, P->nstage > 1 ? (int)nblk : 0),
                                  2 * c->num_cu));
   P->npairs = std::max(1, c->num_cu / 2);
   int st = LRA_OK;
@@ -943,7 +1035,7 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     st = lra_coop_area(c, sizeof(u64x2) * 4 * 74 * 2 * (size_t)P->npairs, &xp, &P->uncached);
     P->xchg = (u64x2*)xp;
   }
-  if (st == LRA_OK && P->nq[0] + P->nq[1] + P->nq[2] > 0) {
+  if (st == LRA_OK && nqueued > 0) {
     std::vector<BDesc> hd(nblk);
     for (i64 b = 0; b < nblk; ++b) {
       hd[b].A = A[b]; hd[b].lda = lda[b]; hd[b].sk = sk[b]; hd[b].rd = rd[b]; hd[b].T = T[b];
@@ -968,10 +1060,6 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     if (e == hipSuccess)
       e = hipMemcpyAsync(P->d_desc, hd.data(), sizeof(BDesc) * nblk, hipMemcpyHostToDevice,
                          c->stream);
-    for (int kind = 0; kind < NKIND && e == hipSuccess; ++kind)
-      if (P->nq[kind] > 0)
-        e = hipMemcpyAsync(P->d_lists + (size_t)kind * nblk, q[kind].data(),
-                           sizeof(int) * P->nq[kind], hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) {
       lra_set_error("batched plan: %s", hipGetErrorString(e));
       st = LRA_E_HIP;
@@ -981,11 +1069,13 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
     for (int r = 0; r < P->nstage && st == LRA_OK; ++r)
       st = lra_launch_gaussian_fill_t(c->stream, o.seed, (uint64_t)r, P->ell[r], mmax,
                                       P->d_omega[r], P->ldo);
-    // the host copies above read `hd` and `q`: finish them before those vectors go away
+    // the host copy above reads `hd`: finish it before that vector goes away
     if (st == LRA_OK && hipStreamSynchronize(c->stream) != hipSuccess) {
       lra_set_error("batched plan: stream synchronize failed");
       st = LRA_E_HIP;
     }
+    if (st == LRA_OK) st = upload_stage0(P);
+    if (st == LRA_OK) st = tune_widths(P);
   }
   if (st != LRA_OK) {
     plan_free(P);
@@ -996,6 +1086,13 @@ extern "C" int lra_idfact_batched_plan(lra_ctx* c, int64_t nblk, const int64_t* 
 }
 
 // Enqueue the batched kernels on the context's stream (a graph launch once captured).
+extern "C" int lra_idfact_batched_plan_widths(const lra_batch_plan* P, int* widths, int count) {
+  LRA_ARG(P != nullptr, 1);
+  LRA_ARG(widths != nullptr || count <= 0, 2);
+  for (int r = 0; r < count; ++r) widths[r] = r < P->nstage ? P->w0[r] : -1;
+  return LRA_OK;
+}
+
 extern "C" int lra_idfact_batched_launch(lra_batch_plan* P) {
   LRA_ARG(P != nullptr, 1);
   LRA_HIP(hipSetDevice(P->c->device));

@@ -214,3 +214,23 @@ def test_batched_graph_replay(gpu):
                 assert np.array_equal(host(V.sk), sk)
                 assert np.array_equal(host(V.T), T), rep
         plan.close()
+
+
+@pytest.mark.parametrize("w0", ["0,0", "256,128", "512,0", "0,256"])
+def test_batched_width_limits_same_bits(gpu, monkeypatch, w0):
+    """The autotuner's choice -- which blocks of each round run in the 512-thread kernel and
+    which in the 1024-thread one -- changes time only: every forced split of the widths gives
+    the bits of the compiled split (ranks, skeletons and T), blocks of every width class and
+    both rounds included."""
+    rng = np.random.default_rng(21)
+    shp = [tuple(s) for s in sizes(36, rng)] + [(800, 1000), (300, 700), (500, 200), (90, 500)]
+    blocks = [kernel_block(m, n, rng) for m, n in shp]
+    ref = [(V.k, host(V.sk), host(V.T)) for V in gpu.idfact_batched(blocks, rtol=1e-10, seed=4)]
+    assert any(k >= 32 for k, _, _ in ref)  # some blocks take the second round
+    monkeypatch.setenv("LRA_BATCHED_W0", w0)
+    print(f"batched width limits {w0}: {len(blocks)} blocks", flush=True)
+    out = gpu.idfact_batched(blocks, rtol=1e-10, seed=4)
+    for (k, sk, T), V in zip(ref, out):
+        assert V.k == k
+        assert np.array_equal(host(V.sk), sk)
+        assert np.array_equal(host(V.T), T)
