@@ -484,7 +484,7 @@ extern "C" int lra_id_T_f64(lra_ctx* c, int64_t k, int64_t n, const double* R, i
   LRA_ARG(n >= k, 3);
   LRA_ARG(ldr >= std::max<int64_t>(1, k), 5);
   LRA_ARG(ldt >= std::max<int64_t>(1, k), 7);
-  return lra_launch_id_T(c->stream, k, n, R, ldr, T, ldt);
+  return lra_id_T(c, k, n, R, ldr, T, ldt);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -957,7 +957,7 @@ static int finish_id(lra_ctx* c, const SketchFactOut& f, i64 N, const lra_opts* 
   }
   if (k > 0 && N > k) {
     LRA_ARG(out->ldt >= k, 8);
-    LRA_CHECK(lra_launch_id_T(c->stream, k, N, f.B, f.ell, out->T, out->ldt));
+    LRA_CHECK(lra_id_T(c, k, N, f.B, f.ell, out->T, out->ldt));
     if (f.prec != LRA_PREC_F64 && f.Bsave) LRA_CHECK(refine(out->sk, out->rd, out->T, out->ldt));
     if (o.maxdet_tol >= 0) {
       // strong-RRQR swaps (pqrback_postproc with maxdet, src/pqr.jl:426-433)

@@ -319,7 +319,7 @@ static int pqrfact_src(lra_ctx* c, int trans, const LraSrc& src, const lra_opts*
     }
     if ((ret & LRA_RET_T) && k > 0 && N > k) {
       if (!out->T) { double* t = nullptr; LRA_CHECK(lra_wsT(c, WS_F8, (size_t)k * (N - k), &t)); out->T = t; out->ldt = k; }
-      LRA_CHECK(lra_launch_id_T(c->stream, k, N, B, std::max<i64>(P, 1), out->T, out->ldt));
+      LRA_CHECK(lra_id_T(c, k, N, B, std::max<i64>(P, 1), out->T, out->ldt));
     }
     return lra_check_device_error(c);
   }

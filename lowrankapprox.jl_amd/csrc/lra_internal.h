@@ -328,6 +328,8 @@ int lra_qrcp_global(lra_ctx* ctx, const QrcpCall& q);
 
 // trsm.hip: T = R11 \ R12; in-place op(U) X = B with U the upper k x k triangle of R
 int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt);
+// the same for any k: row blocks of 1184 (GEMM updates between MFMA block solves) beyond one slab
+int lra_id_T(lra_ctx* c, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt);
 int lra_launch_trsm_upper(hipStream_t st, i64 k, i64 nrhs, const double* R, i64 ldr, double* B,
                           i64 ldb, int trans);
 int lra_launch_axpy2d(hipStream_t st, i64 m, i64 n, double alpha, const double* X, i64 ldx,

@@ -47,7 +47,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 template <int NC>
 __global__ __launch_bounds__(256) void id_T_mfma_kernel(i64 k, i64 nr, const double* __restrict__ R,
-                                                        i64 ldr, double* __restrict__ T, i64 ldt) {
+                                                        i64 ldr, const double* Bsrc, i64 ldb,
+                                                        double* T, i64 ldt) {
   extern __shared__ double slab[];  // (kp+1) x NC, column-major
   __shared__ double s_rbb[16][17];
   const int kp = (int)((k + 15) / 16) * 16;
@@ -55,10 +56,12 @@ __global__ __launch_bounds__(256) void id_T_mfma_kernel(i64 k, i64 nr, const dou
   const i64 c0 = (i64)blockIdx.x * NC;
   const int ncols = (int)std::min<i64>(NC, nr - c0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // load the slab (padding rows and missing columns are zero)
+  // load the slab (padding rows and missing columns are zero); the right-hand sides are
+  // Bsrc(0:k, c0:c0+NC), which may be T itself (each workgroup reads all of its columns before
+  // it writes any)
   for (int e = tid; e < kp * NC; e += 256) {
     const int r = e % kp, c = e / kp;
-    slab[r + c * ld] = (r < k && c < ncols) ? R[r + (k + c0 + c) * ldr] : 0.0;
+    slab[r + c * ld] = (r < k && c < ncols) ? Bsrc[r + (c0 + c) * ldb] : 0.0;
   }
   for (int r0 = kp - 16; r0 >= 0; r0 -= 16) {
     {  // stage the diagonal block R(r0:r0+16, r0:r0+16) (identity in the padding)
@@ -134,27 +137,40 @@ __global__ __launch_bounds__(256) void id_T_mfma_kernel(i64 k, i64 nr, const dou
 
 }  // namespace
 
+namespace {
+constexpr size_t kTMfmaLds = 150 * 1024;  // the MFMA solve's slab budget (dynamic LDS)
+
+// X (k x nr, ldt) = R(0:k, 0:k) \ Bsrc (k x nr, ldb) on MFMA, slabs of 32 (or 16) right-hand
+// sides in LDS; false when even 16 of them do not fit (k above 1199)
+bool launch_T_mfma(hipStream_t st, i64 k, i64 nr, const double* R, i64 ldr, const double* Bsrc,
+                   i64 ldb, double* T, i64 ldt, hipError_t* err) {
+  const i64 kp = (k + 15) / 16 * 16;
+  for (int nc : {32, 16}) {
+    const size_t bytes = (size_t)(kp + 1) * nc * sizeof(double);
+    if (bytes > kTMfmaLds) continue;
+    const i64 blocks = (nr + nc - 1) / nc;
+    static std::atomic<uint64_t> lds32{0}, lds16{0};  // dynamic LDS beyond 64 KiB
+    lra_lds_limit_once(lds32, (const void*)id_T_mfma_kernel<32>, (int)kTMfmaLds);
+    lra_lds_limit_once(lds16, (const void*)id_T_mfma_kernel<16>, (int)kTMfmaLds);
+    if (nc == 32)
+      hipLaunchKernelGGL(id_T_mfma_kernel<32>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
+                         nr, R, ldr, Bsrc, ldb, T, ldt);
+    else
+      hipLaunchKernelGGL(id_T_mfma_kernel<16>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
+                         nr, R, ldr, Bsrc, ldb, T, ldt);
+    *err = hipGetLastError();
+    return true;
+  }
+  return false;
+}
+}  // namespace
+
 int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt) {
   const i64 nr = n - k;
   if (k <= 0 || nr <= 0) return LRA_OK;
   {
-    // MFMA blocked solve when a slab of 32 (or 16) right-hand sides fits in LDS
-    const i64 kp = (k + 15) / 16 * 16;
-    const size_t cap = 150 * 1024;
-    for (int nc : {32, 16}) {
-      const size_t bytes = (size_t)(kp + 1) * nc * sizeof(double);
-      if (bytes > cap) continue;
-      const i64 blocks = (nr + nc - 1) / nc;
-      static std::atomic<uint64_t> lds32{0}, lds16{0};  // dynamic LDS beyond 64 KiB
-      lra_lds_limit_once(lds32, (const void*)id_T_mfma_kernel<32>, (int)cap);
-      lra_lds_limit_once(lds16, (const void*)id_T_mfma_kernel<16>, (int)cap);
-      if (nc == 32)
-        hipLaunchKernelGGL(id_T_mfma_kernel<32>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
-                           nr, R, ldr, T, ldt);
-      else
-        hipLaunchKernelGGL(id_T_mfma_kernel<16>, dim3((unsigned)blocks), dim3(256), bytes, st, k,
-                           nr, R, ldr, T, ldt);
-      hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
+    if (launch_T_mfma(st, k, nr, R, ldr, R + k * ldr, ldr, T, ldt, &e)) {
       if (e != hipSuccess) {
         lra_set_error("id_T launch: %s", hipGetErrorString(e));
         return LRA_E_HIP;
@@ -176,6 +192,43 @@ int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, doub
   if (e != hipSuccess) {
     lra_set_error("id_T launch: %s", hipGetErrorString(e));
     return LRA_E_HIP;
+  }
+  return LRA_OK;
+}
+
+// T = R11 \ R12 for a k beyond one MFMA slab (k > 1199): block back substitution over row
+// blocks of 1184 from the bottom.  Block j's right-hand side R12(j) - R(j, below) T(below) is
+// formed in T by a copy and one GEMM, then solved in place on MFMA.  The slab kernel alone
+// would hold 2 columns of k doubles per workgroup and run k barrier pairs per column.
+int lra_id_T(lra_ctx* c, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt) {
+  const i64 nr = n - k;
+  if (k <= 0 || nr <= 0) return LRA_OK;
+  constexpr i64 KB = 1184;  // (1184 + 1) x 16 doubles fit the slab budget
+  if (k <= KB || (k + KB - 1) / KB > 64) return lra_launch_id_T(c->stream, k, n, R, ldr, T, ldt);
+  for (i64 j = (k + KB - 1) / KB - 1; j >= 0; --j) {
+    const i64 r0 = j * KB, r1 = std::min(k, r0 + KB), kb = r1 - r0;
+    const double* Rjj = R + r0 + r0 * ldr;
+    hipError_t e = hipSuccess;
+    bool ok;
+    if (r1 == k) {
+      ok = launch_T_mfma(c->stream, kb, nr, Rjj, ldr, R + r0 + k * ldr, ldr, T + r0, ldt, &e);
+    } else {
+      LRA_CHECK(lra_launch_copy2d(c->stream, kb, nr, R + r0 + k * ldr, ldr, T + r0, ldt));
+      GemmArgs g{};
+      g.M = kb; g.N = nr; g.K = k - r1;
+      g.alpha = -1.0; g.beta = 1.0;
+      g.A = R + r0 + r1 * ldr; g.lda = ldr; g.transA = 0;
+      g.B = T + r1; g.ldb = ldt; g.transB = 0;
+      g.C = T + r0; g.ldc = ldt;
+      g.bgather = nullptr;
+      g.prec = LRA_PREC_F64;
+      LRA_CHECK(lra_gemm(c, g));
+      ok = launch_T_mfma(c->stream, kb, nr, Rjj, ldr, T + r0, ldt, T + r0, ldt, &e);
+    }
+    if (!ok || e != hipSuccess) {
+      lra_set_error("id_T block launch: %s", ok ? hipGetErrorString(e) : "slab too large");
+      return LRA_E_HIP;
+    }
   }
   return LRA_OK;
 }

@@ -181,6 +181,23 @@ def test_id_T_vs_scipy(gpu, k, n):
     assert np.abs(T - ref).max() <= 1e-14 * cond * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("k,n", [(1185, 1300), (2253, 3758), (2400, 2433), (3600, 3700)])
+def test_id_T_blocked_vs_scipy(gpu, k, n):
+    """k beyond one MFMA slab (1184): the blocked solve (a GEMM update of each row block's
+    right-hand side from the blocks below, then its MFMA solve in place) against scipy's
+    dtrsm, on a well-conditioned R (off-diagonal entries of size 1/sqrt(k))."""
+    rng = np.random.default_rng(k + n)
+    R = np.triu(rng.standard_normal((k, n))) / np.sqrt(k)
+    R[np.arange(k), np.arange(k)] = 1.0 + np.abs(R[np.arange(k), np.arange(k)])
+    R = np.asfortranarray(R)
+    ref = sla.solve_triangular(R[:, :k], R[:, k:], lower=False)
+    print(f"blocked id_T k={k} n={n}", flush=True)
+    T = host(gpu.id_T(gpu.to_device(R), k))
+    res = np.abs(R[:, :k] @ T - R[:, k:]).max()
+    assert res <= 1e-13 * max(1.0, np.abs(ref).max()) * np.sqrt(k)
+    assert np.abs(T - ref).max() <= 1e-11 * max(1.0, np.abs(ref).max())
+
+
 @pytest.mark.parametrize("M,N,K", [(32, 32, 8192), (32, 400, 8160), (17, 1000, 1031),
                                    (1, 5, 256), (30, 33, 4099)])
 @pytest.mark.parametrize("beta", [0.0, -0.5])
