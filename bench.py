@@ -257,11 +257,12 @@ def serving_pool(torch, count: int, seed: int = 6):
     return pool
 
 
-def bench_serving(lra, torch, rates=(200, 500, 1000, 2000, 5000), nreq: int = 200,
+def bench_serving(lra, torch, rates=(25, 50, 100, 200, 500, 1000, 2000, 5000), nreq: int = 200,
                   pool_size: int = 16, workers: int = 4, budget_gb: int = 96, seed: int = 6):
     """Serving: open-loop Poisson arrivals of idfact / psvdfact requests (50/50, rtol 1e-6 or
     1e-10 50/50) on resident matrices; per rate: requests/s completed and p50/p99 latency
-    from submission to result (server clock)."""
+    from submission to result (server clock).  The low rates sit below the service capacity
+    (latency is service time plus little queueing), the high ones above it (throughput)."""
     pool = serving_pool(torch, pool_size, seed)
     rng = np.random.default_rng(seed + 1)
     srv = lra.Server(hbm_budget=budget_gb << 30, workers=workers)
