@@ -234,3 +234,36 @@ def test_batched_width_limits_same_bits(gpu, monkeypatch, w0):
         assert V.k == k
         assert np.array_equal(host(V.sk), sk)
         assert np.array_equal(host(V.T), T)
+
+
+def test_batched_autotuned_plan_same_bits(gpu, monkeypatch):
+    """A plan of 256 blocks or more times candidate width limits on its own blocks before it
+    returns (launching the batch several times into the caller's buffers): its results equal
+    those of the compiled limits (LRA_AUTOTUNE=0), and each limit it keeps is a candidate --
+    the compiled one, half of it, or none."""
+    rng = np.random.default_rng(44)
+    shp = [tuple(s) for s in sizes(300, rng, lo=48, hi=700)]
+    blocks = [kernel_block(m, n, rng) for m, n in shp]
+    # the GEMM autotuner reads LRA_AUTOTUNE once per process: let a GEMM read it unset first
+    gpu.idfact(blocks[0], rtol=1e-9)
+    monkeypatch.setenv("LRA_AUTOTUNE", "0")
+    plan = gpu.BatchedID(blocks, rtol=1e-9, seed=2)
+    compiled = plan.widths
+    plan.launch()
+    ref = [(V.k, host(V.sk), host(V.T)) for V in plan.finish()]
+    plan.close()
+    monkeypatch.delenv("LRA_AUTOTUNE")
+    plan = gpu.BatchedID(blocks, rtol=1e-9, seed=2)
+    widths = plan.widths
+    print(f"autotuned batched plan: {len(blocks)} blocks, widths {compiled} -> {widths}",
+          flush=True)
+    assert len(widths) == len(compiled)
+    for w, c in zip(widths, compiled):
+        assert w in (c, c // 2, 0)
+    plan.launch()
+    out = plan.finish()
+    plan.close()
+    for (k, sk, T), V in zip(ref, out):
+        assert V.k == k
+        assert np.array_equal(host(V.sk), sk)
+        assert np.array_equal(host(V.T), T)
