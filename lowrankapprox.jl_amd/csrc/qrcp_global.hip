@@ -2,8 +2,11 @@
 // dlaqps steps of src/lapack.jl:117-139) for sketches too large for the register-resident
 // kernel of qrcp.hip (e.g. order 1032 or 2056 on thousands of columns).
 //
-// The sketch stays in HBM (column-major, in place, LAPACK layout) and each pivot step is two
-// launches on the context's stream, with every decision kept on the device:
+// The sketch stays in HBM (column-major, in place).  Up to 8192 rows (with the uncached
+// exchange area) each pivot step is ONE launch, `qg_fused` (below): every workgroup rebuilds
+// the step's reflector itself, and the workgroup that finishes last picks the next pivot.
+// Otherwise (or with LRA_QG_FUSED=0) each step is two launches on the context's stream, with
+// every decision kept on the device:
 //   qg_step   one workgroup: dlaqps block bookkeeping and the stop rule for the step just
 //             finished, the first position of the largest partial norm (idamax), LAPACK's
 //             column swap (data, jpvt, vn1, vn2), dlarfg on the pivot column (R's diagonal in
@@ -29,9 +32,10 @@ struct GState {
 
 constexpr int GT = 1024;  // threads of the single-workgroup kernels
 
-// column norms (one wave per column), jpvt = 1..n
+// column norms (one wave per column), jpvt = 1..n (and the identity position map)
 __global__ __launch_bounds__(256) void qg_norms(int m, int n, const double* __restrict__ B,
-                                                i64 ldb, double* vn1, double* vn2, i64* jpvt) {
+                                                i64 ldb, double* vn1, double* vn2, i64* jpvt,
+                                                int* at) {
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;
@@ -42,6 +46,7 @@ __global__ __launch_bounds__(256) void qg_norms(int m, int n, const double* __re
   if (lane == 0) {
     vn1[j] = vn2[j] = sqrt(ss);
     jpvt[j] = j + 1;
+    if (at) at[j] = j;
   }
 }
 
@@ -428,6 +433,399 @@ __global__ void qg_finish(const GState* st, i64* k_out, int* early_out) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Fused steps: one launch per pivot step (sketches up to 8192 rows, with the uncached
+// exchange area).  Columns stay where they are in memory (storage index c); the map `at` from
+// LAPACK position to storage column lives on the device, and the data is permuted to LAPACK
+// positions once at the end.  Launch s, every workgroup:
+//   1. loads its trailing columns (positions s+1+CPB*b ..) and the pivot column at[s], and
+//      rebuilds H(s) (dlarfg) in LDS -- the same reduction order in every workgroup, so all
+//      copies are bitwise the same and no launch of its own is needed for it;
+//   2. applies H(s) and downdates the norms (the arithmetic of qg_apply / qg_applyw);
+//   3. publishes its best candidate (largest norm, then lowest position) as a tagged
+//      16-byte partial in the uncached area.
+// The last workgroup of the grid then reads every partial (re-reading one until its tag
+// arrives: only this workgroup waits, on workgroups that wait for nothing), runs the block
+// bookkeeping and stop rule of step s (pick_body's, for step s+1), and swaps the next pivot
+// into position s+1 of `at`.  (A ticket taken by every workgroup, the last one reducing, cost
+// more: thousands of atomics on one uncached word are serialized.)  Workgroup 0 stores v(s), tau(s) and R(s,s); the pivot column itself is
+// rewritten one launch later, when no workgroup reads it any more.
+
+struct FState {
+  int j0, jb, done, k, early, sprev, pad0, pad1;
+  double ptol;
+};
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// partial tag: step tag (15 bits, never 0) | recompute flag << 15 | position << 16 |
+// storage column << 40; positions and columns below 2^24 - 1
+constexpr uint64_t F_NONE = (1ull << 24) - 1;
+LRA_DEV uint64_t fstep_tag(int s) { return (uint64_t)(s % 32767) + 1; }
+LRA_DEV uint64_t fpack(int s, int flag, int pos, int col) {
+  const uint64_t pp = pos < 0 ? F_NONE : (uint64_t)pos;
+  const uint64_t cc = col < 0 ? F_NONE : (uint64_t)col;
+  return fstep_tag(s) | ((uint64_t)(flag != 0) << 15) | (pp << 16) | (cc << 40);
+}
+
+// ptol = max(atol, rtol * max_j vn1[j]), the pivot of step 0 swapped into position 0 of `at`
+// (the identity map, from qg_norms), state reset
+__global__ __launch_bounds__(GT) void qg_fstart(int n, int kmax, int nb, const double* vn1,
+                                                double atol, double rtol, FState* st, int* at) {
+  __shared__ double sm[GT / 64], sv[GT / 64];
+  __shared__ int sp[GT / 64];
+  double mx = 0.0, bv = -1.0;
+  int bp = INT_MAX;
+  for (int j = threadIdx.x; j < n; j += GT) {
+    const double v = vn1[j];
+    mx = fmax(mx, v);
+    if (v > bv) { bv = v; bp = j; }  // strided order: the first max of this thread's share
+  }
+  mx = wave_max(mx);
+  double wv;
+  const int wl = wave_argmax(bv, bp, 1, &wv);
+  const int wp = __builtin_amdgcn_readlane(bp, wl < 0 ? 0 : wl);
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = mx;
+    sv[threadIdx.x >> 6] = wv;
+    sp[threadIdx.x >> 6] = wl < 0 ? INT_MAX : wp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m2 = sm[0], cv = sv[0];
+    int cp = sp[0];
+    for (int w = 1; w < GT / 64; ++w) {
+      m2 = fmax(m2, sm[w]);
+      if (sv[w] > cv || (sv[w] == cv && sp[w] < cp)) { cv = sv[w]; cp = sp[w]; }
+    }
+    st->ptol = fmax(atol, rtol * m2);
+    st->j0 = 0;
+    st->jb = min(nb, kmax);
+    st->early = 0;
+    st->sprev = -1;
+    int done = kmax <= 0 ? 1 : 0;
+    st->k = kmax > 0 ? kmax : 0;
+    if (!done && (cp < 0 || cp >= n)) {  // no candidate (NaN norms)
+      st->k = 0;
+      done = 1;
+    }
+    if (!done && cp != 0) {
+      const int t = at[0];
+      at[0] = at[cp];
+      at[cp] = t;
+    }
+    st->done = done;
+  }
+}
+
+// one fused step s; NWC waves per column, RPL rows per lane (rows s..m-1 fit 64 NWC RPL)
+template <int NWC, int RPL>
+__global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, int nb,
+                                                int stop_after, int lastrk,
+                                                double* __restrict__ B, i64 ldb, int* at,
+                                                double* vn1, double* vn2, double* tau,
+                                                double* diag, double* vbuf, FState* st,
+                                                u64x2* part, int* err) {
+  constexpr int CPB = 4 / NWC;  // columns per block
+  constexpr int ROWS = 64 * NWC * RPL;
+  __shared__ double sv[ROWS];  // the pivot column, then v(s .. s + ROWS - 1), zero past m
+  __shared__ double sred[4];
+  __shared__ double sw[CPB][NWC], sss[CPB][NWC], sx[CPB], cval[CPB];
+  __shared__ int cpos[CPB], ccol[CPB], cflag[CPB];
+  __shared__ double tv[4];
+  __shared__ int tp[4], tc[4], tf[4], tok[4];
+  if (st->done) return;  // (set by an earlier launch: the same for every thread)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int h = wave % NWC, cl = wave / NWC;
+  const int p = s + 1 + (int)blockIdx.x * CPB + cl;  // trailing position of this column
+  const bool have = p < n;
+  // (p and so col are the same in every lane of a wave: scalar bases for the column accesses)
+  const int col = __builtin_amdgcn_readfirstlane(have ? at[p] : 0);
+  const int len = m - s;  // rows s..m-1 (len <= ROWS)
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
+  double* c = B + (i64)col * ldb + s;
+  // H(s) from the pivot column (dlarfg on rows s..m-1)
+  const double* pc = B + (i64)__builtin_amdgcn_readfirstlane(at[s]) * ldb + s;
+  double ss = 0.0;
+#pragma unroll 4
+  for (int t = tid; t < ROWS; t += 256) {
+    const double x = t < len ? pc[t] : 0.0;
+    sv[t] = x;
+    ss += t >= 1 ? x * x : 0.0;
+  }
+  ss = group_sum(ss, 64);
+  if (lane == 0) sred[wave] = ss;
+  __syncthreads();
+  const double alpha0 = sv[0];
+  double alpha = alpha0, xnorm = sqrt((sred[0] + sred[1]) + (sred[2] + sred[3]));
+  double beta = alpha0, tu = 0.0, scal = 1.0, fac = 1.0;
+  if (len > 1 && xnorm != 0.0) {
+    beta = -copysign(lapy2(alpha, xnorm), alpha);
+    const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+    int knt = 0;
+    if (fabs(beta) < safmin) {  // (the same in every thread)
+      const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+      do {
+        ++knt;
+        fac *= rsafmn;
+        beta *= rsafmn;
+        alpha *= rsafmn;
+      } while (fabs(beta) < safmin && knt < 20);
+      __syncthreads();  // every thread has read sred
+      double s2 = 0.0;
+      for (int t = tid; t < ROWS; t += 256) {
+        const double x = sv[t] * fac;
+        s2 += (t >= 1 && t < len) ? x * x : 0.0;
+      }
+      s2 = group_sum(s2, 64);
+      if (lane == 0) sred[wave] = s2;
+      __syncthreads();
+      xnorm = sqrt((sred[0] + sred[1]) + (sred[2] + sred[3]));
+      beta = -copysign(lapy2(alpha, xnorm), alpha);
+    }
+    tu = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+    for (int j = 0; j < knt; ++j) beta *= safmin;
+  }
+  __syncthreads();  // every thread has read sv[0]
+#pragma unroll 4
+  for (int t = tid; t < ROWS; t += 256) sv[t] = t == 0 ? 1.0 : (sv[t] * fac) * scal;
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    // v(s), tau(s), R(s,s); the pivot column of step s-1 gets R(s-1,s-1) and its v
+    double* vb = vbuf + (size_t)(s & 1) * m;
+    for (int t = tid; t < len; t += 256) vb[s + t] = sv[t];
+    if (tid == 0) {
+      tau[s] = tu;
+      diag[s] = beta;
+    }
+    if (s > 0) {
+      const double* vp = vbuf + (size_t)((s - 1) & 1) * m;
+      double* d = B + (i64)at[s - 1] * ldb;
+      for (int t = s - 1 + tid; t < m; t += 256) d[t] = t == s - 1 ? diag[s - 1] : vp[t];
+    }
+  }
+  // H(s) applied to this block's columns (loaded only now: holding them across the
+  // reflector's reductions cost occupancy, which this HBM-bound update needs more)
+  double cr[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (NWC * r + h) + lane;
+    cr[r] = (have && o < len) ? c[o] : 0.0;
+  }
+  double w = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) w += sv[64 * (NWC * r + h) + lane] * cr[r];
+  // re-read v from LDS below instead of keeping RPL values live across the reductions
+  asm volatile("" ::: "memory");
+  w = group_sum(w, 64);
+  double wt = w;
+  if constexpr (NWC > 1) {
+    if (lane == 0) sw[cl][h] = w;
+    __syncthreads();
+    wt = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWC; ++q) wt += sw[cl][q];
+  }
+  const double tw = tu * wt;
+  double ss2 = 0.0, x = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (NWC * r + h) + lane;
+    const double y = cr[r] - tw * sv[o];
+    if (have && o < len) {
+      c[o] = y;
+      if (o == 0) x = y;
+      else ss2 += y * y;
+    }
+  }
+  ss2 = group_sum(ss2, 64);
+  if (lane == 0) {
+    sss[cl][h] = ss2;
+    if (h == 0) sx[cl] = x;  // row s is offset 0: lane 0 of the column's first wave
+  }
+  __syncthreads();
+  // norm downdate (LAWN 176) or recompute, and the column's candidate entry
+  if (h == 0 && lane == 0) {
+    double v = -1.0;
+    int fl = 0;
+    if (have) {
+      double a = vn1[col];
+      if (s + 1 < lastrk && a != 0.0) {
+        double temp = fabs(sx[cl]) / a;
+        temp = (1.0 + temp) * (1.0 - temp);
+        temp = temp < 0.0 ? 0.0 : temp;
+        const double q = a / vn2[col];
+        const double temp2 = temp * q * q;
+        if (temp2 <= tol3z) {
+          double ssum = 0.0;
+#pragma unroll
+          for (int q2 = 0; q2 < NWC; ++q2) ssum += sss[cl][q2];
+          a = sqrt(ssum);
+          vn1[col] = vn2[col] = a;
+          fl = 1;
+        } else {
+          a *= sqrt(temp);
+          vn1[col] = a;
+        }
+      }
+      v = a >= 0.0 ? a : -1.0;  // (a NaN norm takes no part)
+    }
+    cval[cl] = v;
+    cpos[cl] = have ? p : INT_MAX;
+    ccol[cl] = col;
+    cflag[cl] = fl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double bv = cval[0];
+    int bp = cpos[0], bc = ccol[0], bf = cflag[0];
+#pragma unroll
+    for (int q = 1; q < CPB; ++q) {
+      if (cval[q] > bv || (cval[q] == bv && cpos[q] < bp)) { bv = cval[q]; bp = cpos[q]; bc = ccol[q]; }
+      bf |= cflag[q];
+    }
+    u64x2 e;
+    e.x = (uint64_t)__double_as_longlong(bv);
+    e.y = bv < 0.0 ? fpack(s, bf, -1, -1) : fpack(s, bf, bp, bc);
+    part[blockIdx.x] = e;  // uncached: a plain store; its tag validates it for the reader
+  }
+  if (blockIdx.x != gridDim.x - 1) return;
+
+  // ---- the last block: every partial (loaded 8 at a time per thread, then validated; a
+  // partial not written yet is re-read until its tag arrives -- no other block waits) ----
+  const int G = (int)gridDim.x;
+  const uint64_t want = fstep_tag(s);
+  double bv = -1.0;
+  int bp = INT_MAX, bc = -1, bf = 0, ok = 1;
+  for (int b0 = tid; b0 < G; b0 += 256 * 8) {
+    u64x2 e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = b0 + 256 * q;
+      e[q] = b < G ? part[b] : u64x2{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = b0 + 256 * q;
+      if (b >= G || !ok) continue;
+      if ((e[q].y & 0x7FFF) != want) {  // not landed yet: bounded re-read
+        const volatile u64x2* src = part + b;
+        unsigned spins = 0;
+        do {
+          __builtin_amdgcn_s_sleep(1);
+          e[q] = *src;
+        } while ((e[q].y & 0x7FFF) != want && ++spins < LRA_SPIN_LIMIT);
+        if ((e[q].y & 0x7FFF) != want) { ok = 0; continue; }
+      }
+      const double v = __longlong_as_double((long long)e[q].x);
+      const uint64_t pp = (e[q].y >> 16) & F_NONE;
+      const int pos = pp == F_NONE ? INT_MAX : (int)pp;
+      if (v > bv || (v == bv && pos < bp)) {
+        bv = v;
+        bp = pos;
+        bc = (int)((e[q].y >> 40) & F_NONE);
+      }
+      bf |= (int)((e[q].y >> 15) & 1);
+    }
+  }
+  double wv;
+  const int wl = wave_argmax(bv, bp, 1, &wv);
+  const int wp = __builtin_amdgcn_readlane(bp, wl < 0 ? 0 : wl);
+  const int wc = __builtin_amdgcn_readlane(bc, wl < 0 ? 0 : wl);
+  const int wf = __any(bf) ? 1 : 0;
+  const int wok = __all(ok) ? 1 : 0;
+  if (lane == 0) {
+    tv[wave] = wv;
+    tp[wave] = wl < 0 ? INT_MAX : wp;
+    tc[wave] = wl < 0 ? -1 : wc;
+    tf[wave] = wf;
+    tok[wave] = wok;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  double cv = tv[0];
+  int cp = tp[0], cc = tc[0], flag = tf[0], allok = tok[0];
+  for (int q = 1; q < 4; ++q) {
+    if (tv[q] > cv || (tv[q] == cv && tp[q] < cp)) { cv = tv[q]; cp = tp[q]; cc = tc[q]; }
+    flag |= tf[q];
+    allok &= tok[q];
+  }
+  st->sprev = s;
+  if (!allok) {  // a partial never landed: stop with an error
+    __hip_atomic_store(err, LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->k = s + 1;
+    st->done = 1;
+    return;
+  }
+  // bookkeeping of step s (pick_body's at step s+1; R(s,s) is this launch's beta)
+  const int s1 = s + 1;
+  int done = 0;
+  const bool blk_end = (s - st->j0 + 1 == st->jb) || flag != 0 || s1 == kmax;
+  if (blk_end) {
+    const double ptol = st->ptol;
+    bool stop = false;
+    if (fabs(beta) <= ptol) {
+      int i = st->j0;
+      for (; i < s; ++i)
+        if (fabs(diag[i]) <= ptol) break;
+      st->k = i;
+      stop = true;
+    }
+    if (!stop && s1 == kmax) { st->k = kmax; stop = true; }
+    if (!stop && stop_after > 0 && s1 >= stop_after) {
+      st->k = s1;
+      st->early = 1;
+      stop = true;
+    }
+    if (stop) {
+      done = 1;
+    } else {
+      st->j0 = s1;
+      st->jb = min(nb, kmax - s1);
+    }
+  }
+  if (!done && s1 == kmax) { st->k = kmax; done = 1; }
+  if (!done) {
+    if (cp < s1 || cp >= n || cc < 0) {  // no candidate (NaN norms)
+      st->k = s1;
+      done = 1;
+    } else if (cp != s1) {  // LAPACK's swap of positions s+1 and the pivot's
+      at[cp] = at[s1];
+      at[s1] = cc;
+    }
+  }
+  st->done = done;
+}
+
+// after the last fused step: the last pivot column's R(s,s) and v, k and the early flag
+__global__ __launch_bounds__(256) void qg_ffinish(int m, double* B, i64 ldb, const int* at,
+                                                  const double* diag, const double* vbuf,
+                                                  const FState* st, i64* k_out, int* early_out) {
+  const int sp = st->sprev;
+  if (sp >= 0) {
+    const double* vp = vbuf + (size_t)(sp & 1) * m;
+    double* d = B + (i64)at[sp] * ldb;
+    for (int t = sp + (int)threadIdx.x; t < m; t += 256) d[t] = t == sp ? diag[sp] : vp[t];
+  }
+  if (threadIdx.x == 0) {
+    *k_out = st->k;
+    if (early_out) *early_out = st->early;
+  }
+}
+
+// columns to LAPACK positions: B(:, p) = S(:, at[p]), jpvt[p] = at[p] + 1 (one block per p)
+__global__ __launch_bounds__(256) void qg_gather(int m, const double* __restrict__ S, i64 lds,
+                                                 double* __restrict__ B, i64 ldb, const int* at,
+                                                 i64* jpvt) {
+  const int p = blockIdx.x;
+  const int c = at[p];
+  const double* src = S + (i64)c * lds;
+  double* dst = B + (i64)p * ldb;
+  for (int i = threadIdx.x; i < m; i += 256) dst[i] = src[i];
+  if (threadIdx.x == 0) jpvt[p] = (i64)c + 1;
+}
+
+// ---------------------------------------------------------------------------------------
 // Complex (ComplexF64) steps: zlaqps' arithmetic (src/lapack.jl:117-139 calls zlaqps for
 // complex element types).  Columns are interleaved (re, im) pairs; the norms, the diagonal
 // and the stop rule are real.  H(s) = I - tau v v^H is applied as H^H from the left
@@ -664,6 +1062,32 @@ int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, in
   return LRA_OK;
 }
 
+namespace {
+
+// the fused launch of step s: rows s..m-1 (at most 8192) pick the form, as for the applies
+void launch_fused(hipStream_t s0, int s, const QrcpCall& q, int lastrk, int* at, double* vn1,
+                  double* vn2, double* diag, double* vbuf, FState* st, u64x2* part, int* err) {
+  const int m = q.m, n = q.n, rows = m - s, tr = n - s - 1;
+  auto grid = [&](int cpb) { return dim3((unsigned)std::max(1, (tr + cpb - 1) / cpb)); };
+#define QF_ARGS s, m, n, q.kmax, q.nb, q.stop_after, lastrk, q.A, q.lda, at, vn1, vn2, q.tau, \
+                diag, vbuf, st, part, err
+  if (rows <= 64 * 8)
+    hipLaunchKernelGGL((qg_fused<1, 8>), grid(4), dim3(256), 0, s0, QF_ARGS);
+  else if (rows <= 64 * 16)
+    hipLaunchKernelGGL((qg_fused<1, 16>), grid(4), dim3(256), 0, s0, QF_ARGS);
+  else if (rows <= 128 * 16)
+    hipLaunchKernelGGL((qg_fused<2, 16>), grid(2), dim3(256), 0, s0, QF_ARGS);
+  else if (rows <= 128 * 24)
+    hipLaunchKernelGGL((qg_fused<2, 24>), grid(2), dim3(256), 0, s0, QF_ARGS);
+  else if (rows <= 128 * 32)
+    hipLaunchKernelGGL((qg_fused<2, 32>), grid(2), dim3(256), 0, s0, QF_ARGS);
+  else
+    hipLaunchKernelGGL((qg_fused<4, 32>), grid(1), dim3(256), 0, s0, QF_ARGS);
+#undef QF_ARGS
+}
+
+}  // namespace
+
 // geqp3_adap (mode 0 of QrcpCall) on a sketch of any size, in HBM.
 int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
   if (q.mode != 0) {
@@ -677,18 +1101,64 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
   }
   const int m = q.m, n = q.n;
   double* w = nullptr;
-  // vn1, vn2, diag (kmax), v (m), state
-  const size_t nd = 2 * (size_t)n + (size_t)std::max(q.kmax, 1) + (size_t)m + 16;
+  // vn1, vn2, diag (kmax), v (2 m), state, position map (n ints)
+  const size_t nd = 2 * (size_t)n + (size_t)std::max(q.kmax, 1) + 2 * (size_t)m + 16 +
+                    ((size_t)n + 1) / 2;
   LRA_CHECK(lra_wsT(c, WS_QG, nd, &w));
   double* vn1 = w;
   double* vn2 = vn1 + n;
   double* diag = vn2 + n;
   double* v = diag + std::max(q.kmax, 1);
-  GState* st = reinterpret_cast<GState*>(v + m);
+  GState* st = reinterpret_cast<GState*>(v + 2 * (size_t)m);
+  int* at = reinterpret_cast<int*>(v + 2 * (size_t)m + 16);
   const int lastrk = std::min(m, n);
   hipStream_t s0 = c->stream;
+
+  // one launch per step (rows up to 8192, with the uncached exchange area for the partials);
+  // LRA_QG_FUSED=0 keeps the two launches per step below
+  const char* fe = getenv("LRA_QG_FUSED");
+  if (!(fe && fe[0] == '0') && m <= 8192 && n < (int)F_NONE && q.kmax > 0) {
+    void* area = nullptr;
+    bool uc = false;
+    const size_t bytes = 256 + 16 * (size_t)n;
+    LRA_CHECK(lra_coop_area(c, bytes, &area, &uc));
+    if (uc) {
+      double* S = nullptr;  // the columns in storage order, for the final permutation
+      LRA_CHECK(lra_wsT(c, WS_QGT, (size_t)m * n, &S));
+      u64x2* part = reinterpret_cast<u64x2*>((char*)area + 256);
+      FState* fs = reinterpret_cast<FState*>(st);
+      LRA_CHECK(lra_launch_zero_u32(s0, (unsigned*)area, (i64)(bytes / 4)));
+      hipLaunchKernelGGL(qg_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, q.A,
+                         q.lda, vn1, vn2, q.jpvt, at);
+      hipLaunchKernelGGL(qg_fstart, dim3(1), dim3(GT), 0, s0, n, q.kmax, q.nb, vn1, q.atol,
+                         q.rtol, fs, at);
+      LRA_HIP(hipGetLastError());
+      for (int s = 0; s < q.kmax; ++s) {
+        launch_fused(s0, s, q, lastrk, at, vn1, vn2, diag, v, fs, part, c->d_err);
+        LRA_HIP(hipGetLastError());
+        if ((s & 63) == 63) {  // stop enqueueing once the device has stopped
+          int done = 0;
+          LRA_HIP(hipMemcpyAsync(c->host_pinned, &fs->done, sizeof(int), hipMemcpyDeviceToHost,
+                                 s0));
+          LRA_HIP(hipStreamSynchronize(s0));
+          std::memcpy(&done, c->host_pinned, sizeof(int));
+          if (done) break;
+        }
+      }
+      hipLaunchKernelGGL(qg_ffinish, dim3(1), dim3(256), 0, s0, m, q.A, q.lda, at, diag, v, fs,
+                         q.k_out, q.early_out);
+      LRA_HIP(hipMemcpy2DAsync(S, (size_t)m * sizeof(double), q.A, (size_t)q.lda * sizeof(double),
+                               (size_t)m * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice,
+                               s0));
+      hipLaunchKernelGGL(qg_gather, dim3((unsigned)n), dim3(256), 0, s0, m, S, (i64)m, q.A, q.lda,
+                         at, q.jpvt);
+      LRA_HIP(hipGetLastError());
+      return LRA_OK;
+    }
+  }
+
   hipLaunchKernelGGL(qg_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, q.A, q.lda,
-                     vn1, vn2, q.jpvt);
+                     vn1, vn2, q.jpvt, nullptr);
   hipLaunchKernelGGL(qg_init, dim3(1), dim3(GT), 0, s0, n, vn1, q.atol, q.rtol, q.kmax, q.nb,
                      st);
   LRA_HIP(hipGetLastError());

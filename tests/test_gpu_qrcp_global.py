@@ -13,11 +13,15 @@ pytestmark = pytest.mark.gpu
                                       (12, 5, 0.0), (33, 700, 1e-6), (700, 400, 1e-12),
                                       (1032, 3000, 1e-10), (2056, 2500, 1e-12),
                                       (3500, 1200, 1e-12), (6000, 300, 1e-12)])
-def test_global_qrcp_vs_oracle(gpu, orc, monkeypatch, m, n, rtol):
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_global_qrcp_vs_oracle(gpu, orc, monkeypatch, m, n, rtol, fused):
+    """Both forms of the HBM path: one fused launch per step (columns in place, permuted at
+    the end) and the two launches per step (LRA_QG_FUSED=0)."""
     import torch
     monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
+    monkeypatch.setenv("LRA_QG_FUSED", fused)
     B = graded(m, n, m * 7 + n, decay=0.85 if m < 100 else 0.95)
-    print(f"global geqp3_adap m={m} n={n} rtol={rtol}", flush=True)
+    print(f"global geqp3_adap m={m} n={n} rtol={rtol} fused={fused}", flush=True)
     Bf, p_o, tau_o, k_o, gap = orc.geqp3_adap(B, -1, 0.0, rtol, 32, want_gap=True)
     Bd = gpu.to_device(B)
     p_g, tau_g, k_g = gpu.geqp3_adap(Bd, -1, 0.0, rtol, 32)
@@ -45,6 +49,37 @@ def test_global_matches_register_kernel(gpu, monkeypatch):
     assert np.array_equal(host(p1)[:k1], host(p2)[:k2])
     R1, R2 = np.triu(host(B1)[:k1]), np.triu(host(B2)[:k2])
     assert np.abs(R1 - R2).max() <= 1e-10 * np.abs(R1).max()
+
+
+@pytest.mark.parametrize("m,n,rtol,rank", [(2056, 3000, 1e-10, -1), (1032, 2600, 0.0, 700),
+                                            (700, 5000, 1e-13, -1), (5000, 900, 1e-12, -1)])
+def test_fused_matches_two_launch_form(gpu, monkeypatch, m, n, rtol, rank):
+    """The fused HBM steps against the two-launch form on the same sketch: the same rank, the
+    same pivots, R to rounding, and each reflector and tau to rounding amplified by the
+    condition of its step, |R(0,0)| / |R(j,j)| (the two forms sum norms and dot products in
+    different orders; graded columns, so no near ties); rank 700 ends inside a block
+    (700 = 21 blocks of 32 + 28 steps)."""
+    import torch
+    monkeypatch.setenv("LRA_QRCP_GLOBAL", "1")
+    B = graded(m, n, m + 3 * n, decay=0.97)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("LRA_QG_FUSED", fused)
+        print(f"fused vs two-launch m={m} n={n} rtol={rtol} rank={rank} fused={fused}", flush=True)
+        Bd = gpu.to_device(B)
+        p, tau, k = gpu.geqp3_adap(Bd, rank, 0.0, rtol, 32)
+        torch.cuda.synchronize()
+        out[fused] = (host(p), host(tau), k, host(Bd))
+    (p1, t1, k1, B1), (p0, t0, k0, B0) = out["1"], out["0"]
+    assert k1 == k0
+    assert np.array_equal(p1, p0)
+    kk = min(k1, m, n)
+    scale = np.abs(B0[:kk]).max()
+    assert np.abs(np.triu(B1[:kk]) - np.triu(B0[:kk])).max() <= 1e-11 * scale
+    cond = np.abs(B0[0, 0]) / np.abs(np.diag(B0[:kk, :kk]))
+    dv = np.abs(np.tril(B1[:, :kk], -1) - np.tril(B0[:, :kk], -1)).max(axis=0)
+    assert np.all(dv <= 1e-12 * cond), int(np.argmax(dv / cond))
+    assert np.all(np.abs(t1[:kk] - t0[:kk]) <= 1e-12 * cond)
 
 
 def test_wide_matrix_beyond_register_kernel(gpu, orc):
