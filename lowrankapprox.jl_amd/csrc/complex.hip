@@ -171,6 +171,63 @@ __global__ void ztrsm_T_kernel(i64 k, i64 nr, const zc* R, i64 ldr, zc* T, i64 l
   }
 }
 
+// The complex T = R11 \ R12 as one REAL triangular solve on the MFMA T kernel.  Row i of
+// [R11 R12] is scaled by u_i = conj(R(i,i)) / |R(i,i)|, which leaves T unchanged and makes
+// the diagonal real (u_i = +-1, an exact scaling, when R(i,i) is real, as on every R of the
+// pivoted QR here: zlarfg leaves beta real).  With each complex entry a + ib written as the
+// real 2 x 2 block [a -b; b a], the scaled R11 is an upper-triangular 2k x 2k real matrix
+// (diagonal blocks |R(i,i)| I), and a complex column of T read as interleaved (re, im)
+// doubles is a real column of 2k rows: the real solve is the complex one, term for term.
+LRA_DEV zc zmul(zc a, zc b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+LRA_DEV zc zunit_conj(zc d) {
+  const double a = hypot(d.x, d.y);
+  return a == 0.0 ? make_double2(1.0, 0.0) : make_double2(d.x / a, -d.y / a);
+}
+
+// Rr (2k x 2k, column-major) from the row-scaled R11
+__global__ void zrealify_R11_kernel(i64 k, const zc* R, i64 ldr, double* Rr) {
+  const i64 k2 = 2 * k;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < k2 * k2;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i2 = e % k2, j2 = e / k2, i = i2 >> 1, j = j2 >> 1;
+    double v = 0.0;
+    if (i <= j) {
+      const zc d = R[i + i * ldr];
+      const zc a = i == j ? make_double2(hypot(d.x, d.y), 0.0) : zmul(zunit_conj(d), R[i + j * ldr]);
+      v = (i2 & 1) == 0 ? ((j2 & 1) == 0 ? a.x : -a.y) : ((j2 & 1) == 0 ? a.y : a.x);
+    }
+    Rr[e] = v;
+  }
+}
+
+// T(i, c) = u_i R12(i, c)
+__global__ void zscale_R12_kernel(i64 k, i64 nr, const zc* R, i64 ldr, zc* T, i64 ldt) {
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < k * nr;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % k, c = e / k;
+    T[i + c * ldt] = zmul(zunit_conj(R[i + i * ldr]), R[i + (k + c) * ldr]);
+  }
+}
+
+// T (k x nr, ldt) = R11 \ R12 for the complex R = [R11 R12] (ldr)
+int zid_T_solve(lra_ctx* c, i64 k, i64 nr, const zc* R, i64 ldr, zc* T, i64 ldt) {
+  if (k <= 0 || nr <= 0) return LRA_OK;
+  if (2 * k > 64 * 1184) {  // beyond the blocked real solve: one thread per right-hand side
+    hipLaunchKernelGGL(ztrsm_T_kernel, dim3(grid_for(nr)), dim3(256), 0, c->stream, k, nr, R,
+                       ldr, T, ldt);
+    LRA_LAUNCH_CHECK();
+    return LRA_OK;
+  }
+  double* Rr = nullptr;
+  LRA_CHECK(lra_wsT(c, WS_ZT, (size_t)4 * k * k, &Rr));
+  hipLaunchKernelGGL(zrealify_R11_kernel, dim3(grid_for(4 * k * k)), dim3(256), 0, c->stream, k,
+                     R, ldr, Rr);
+  hipLaunchKernelGGL(zscale_R12_kernel, dim3(grid_for(k * nr)), dim3(256), 0, c->stream, k, nr,
+                     R, ldr, T, ldt);
+  LRA_LAUNCH_CHECK();
+  return lra_trsm_upper_inplace(c, 2 * k, nr, Rr, 2 * k, reinterpret_cast<double*>(T), 2 * ldt);
+}
+
 // the random objects of a complex left sketch of round `stream` on a P-row operand
 struct ZDraw {
   zc* d;    // srft signs (P) / sprn coefficients (P)
@@ -364,10 +421,8 @@ extern "C" int lra_zid_T(lra_ctx* c, int64_t k, int64_t n, const double* R, int6
   LRA_ARG(ldr >= std::max<int64_t>(1, k), 5);
   LRA_ARG(ldt >= std::max<int64_t>(1, k), 7);
   if (k == 0 || n == k) return LRA_OK;
-  hipLaunchKernelGGL(ztrsm_T_kernel, dim3(grid_for(n - k)), dim3(256), 0, c->stream, k, n - k,
-                     reinterpret_cast<const zc*>(R), ldr, reinterpret_cast<zc*>(T), ldt);
-  LRA_LAUNCH_CHECK();
-  return LRA_OK;
+  return zid_T_solve(c, k, n - k, reinterpret_cast<const zc*>(R), ldr, reinterpret_cast<zc*>(T),
+                     ldt);
 }
 
 // idfact(trans, A) for a ComplexF64 A (interleaved re/im, lda in complex elements).  T is
@@ -465,9 +520,8 @@ extern "C" int lra_zidfact(lra_ctx* c, int trans, int64_t m, int64_t n, const do
   }
   if (k > 0 && N > k) {
     LRA_ARG(out->ldt >= k, 8);
-    hipLaunchKernelGGL(ztrsm_T_kernel, dim3(grid_for(N - k)), dim3(256), 0, c->stream, k, N - k, B,
-                       std::max<i64>(ell, 1), reinterpret_cast<zc*>(out->T), out->ldt);
-    LRA_LAUNCH_CHECK();
+    LRA_CHECK(zid_T_solve(c, k, N - k, B, std::max<i64>(ell, 1), reinterpret_cast<zc*>(out->T),
+                          out->ldt));
   }
   out->k = k;
   out->rounds = rounds;

@@ -517,6 +517,139 @@ __global__ __launch_bounds__(GT) void qg_fstart(int n, int kmax, int nb, const d
   }
 }
 
+// The end of a fused step s, after the block's column candidates are in LDS (cval, cpos,
+// ccol, cflag): the block's best (largest norm, then lowest position) as its tagged partial;
+// then the grid's last block reads every partial, runs the bookkeeping and stop rule of step
+// s (pick_body's, for step s+1; rdiag = |R(s,s)| of this step) and swaps the winner into
+// position s+1 of `at`.
+template <int CPB>
+LRA_DEV void fused_finish(int s, int n, int kmax, int nb, int stop_after, double rdiag,
+                          const double* cval, const int* cpos, const int* ccol, const int* cflag,
+                          int* at, const double* diag, FState* st, u64x2* part, int* err) {
+  __shared__ double tv[4];
+  __shared__ int tp[4], tc[4], tf[4], tok[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    double bv = cval[0];
+    int bp = cpos[0], bc = ccol[0], bf = cflag[0];
+#pragma unroll
+    for (int q = 1; q < CPB; ++q) {
+      if (cval[q] > bv || (cval[q] == bv && cpos[q] < bp)) { bv = cval[q]; bp = cpos[q]; bc = ccol[q]; }
+      bf |= cflag[q];
+    }
+    u64x2 e;
+    e.x = (uint64_t)__double_as_longlong(bv);
+    e.y = bv < 0.0 ? fpack(s, bf, -1, -1) : fpack(s, bf, bp, bc);
+    part[blockIdx.x] = e;  // uncached: a plain store; its tag validates it for the reader
+  }
+  if (blockIdx.x != gridDim.x - 1) return;
+
+  // ---- the last block: every partial (loaded 8 at a time per thread, then validated; a
+  // partial not written yet is re-read until its tag arrives -- no other block waits) ----
+  const int G = (int)gridDim.x;
+  const uint64_t want = fstep_tag(s);
+  double bv = -1.0;
+  int bp = INT_MAX, bc = -1, bf = 0, ok = 1;
+  for (int b0 = tid; b0 < G; b0 += 256 * 8) {
+    u64x2 e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = b0 + 256 * q;
+      e[q] = b < G ? part[b] : u64x2{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int b = b0 + 256 * q;
+      if (b >= G || !ok) continue;
+      if ((e[q].y & 0x7FFF) != want) {  // not landed yet: bounded re-read
+        const volatile u64x2* src = part + b;
+        unsigned spins = 0;
+        do {
+          __builtin_amdgcn_s_sleep(1);
+          e[q] = *src;
+        } while ((e[q].y & 0x7FFF) != want && ++spins < LRA_SPIN_LIMIT);
+        if ((e[q].y & 0x7FFF) != want) { ok = 0; continue; }
+      }
+      const double v = __longlong_as_double((long long)e[q].x);
+      const uint64_t pp = (e[q].y >> 16) & F_NONE;
+      const int pos = pp == F_NONE ? INT_MAX : (int)pp;
+      if (v > bv || (v == bv && pos < bp)) {
+        bv = v;
+        bp = pos;
+        bc = (int)((e[q].y >> 40) & F_NONE);
+      }
+      bf |= (int)((e[q].y >> 15) & 1);
+    }
+  }
+  double wv;
+  const int wl = wave_argmax(bv, bp, 1, &wv);
+  const int wp = __builtin_amdgcn_readlane(bp, wl < 0 ? 0 : wl);
+  const int wc = __builtin_amdgcn_readlane(bc, wl < 0 ? 0 : wl);
+  const int wf = __any(bf) ? 1 : 0;
+  const int wok = __all(ok) ? 1 : 0;
+  if (lane == 0) {
+    tv[wave] = wv;
+    tp[wave] = wl < 0 ? INT_MAX : wp;
+    tc[wave] = wl < 0 ? -1 : wc;
+    tf[wave] = wf;
+    tok[wave] = wok;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  double cv = tv[0];
+  int cp = tp[0], cc = tc[0], flag = tf[0], allok = tok[0];
+  for (int q = 1; q < 4; ++q) {
+    if (tv[q] > cv || (tv[q] == cv && tp[q] < cp)) { cv = tv[q]; cp = tp[q]; cc = tc[q]; }
+    flag |= tf[q];
+    allok &= tok[q];
+  }
+  st->sprev = s;
+  if (!allok) {  // a partial never landed: stop with an error
+    __hip_atomic_store(err, LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st->k = s + 1;
+    st->done = 1;
+    return;
+  }
+  // bookkeeping of step s (pick_body's at step s+1; R(s,s) is this launch's beta)
+  const int s1 = s + 1;
+  int done = 0;
+  const bool blk_end = (s - st->j0 + 1 == st->jb) || flag != 0 || s1 == kmax;
+  if (blk_end) {
+    const double ptol = st->ptol;
+    bool stop = false;
+    if (fabs(rdiag) <= ptol) {
+      int i = st->j0;
+      for (; i < s; ++i)
+        if (fabs(diag[i]) <= ptol) break;
+      st->k = i;
+      stop = true;
+    }
+    if (!stop && s1 == kmax) { st->k = kmax; stop = true; }
+    if (!stop && stop_after > 0 && s1 >= stop_after) {
+      st->k = s1;
+      st->early = 1;
+      stop = true;
+    }
+    if (stop) {
+      done = 1;
+    } else {
+      st->j0 = s1;
+      st->jb = min(nb, kmax - s1);
+    }
+  }
+  if (!done && s1 == kmax) { st->k = kmax; done = 1; }
+  if (!done) {
+    if (cp < s1 || cp >= n || cc < 0) {  // no candidate (NaN norms)
+      st->k = s1;
+      done = 1;
+    } else if (cp != s1) {  // LAPACK's swap of positions s+1 and the pivot's
+      at[cp] = at[s1];
+      at[s1] = cc;
+    }
+  }
+  st->done = done;
+}
+
 // one fused step s; NWC waves per column, RPL rows per lane (rows s..m-1 fit 64 NWC RPL)
 template <int NWC, int RPL>
 __global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, int nb,
@@ -531,8 +664,6 @@ __global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, i
   __shared__ double sred[4];
   __shared__ double sw[CPB][NWC], sss[CPB][NWC], sx[CPB], cval[CPB];
   __shared__ int cpos[CPB], ccol[CPB], cflag[CPB];
-  __shared__ double tv[4];
-  __shared__ int tp[4], tc[4], tf[4], tok[4];
   if (st->done) return;  // (set by an earlier launch: the same for every thread)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int h = wave % NWC, cl = wave / NWC;
@@ -676,125 +807,8 @@ __global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, i
     cflag[cl] = fl;
   }
   __syncthreads();
-  if (tid == 0) {
-    double bv = cval[0];
-    int bp = cpos[0], bc = ccol[0], bf = cflag[0];
-#pragma unroll
-    for (int q = 1; q < CPB; ++q) {
-      if (cval[q] > bv || (cval[q] == bv && cpos[q] < bp)) { bv = cval[q]; bp = cpos[q]; bc = ccol[q]; }
-      bf |= cflag[q];
-    }
-    u64x2 e;
-    e.x = (uint64_t)__double_as_longlong(bv);
-    e.y = bv < 0.0 ? fpack(s, bf, -1, -1) : fpack(s, bf, bp, bc);
-    part[blockIdx.x] = e;  // uncached: a plain store; its tag validates it for the reader
-  }
-  if (blockIdx.x != gridDim.x - 1) return;
-
-  // ---- the last block: every partial (loaded 8 at a time per thread, then validated; a
-  // partial not written yet is re-read until its tag arrives -- no other block waits) ----
-  const int G = (int)gridDim.x;
-  const uint64_t want = fstep_tag(s);
-  double bv = -1.0;
-  int bp = INT_MAX, bc = -1, bf = 0, ok = 1;
-  for (int b0 = tid; b0 < G; b0 += 256 * 8) {
-    u64x2 e[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int b = b0 + 256 * q;
-      e[q] = b < G ? part[b] : u64x2{0, 0};
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int b = b0 + 256 * q;
-      if (b >= G || !ok) continue;
-      if ((e[q].y & 0x7FFF) != want) {  // not landed yet: bounded re-read
-        const volatile u64x2* src = part + b;
-        unsigned spins = 0;
-        do {
-          __builtin_amdgcn_s_sleep(1);
-          e[q] = *src;
-        } while ((e[q].y & 0x7FFF) != want && ++spins < LRA_SPIN_LIMIT);
-        if ((e[q].y & 0x7FFF) != want) { ok = 0; continue; }
-      }
-      const double v = __longlong_as_double((long long)e[q].x);
-      const uint64_t pp = (e[q].y >> 16) & F_NONE;
-      const int pos = pp == F_NONE ? INT_MAX : (int)pp;
-      if (v > bv || (v == bv && pos < bp)) {
-        bv = v;
-        bp = pos;
-        bc = (int)((e[q].y >> 40) & F_NONE);
-      }
-      bf |= (int)((e[q].y >> 15) & 1);
-    }
-  }
-  double wv;
-  const int wl = wave_argmax(bv, bp, 1, &wv);
-  const int wp = __builtin_amdgcn_readlane(bp, wl < 0 ? 0 : wl);
-  const int wc = __builtin_amdgcn_readlane(bc, wl < 0 ? 0 : wl);
-  const int wf = __any(bf) ? 1 : 0;
-  const int wok = __all(ok) ? 1 : 0;
-  if (lane == 0) {
-    tv[wave] = wv;
-    tp[wave] = wl < 0 ? INT_MAX : wp;
-    tc[wave] = wl < 0 ? -1 : wc;
-    tf[wave] = wf;
-    tok[wave] = wok;
-  }
-  __syncthreads();
-  if (tid != 0) return;
-  double cv = tv[0];
-  int cp = tp[0], cc = tc[0], flag = tf[0], allok = tok[0];
-  for (int q = 1; q < 4; ++q) {
-    if (tv[q] > cv || (tv[q] == cv && tp[q] < cp)) { cv = tv[q]; cp = tp[q]; cc = tc[q]; }
-    flag |= tf[q];
-    allok &= tok[q];
-  }
-  st->sprev = s;
-  if (!allok) {  // a partial never landed: stop with an error
-    __hip_atomic_store(err, LRA_E_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    st->k = s + 1;
-    st->done = 1;
-    return;
-  }
-  // bookkeeping of step s (pick_body's at step s+1; R(s,s) is this launch's beta)
-  const int s1 = s + 1;
-  int done = 0;
-  const bool blk_end = (s - st->j0 + 1 == st->jb) || flag != 0 || s1 == kmax;
-  if (blk_end) {
-    const double ptol = st->ptol;
-    bool stop = false;
-    if (fabs(beta) <= ptol) {
-      int i = st->j0;
-      for (; i < s; ++i)
-        if (fabs(diag[i]) <= ptol) break;
-      st->k = i;
-      stop = true;
-    }
-    if (!stop && s1 == kmax) { st->k = kmax; stop = true; }
-    if (!stop && stop_after > 0 && s1 >= stop_after) {
-      st->k = s1;
-      st->early = 1;
-      stop = true;
-    }
-    if (stop) {
-      done = 1;
-    } else {
-      st->j0 = s1;
-      st->jb = min(nb, kmax - s1);
-    }
-  }
-  if (!done && s1 == kmax) { st->k = kmax; done = 1; }
-  if (!done) {
-    if (cp < s1 || cp >= n || cc < 0) {  // no candidate (NaN norms)
-      st->k = s1;
-      done = 1;
-    } else if (cp != s1) {  // LAPACK's swap of positions s+1 and the pivot's
-      at[cp] = at[s1];
-      at[s1] = cc;
-    }
-  }
-  st->done = done;
+  fused_finish<CPB>(s, n, kmax, nb, stop_after, beta, cval, cpos, ccol, cflag, at, diag, st,
+                    part, err);
 }
 
 // after the last fused step: the last pivot column's R(s,s) and v, k and the early flag
@@ -855,7 +869,7 @@ LRA_DEV double lapy3(double x, double y, double z) {
 }
 
 __global__ __launch_bounds__(256) void zq_norms(int m, int n, const zc* __restrict__ B, i64 ldb,
-                                                double* vn1, double* vn2, i64* jpvt) {
+                                                double* vn1, double* vn2, i64* jpvt, int* at) {
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;
@@ -866,6 +880,7 @@ __global__ __launch_bounds__(256) void zq_norms(int m, int n, const zc* __restri
   if (lane == 0) {
     vn1[j] = vn2[j] = sqrt(ss);
     jpvt[j] = j + 1;
+    if (at) at[j] = j;
   }
 }
 
@@ -1006,6 +1021,223 @@ __global__ __launch_bounds__(256) void zq_apply(int s, int m, int n, int lastrk,
   }
 }
 
+// One fused complex step s: zq_step's zlarfg and zq_apply's update and downdate in the
+// structure of qg_fused (every block rebuilds H(s) from the pivot column at[s]; its columns are
+// read into registers once and written once; the grid's last block picks the next pivot).
+// vbuf holds R(s,s) at row s and v below it (v(s) = 1 is implicit).
+template <int NWC, int RPL>
+__global__ __launch_bounds__(256) void zq_fused(int s, int m, int n, int kmax, int nb,
+                                                int stop_after, int lastrk,
+                                                zc* __restrict__ B, i64 ldb, int* at,
+                                                double* vn1, double* vn2, zc* tau,
+                                                double* diag, zc* vbuf, FState* st,
+                                                u64x2* part, int* err) {
+  constexpr int CPB = 4 / NWC;  // columns per block
+  constexpr int ROWS = 64 * NWC * RPL;
+  __shared__ zc sv[ROWS];  // the pivot column, then v(s .. s + ROWS - 1), zero past m
+  __shared__ double sred[4];
+  __shared__ double swr[CPB][NWC], swi[CPB][NWC], sss[CPB][NWC], cval[CPB];
+  __shared__ zc sx[CPB];
+  __shared__ int cpos[CPB], ccol[CPB], cflag[CPB];
+  if (st->done) return;  // (set by an earlier launch: the same for every thread)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int h = wave % NWC, cl = wave / NWC;
+  const int p = s + 1 + (int)blockIdx.x * CPB + cl;  // trailing position of this column
+  const bool have = p < n;
+  // (p and so col are the same in every lane of a wave: scalar bases for the column accesses)
+  const int col = __builtin_amdgcn_readfirstlane(have ? at[p] : 0);
+  const int len = m - s;  // rows s..m-1 (len <= ROWS)
+  const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('E'))
+  zc* c = B + (i64)col * ldb + s;
+  // H(s) from the pivot column (zlarfg on rows s..m-1)
+  const zc* pc = B + (i64)__builtin_amdgcn_readfirstlane(at[s]) * ldb + s;
+  double ss = 0.0;
+#pragma unroll 4
+  for (int t = tid; t < ROWS; t += 256) {
+    const zc x = t < len ? pc[t] : make_double2(0.0, 0.0);
+    sv[t] = x;
+    ss += t >= 1 ? zabs2(x) : 0.0;
+  }
+  ss = group_sum(ss, 64);
+  if (lane == 0) sred[wave] = ss;
+  __syncthreads();
+  const zc alpha0 = sv[0];
+  double alphr = alpha0.x, alphi = alpha0.y;
+  double xnorm = sqrt((sred[0] + sred[1]) + (sred[2] + sred[3])), fac = 1.0;
+  zc tu = make_double2(0.0, 0.0), scal = make_double2(1.0, 0.0);
+  double beta = alphr;
+  const bool reflect = len > 1 ? (xnorm != 0.0 || alphi != 0.0) : (alphi != 0.0);
+  if (reflect) {
+    beta = -copysign(lapy3(alphr, alphi, xnorm), alphr);
+    const double safmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+    int knt = 0;
+    if (fabs(beta) < safmin) {  // (the same in every thread)
+      const double rsafmn = 1.0 / safmin;  // a power of two: the rescaling is exact
+      do {
+        ++knt;
+        fac *= rsafmn;
+        beta *= rsafmn;
+        alphr *= rsafmn;
+        alphi *= rsafmn;
+      } while (fabs(beta) < safmin && knt < 20);
+      __syncthreads();  // every thread has read sred
+      double s2 = 0.0;
+      for (int t = tid; t < ROWS; t += 256) {
+        const zc x = sv[t];
+        s2 += (t >= 1 && t < len) ? zabs2(make_double2(x.x * fac, x.y * fac)) : 0.0;
+      }
+      s2 = group_sum(s2, 64);
+      if (lane == 0) sred[wave] = s2;
+      __syncthreads();
+      xnorm = sqrt((sred[0] + sred[1]) + (sred[2] + sred[3]));
+      beta = -copysign(lapy3(alphr, alphi, xnorm), alphr);
+    }
+    tu = make_double2((beta - alphr) / beta, -alphi / beta);
+    scal = zrecip(make_double2(alphr - beta, alphi));
+    for (int j = 0; j < knt; ++j) beta *= safmin;
+  }
+  const zc rss = reflect ? make_double2(beta, 0.0) : alpha0;  // R(s,s)
+  const double dss = reflect ? beta : hypot(alpha0.x, alpha0.y);
+  __syncthreads();  // every thread has read sv[0]
+#pragma unroll 4
+  for (int t = tid; t < ROWS; t += 256) {
+    const zc x = sv[t];
+    sv[t] = t == 0 ? make_double2(1.0, 0.0) : zmul(make_double2(x.x * fac, x.y * fac), scal);
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    // R(s,s), v(s+1..), tau(s), |R(s,s)|; the pivot column of step s-1 rewritten
+    zc* vb = vbuf + (size_t)(s & 1) * m;
+    for (int t = tid; t < len; t += 256) vb[s + t] = t == 0 ? rss : sv[t];
+    if (tid == 0) {
+      tau[s] = tu;
+      diag[s] = dss;
+    }
+    if (s > 0) {
+      const zc* vp = vbuf + (size_t)((s - 1) & 1) * m;
+      zc* d = B + (i64)at[s - 1] * ldb;
+      for (int t = s - 1 + tid; t < m; t += 256) d[t] = vp[t];
+    }
+  }
+  // a_j -= conj(tau) v (v^H a_j) on this block's columns (loaded only now)
+  zc cr[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (NWC * r + h) + lane;
+    cr[r] = (have && o < len) ? c[o] : make_double2(0.0, 0.0);
+  }
+  double wr = 0.0, wi = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const zc q = zconj_mul(sv[64 * (NWC * r + h) + lane], cr[r]);
+    wr += q.x;
+    wi += q.y;
+  }
+  // re-read v from LDS below instead of keeping RPL values live across the reductions
+  asm volatile("" ::: "memory");
+  wr = group_sum(wr, 64);
+  wi = group_sum(wi, 64);
+  if constexpr (NWC > 1) {
+    if (lane == 0) {
+      swr[cl][h] = wr;
+      swi[cl][h] = wi;
+    }
+    __syncthreads();
+    wr = wi = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWC; ++q) {
+      wr += swr[cl][q];
+      wi += swi[cl][q];
+    }
+  }
+  const zc tw = zmul(make_double2(tu.x, -tu.y), make_double2(wr, wi));
+  double ss2 = 0.0;
+  zc x = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int o = 64 * (NWC * r + h) + lane;
+    const zc q = zmul(tw, sv[o]);
+    const zc y = make_double2(cr[r].x - q.x, cr[r].y - q.y);
+    if (have && o < len) {
+      c[o] = y;
+      if (o == 0) x = y;
+      else ss2 += zabs2(y);
+    }
+  }
+  ss2 = group_sum(ss2, 64);
+  if (lane == 0) {
+    sss[cl][h] = ss2;
+    if (h == 0) sx[cl] = x;  // row s is offset 0: lane 0 of the column's first wave
+  }
+  __syncthreads();
+  // norm downdate (LAWN 176) or recompute, and the column's candidate entry
+  if (h == 0 && lane == 0) {
+    double v = -1.0;
+    int fl = 0;
+    if (have) {
+      double a = vn1[col];
+      if (s + 1 < lastrk && a != 0.0) {
+        double temp = hypot(sx[cl].x, sx[cl].y) / a;
+        temp = (1.0 + temp) * (1.0 - temp);
+        temp = temp < 0.0 ? 0.0 : temp;
+        const double q = a / vn2[col];
+        const double temp2 = temp * q * q;
+        if (temp2 <= tol3z) {
+          double ssum = 0.0;
+#pragma unroll
+          for (int q2 = 0; q2 < NWC; ++q2) ssum += sss[cl][q2];
+          a = sqrt(ssum);
+          vn1[col] = vn2[col] = a;
+          fl = 1;
+        } else {
+          a *= sqrt(temp);
+          vn1[col] = a;
+        }
+      }
+      v = a >= 0.0 ? a : -1.0;  // (a NaN norm takes no part)
+    }
+    cval[cl] = v;
+    cpos[cl] = have ? p : INT_MAX;
+    ccol[cl] = col;
+    cflag[cl] = fl;
+  }
+  __syncthreads();
+  fused_finish<CPB>(s, n, kmax, nb, stop_after, dss, cval, cpos, ccol, cflag, at, diag, st,
+                    part, err);
+}
+
+// after the last fused complex step: the last pivot column's R(s,s) and v, and k
+__global__ __launch_bounds__(256) void zq_ffinish(int m, zc* B, i64 ldb, const int* at,
+                                                  const zc* vbuf, const FState* st,
+                                                  i64* k_out) {
+  const int sp = st->sprev;
+  if (sp >= 0) {
+    const zc* vp = vbuf + (size_t)(sp & 1) * m;
+    zc* d = B + (i64)at[sp] * ldb;
+    for (int t = sp + (int)threadIdx.x; t < m; t += 256) d[t] = vp[t];
+  }
+  if (threadIdx.x == 0) *k_out = st->k;
+}
+
+// the fused complex launch of step s: rows s..m-1 (at most 4096) pick the form
+void zlaunch_fused(hipStream_t s0, int s, int m, int n, int kmax, int nb, int stop_after,
+                   int lastrk, zc* B, i64 ldb, int* at, double* vn1, double* vn2, zc* tau,
+                   double* diag, zc* vbuf, FState* st, u64x2* part, int* err) {
+  const int rows = m - s, tr = n - s - 1;
+  auto grid = [&](int cpb) { return dim3((unsigned)std::max(1, (tr + cpb - 1) / cpb)); };
+#define ZF_ARGS s, m, n, kmax, nb, stop_after, lastrk, B, ldb, at, vn1, vn2, tau, diag, vbuf, \
+                st, part, err
+  if (rows <= 64 * 8)
+    hipLaunchKernelGGL((zq_fused<1, 8>), grid(4), dim3(256), 0, s0, ZF_ARGS);
+  else if (rows <= 128 * 8)
+    hipLaunchKernelGGL((zq_fused<2, 8>), grid(2), dim3(256), 0, s0, ZF_ARGS);
+  else if (rows <= 256 * 8)
+    hipLaunchKernelGGL((zq_fused<4, 8>), grid(1), dim3(256), 0, s0, ZF_ARGS);
+  else
+    hipLaunchKernelGGL((zq_fused<4, 16>), grid(1), dim3(256), 0, s0, ZF_ARGS);
+#undef ZF_ARGS
+}
+
 }  // namespace
 
 // Complex rank-adaptive pivoted QR (geqp3_adap!, src/pqr.jl:348-418, with zlaqps' steps) on
@@ -1022,19 +1254,63 @@ int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, in
     return LRA_OK;
   }
   double* w = nullptr;
-  // vn1, vn2, diag (kmax), v (m complex), state
-  const size_t nd = 2 * (size_t)n + (size_t)kmax + 2 * (size_t)m + 16;
+  // vn1, vn2, diag (kmax), v (2 m complex), state, position map (n ints)
+  const size_t nd = 2 * (size_t)n + (size_t)kmax + 1 + 4 * (size_t)m + 16 + ((size_t)n + 1) / 2;
   LRA_CHECK(lra_wsT(c, WS_QG, nd, &w));
   double* vn1 = w;
   double* vn2 = vn1 + n;
   double* diag = vn2 + n;
   zc* v = reinterpret_cast<zc*>(diag + kmax + (kmax & 1));
-  GState* st = reinterpret_cast<GState*>(v + m);
+  GState* st = reinterpret_cast<GState*>(v + 2 * (size_t)m);
+  int* at = reinterpret_cast<int*>(reinterpret_cast<double*>(st) + 16);
   const int lastrk = std::min(m, n);
   hipStream_t s0 = c->stream;
   zc* zt = reinterpret_cast<zc*>(tau);
+
+  // one launch per step (rows up to 4096, with the uncached exchange area for the partials);
+  // LRA_QG_FUSED=0 keeps the two launches per step below
+  const char* fe = getenv("LRA_QG_FUSED");
+  if (!(fe && fe[0] == '0') && m <= 4096 && n < (int)F_NONE) {
+    void* area = nullptr;
+    bool uc = false;
+    const size_t bytes = 256 + 16 * (size_t)n;
+    LRA_CHECK(lra_coop_area(c, bytes, &area, &uc));
+    if (uc) {
+      double* S = nullptr;  // the columns in storage order, for the final permutation
+      LRA_CHECK(lra_wsT(c, WS_QGT, 2 * (size_t)m * n, &S));
+      u64x2* part = reinterpret_cast<u64x2*>((char*)area + 256);
+      FState* fs = reinterpret_cast<FState*>(st);
+      LRA_CHECK(lra_launch_zero_u32(s0, (unsigned*)area, (i64)(bytes / 4)));
+      hipLaunchKernelGGL(zq_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, B,
+                         ldb, vn1, vn2, jpvt, at);
+      hipLaunchKernelGGL(qg_fstart, dim3(1), dim3(GT), 0, s0, n, kmax, nb, vn1, atol, rtol, fs,
+                         at);
+      LRA_HIP(hipGetLastError());
+      for (int s = 0; s < kmax; ++s) {
+        zlaunch_fused(s0, s, m, n, kmax, nb, stop_after, lastrk, B, ldb, at, vn1, vn2, zt, diag,
+                      v, fs, part, c->d_err);
+        LRA_HIP(hipGetLastError());
+        if ((s & 63) == 63) {  // stop enqueueing once the device has stopped
+          int done = 0;
+          LRA_HIP(hipMemcpyAsync(c->host_pinned, &fs->done, sizeof(int), hipMemcpyDeviceToHost,
+                                 s0));
+          LRA_HIP(hipStreamSynchronize(s0));
+          std::memcpy(&done, c->host_pinned, sizeof(int));
+          if (done) break;
+        }
+      }
+      hipLaunchKernelGGL(zq_ffinish, dim3(1), dim3(256), 0, s0, m, B, ldb, at, v, fs, k_out);
+      LRA_HIP(hipMemcpy2DAsync(S, (size_t)m * sizeof(zc), B, (size_t)ldb * sizeof(zc),
+                               (size_t)m * sizeof(zc), (size_t)n, hipMemcpyDeviceToDevice, s0));
+      hipLaunchKernelGGL(qg_gather, dim3((unsigned)n), dim3(256), 0, s0, 2 * m, S, 2 * (i64)m,
+                         Bz, 2 * ldb, at, jpvt);
+      LRA_HIP(hipGetLastError());
+      return LRA_OK;
+    }
+  }
+
   hipLaunchKernelGGL(zq_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, B, ldb,
-                     vn1, vn2, jpvt);
+                     vn1, vn2, jpvt, nullptr);
   hipLaunchKernelGGL(qg_init, dim3(1), dim3(GT), 0, s0, n, vn1, atol, rtol, kmax, nb, st);
   LRA_HIP(hipGetLastError());
   for (int s = 0; s <= kmax; ++s) {

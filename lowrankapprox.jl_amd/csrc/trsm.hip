@@ -200,6 +200,43 @@ int lra_launch_id_T(hipStream_t st, i64 k, i64 n, const double* R, i64 ldr, doub
 // blocks of 1184 from the bottom.  Block j's right-hand side R12(j) - R(j, below) T(below) is
 // formed in T by a copy and one GEMM, then solved in place on MFMA.  The slab kernel alone
 // would hold 2 columns of k doubles per workgroup and run k barrier pairs per column.
+// X = R \ X in place for a k x k upper-triangular R (ldr): T holds the nr right-hand sides
+// on entry and the solution on exit.  Row blocks of KB from the bottom, as lra_id_T: each
+// block's right-hand side loses R(block, below) * X(below) in one GEMM, then the MFMA kernel
+// solves it in place.  (At most 64 blocks: LRA_E_UNSUPPORTED beyond.)
+int lra_trsm_upper_inplace(lra_ctx* c, i64 k, i64 nr, const double* R, i64 ldr, double* T,
+                           i64 ldt) {
+  if (k <= 0 || nr <= 0) return LRA_OK;
+  constexpr i64 KB = 1184;
+  const i64 nblk = (k + KB - 1) / KB;
+  if (nblk > 64) {
+    lra_set_error("trsm: k=%lld beyond %lld row blocks", (long long)k, (long long)(64 * KB));
+    return LRA_E_UNSUPPORTED;
+  }
+  for (i64 j = nblk - 1; j >= 0; --j) {
+    const i64 r0 = j * KB, r1 = std::min(k, r0 + KB), kb = r1 - r0;
+    if (r1 < k) {
+      GemmArgs g{};
+      g.M = kb; g.N = nr; g.K = k - r1;
+      g.alpha = -1.0; g.beta = 1.0;
+      g.A = R + r0 + r1 * ldr; g.lda = ldr; g.transA = 0;
+      g.B = T + r1; g.ldb = ldt; g.transB = 0;
+      g.C = T + r0; g.ldc = ldt;
+      g.bgather = nullptr;
+      g.prec = LRA_PREC_F64;
+      LRA_CHECK(lra_gemm(c, g));
+    }
+    hipError_t e = hipSuccess;
+    const bool ok = launch_T_mfma(c->stream, kb, nr, R + r0 + r0 * ldr, ldr, T + r0, ldt, T + r0,
+                                  ldt, &e);
+    if (!ok || e != hipSuccess) {
+      lra_set_error("trsm block launch: %s", ok ? hipGetErrorString(e) : "slab too large");
+      return LRA_E_HIP;
+    }
+  }
+  return LRA_OK;
+}
+
 int lra_id_T(lra_ctx* c, i64 k, i64 n, const double* R, i64 ldr, double* T, i64 ldt) {
   const i64 nr = n - k;
   if (k <= 0 || nr <= 0) return LRA_OK;

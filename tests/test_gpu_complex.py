@@ -79,9 +79,12 @@ def test_zsketch_vs_closed_form(gpu, orc, kind, trans):
     assert np.abs(host(B) - Bo).max() <= 1e-12 * np.sqrt(P) * np.abs(A).max()
 
 
-def test_zgeqp3_vs_zlaqps_loop(gpu, orc):
-    """The complex rank-adaptive QRCP against the reference's loop over OpenBLAS zlaqps."""
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_zgeqp3_vs_zlaqps_loop(gpu, orc, monkeypatch, fused):
+    """The complex rank-adaptive QRCP against the reference's loop over OpenBLAS zlaqps, in
+    both forms of the HBM steps (one fused launch per step; two with LRA_QG_FUSED=0)."""
     import torch
+    monkeypatch.setenv("LRA_QG_FUSED", fused)
     B = zdecay(120, 400, 60, 0.8, 1)
     Bd = gpu.to_device_z(B).clone()
     n = B.shape[1]
@@ -98,6 +101,74 @@ def test_zgeqp3_vs_zlaqps_loop(gpu, orc):
     assert i >= 20
     R, Ro = np.triu(host(Bd)[:ko]), np.triu(Bf[:ko])
     assert np.allclose(np.abs(np.diag(R))[:i], np.abs(np.diag(Ro))[:i], rtol=1e-10)
+
+
+def zgeqp3(gpu, B, rank=-1, rtol=1e-12):
+    import torch
+    m, n = B.shape
+    Bd = gpu.to_device_z(B).clone()
+    jp = torch.empty(n, dtype=torch.int64, device="cuda")
+    tau = torch.empty(min(m, n), dtype=torch.complex128, device="cuda")
+    k = C.c_int64(0)
+    gpu.check(gpu._lib.load().lra_zgeqp3_adap(gpu.context().h, m, n, Bd.data_ptr(), m, rank,
+                                              0.0, rtol, 32, jp.data_ptr(), tau.data_ptr(),
+                                              C.byref(k)), "lra_zgeqp3_adap")
+    torch.cuda.synchronize()
+    return host(jp), host(tau), int(k.value), host(Bd)
+
+
+@pytest.mark.parametrize("m,n,rank", [(120, 400, -1), (700, 1500, -1), (1500, 900, 500),
+                                      (3000, 1200, -1)])
+def test_zgeqp3_fused_matches_two_launch_form(gpu, monkeypatch, m, n, rank):
+    """The fused complex HBM steps against the two-launch form on the same graded matrix (no
+    near ties): the same rank and pivots, R to rounding, each reflector and tau to rounding
+    amplified by the condition of its step, |R(0,0)| / |R(j,j)|.  The shapes take each form
+    of the fused kernel (one, two and four waves per column, 8 or 16 rows per lane); rank 500
+    ends inside a block."""
+    rng = np.random.default_rng(m + n)
+    r = min(m, n)
+    U = rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))
+    V = rng.standard_normal((r, n)) + 1j * rng.standard_normal((r, n))
+    B = np.asfortranarray((U * 0.97 ** np.arange(r)) @ V)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("LRA_QG_FUSED", fused)
+        print(f"complex fused vs two-launch m={m} n={n} rank={rank} fused={fused}", flush=True)
+        out[fused] = zgeqp3(gpu, B, rank)
+    (p1, t1, k1, B1), (p0, t0, k0, B0) = out["1"], out["0"]
+    assert k1 == k0
+    assert np.array_equal(p1, p0)
+    kk = min(k1, m, n)
+    assert np.abs(np.triu(B1[:kk]) - np.triu(B0[:kk])).max() <= 1e-11 * np.abs(B0[:kk]).max()
+    cond = np.abs(B0[0, 0]) / np.abs(np.diag(B0[:kk, :kk]))
+    dv = np.abs(np.tril(B1[:, :kk], -1) - np.tril(B0[:, :kk], -1)).max(axis=0)
+    assert np.all(dv <= 1e-12 * cond), int(np.argmax(dv / cond))
+    assert np.all(np.abs(t1[:kk] - t0[:kk]) <= 1e-12 * cond)
+
+
+@pytest.mark.parametrize("k,nr,cdiag", [(40, 300, False), (437, 3659, False), (437, 500, True),
+                                        (700, 900, False), (1300, 700, True)])
+def test_zid_T_vs_scipy(gpu, k, nr, cdiag):
+    """lra_zid_T (T = R11 \\ R12, complex) against scipy's triangular solve: a well-conditioned
+    upper-triangular R11 with a real diagonal (as the pivoted QR leaves it) or a complex one
+    (the solve then scales each row by a unit factor first); k = 700 and 1300 take the real
+    2k-row solve in row blocks."""
+    import scipy.linalg as sl
+    import torch
+    rng = np.random.default_rng(k + nr)
+    R11 = np.triu(rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))) / np.sqrt(k)
+    d = 2.0 + rng.random(k)
+    R11[np.diag_indices(k)] = d * np.exp(1j * rng.uniform(0, 2 * np.pi, k)) if cdiag else d
+    R12 = rng.standard_normal((k, nr)) + 1j * rng.standard_normal((k, nr))
+    R = np.asfortranarray(np.hstack([R11, R12]))
+    Rd = gpu.to_device_z(R)
+    T = torch.empty((nr, k), dtype=torch.complex128, device="cuda").t()  # column-major k x nr
+    print(f"zid_T k={k} nr={nr} complex diagonal={cdiag}", flush=True)
+    gpu.check(gpu._lib.load().lra_zid_T(gpu.context().h, k, k + nr, Rd.data_ptr(), k,
+                                        T.data_ptr(), k), "lra_zid_T")
+    torch.cuda.synchronize()
+    Tref = sl.solve_triangular(R11, R12)
+    assert np.abs(host(T) - Tref).max() <= 1e-12 * np.abs(Tref).max()
 
 
 @pytest.mark.parametrize("kind", KINDS + ["none"])
