@@ -442,12 +442,16 @@ __global__ void qg_finish(const GState* st, i64* k_out, int* early_out) {
 //      copies are bitwise the same and no launch of its own is needed for it;
 //   2. applies H(s) and downdates the norms (the arithmetic of qg_apply / qg_applyw);
 //   3. publishes its best candidate (largest norm, then lowest position) as a tagged
-//      16-byte partial in the uncached area.
-// The last workgroup of the grid then reads every partial (re-reading one until its tag
-// arrives: only this workgroup waits, on workgroups that wait for nothing), runs the block
+//      16-byte partial in the uncached area, then takes a ticket from a two-level tree of
+//      counters (64 sub-counters on their own lines, then the top one: thousands of atomics
+//      on one uncached word were serialized).
+// The workgroup that completes the top counter is the last of the grid: it reads every
+// partial (a store still in flight is re-read until its tag arrives), runs the block
 // bookkeeping and stop rule of step s (pick_body's, for step s+1), and swaps the next pivot
-// into position s+1 of `at`.  (A ticket taken by every workgroup, the last one reducing, cost
-// more: thousands of atomics on one uncached word are serialized.)  Workgroup 0 stores v(s), tau(s) and R(s,s); the pivot column itself is
+// into position s+1 of `at`.  No workgroup ever waits for one that has not finished its
+// columns, so the launch drains beside any other kernel, grid-synchronising ones included.
+// (A designated last workgroup that polled the partials instead could wait for workgroups
+// that a co-resident cooperative grid kept from being dispatched.)  Workgroup 0 stores v(s), tau(s) and R(s,s); the pivot column itself is
 // rewritten one launch later, when no workgroup reads it any more.
 
 struct FState {
@@ -456,6 +460,11 @@ struct FState {
 };
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// The exchange area of the fused steps: kSubCtr + 1 ticket counters kCtrStride words apart
+// (the top one first), then one 16-byte partial per block from byte kPartOff.
+constexpr unsigned kSubCtr = 64, kCtrStride = 16;
+constexpr size_t kPartOff = 8192;
 
 // partial tag: step tag (15 bits, never 0) | recompute flag << 15 | position << 16 |
 // storage column << 40; positions and columns below 2^24 - 1
@@ -525,10 +534,13 @@ __global__ __launch_bounds__(GT) void qg_fstart(int n, int kmax, int nb, const d
 template <int CPB>
 LRA_DEV void fused_finish(int s, int n, int kmax, int nb, int stop_after, double rdiag,
                           const double* cval, const int* cpos, const int* ccol, const int* cflag,
-                          int* at, const double* diag, FState* st, u64x2* part, int* err) {
+                          int* at, const double* diag, FState* st, u64x2* part,
+                          unsigned* ctr, int* err) {
   __shared__ double tv[4];
   __shared__ int tp[4], tc[4], tf[4], tok[4];
+  __shared__ int s_last;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const unsigned nsub = gridDim.x < kSubCtr ? gridDim.x : kSubCtr;
   if (tid == 0) {
     double bv = cval[0];
     int bp = cpos[0], bc = ccol[0], bf = cflag[0];
@@ -541,11 +553,26 @@ LRA_DEV void fused_finish(int s, int n, int kmax, int nb, int stop_after, double
     e.x = (uint64_t)__double_as_longlong(bv);
     e.y = bv < 0.0 ? fpack(s, bf, -1, -1) : fpack(s, bf, bp, bc);
     part[blockIdx.x] = e;  // uncached: a plain store; its tag validates it for the reader
+    asm volatile("" ::: "memory");
+    // a two-level ticket: sub-counter blockIdx % nsub, then the top counter; the block that
+    // completes the top one is the last of the grid
+    const unsigned sub = blockIdx.x % nsub;
+    const unsigned cnt = gridDim.x / nsub + (sub < gridDim.x % nsub ? 1u : 0u);
+    int last = 0;
+    if (__hip_atomic_fetch_add(ctr + kCtrStride * (1 + sub), 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == cnt - 1)
+      last = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             nsub - 1;
+    s_last = last;
   }
-  if (blockIdx.x != gridDim.x - 1) return;
+  __syncthreads();
+  if (!s_last) return;
+  // every block has taken its tickets: the counters start again from zero for the next step
+  if (tid <= (int)nsub)
+    __hip_atomic_store(ctr + kCtrStride * tid, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
   // ---- the last block: every partial (loaded 8 at a time per thread, then validated; a
-  // partial not written yet is re-read until its tag arrives -- no other block waits) ----
+  // partial whose store is still in flight is re-read until its tag arrives) ----
   const int G = (int)gridDim.x;
   const uint64_t want = fstep_tag(s);
   double bv = -1.0;
@@ -657,7 +684,7 @@ __global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, i
                                                 double* __restrict__ B, i64 ldb, int* at,
                                                 double* vn1, double* vn2, double* tau,
                                                 double* diag, double* vbuf, FState* st,
-                                                u64x2* part, int* err) {
+                                                u64x2* part, unsigned* ctr, int* err) {
   constexpr int CPB = 4 / NWC;  // columns per block
   constexpr int ROWS = 64 * NWC * RPL;
   __shared__ double sv[ROWS];  // the pivot column, then v(s .. s + ROWS - 1), zero past m
@@ -808,7 +835,7 @@ __global__ __launch_bounds__(256) void qg_fused(int s, int m, int n, int kmax, i
   }
   __syncthreads();
   fused_finish<CPB>(s, n, kmax, nb, stop_after, beta, cval, cpos, ccol, cflag, at, diag, st,
-                    part, err);
+                    part, ctr, err);
 }
 
 // after the last fused step: the last pivot column's R(s,s) and v, k and the early flag
@@ -1031,7 +1058,7 @@ __global__ __launch_bounds__(256) void zq_fused(int s, int m, int n, int kmax, i
                                                 zc* __restrict__ B, i64 ldb, int* at,
                                                 double* vn1, double* vn2, zc* tau,
                                                 double* diag, zc* vbuf, FState* st,
-                                                u64x2* part, int* err) {
+                                                u64x2* part, unsigned* ctr, int* err) {
   constexpr int CPB = 4 / NWC;  // columns per block
   constexpr int ROWS = 64 * NWC * RPL;
   __shared__ zc sv[ROWS];  // the pivot column, then v(s .. s + ROWS - 1), zero past m
@@ -1203,7 +1230,7 @@ __global__ __launch_bounds__(256) void zq_fused(int s, int m, int n, int kmax, i
   }
   __syncthreads();
   fused_finish<CPB>(s, n, kmax, nb, stop_after, dss, cval, cpos, ccol, cflag, at, diag, st,
-                    part, err);
+                    part, ctr, err);
 }
 
 // after the last fused complex step: the last pivot column's R(s,s) and v, and k
@@ -1222,11 +1249,11 @@ __global__ __launch_bounds__(256) void zq_ffinish(int m, zc* B, i64 ldb, const i
 // the fused complex launch of step s: rows s..m-1 (at most 4096) pick the form
 void zlaunch_fused(hipStream_t s0, int s, int m, int n, int kmax, int nb, int stop_after,
                    int lastrk, zc* B, i64 ldb, int* at, double* vn1, double* vn2, zc* tau,
-                   double* diag, zc* vbuf, FState* st, u64x2* part, int* err) {
+                   double* diag, zc* vbuf, FState* st, u64x2* part, unsigned* ctr, int* err) {
   const int rows = m - s, tr = n - s - 1;
   auto grid = [&](int cpb) { return dim3((unsigned)std::max(1, (tr + cpb - 1) / cpb)); };
 #define ZF_ARGS s, m, n, kmax, nb, stop_after, lastrk, B, ldb, at, vn1, vn2, tau, diag, vbuf, \
-                st, part, err
+                st, part, ctr, err
   if (rows <= 64 * 8)
     hipLaunchKernelGGL((zq_fused<1, 8>), grid(4), dim3(256), 0, s0, ZF_ARGS);
   else if (rows <= 128 * 8)
@@ -1273,12 +1300,13 @@ int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, in
   if (!(fe && fe[0] == '0') && m <= 4096 && n < (int)F_NONE) {
     void* area = nullptr;
     bool uc = false;
-    const size_t bytes = 256 + 16 * (size_t)n;
+    const size_t bytes = kPartOff + 16 * (size_t)n;
     LRA_CHECK(lra_coop_area(c, bytes, &area, &uc));
     if (uc) {
       double* S = nullptr;  // the columns in storage order, for the final permutation
       LRA_CHECK(lra_wsT(c, WS_QGT, 2 * (size_t)m * n, &S));
-      u64x2* part = reinterpret_cast<u64x2*>((char*)area + 256);
+      unsigned* ctr = reinterpret_cast<unsigned*>(area);
+      u64x2* part = reinterpret_cast<u64x2*>((char*)area + kPartOff);
       FState* fs = reinterpret_cast<FState*>(st);
       LRA_CHECK(lra_launch_zero_u32(s0, (unsigned*)area, (i64)(bytes / 4)));
       hipLaunchKernelGGL(zq_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, B,
@@ -1288,7 +1316,7 @@ int lra_zqrcp_global(lra_ctx* c, double* Bz, i64 ldb, int m, int n, int kmax, in
       LRA_HIP(hipGetLastError());
       for (int s = 0; s < kmax; ++s) {
         zlaunch_fused(s0, s, m, n, kmax, nb, stop_after, lastrk, B, ldb, at, vn1, vn2, zt, diag,
-                      v, fs, part, c->d_err);
+                      v, fs, part, ctr, c->d_err);
         LRA_HIP(hipGetLastError());
         if ((s & 63) == 63) {  // stop enqueueing once the device has stopped
           int done = 0;
@@ -1342,11 +1370,12 @@ namespace {
 
 // the fused launch of step s: rows s..m-1 (at most 8192) pick the form, as for the applies
 void launch_fused(hipStream_t s0, int s, const QrcpCall& q, int lastrk, int* at, double* vn1,
-                  double* vn2, double* diag, double* vbuf, FState* st, u64x2* part, int* err) {
+                  double* vn2, double* diag, double* vbuf, FState* st, u64x2* part,
+                  unsigned* ctr, int* err) {
   const int m = q.m, n = q.n, rows = m - s, tr = n - s - 1;
   auto grid = [&](int cpb) { return dim3((unsigned)std::max(1, (tr + cpb - 1) / cpb)); };
 #define QF_ARGS s, m, n, q.kmax, q.nb, q.stop_after, lastrk, q.A, q.lda, at, vn1, vn2, q.tau, \
-                diag, vbuf, st, part, err
+                diag, vbuf, st, part, ctr, err
   if (rows <= 64 * 8)
     hipLaunchKernelGGL((qg_fused<1, 8>), grid(4), dim3(256), 0, s0, QF_ARGS);
   else if (rows <= 64 * 16)
@@ -1396,12 +1425,13 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
   if (!(fe && fe[0] == '0') && m <= 8192 && n < (int)F_NONE && q.kmax > 0) {
     void* area = nullptr;
     bool uc = false;
-    const size_t bytes = 256 + 16 * (size_t)n;
+    const size_t bytes = kPartOff + 16 * (size_t)n;
     LRA_CHECK(lra_coop_area(c, bytes, &area, &uc));
     if (uc) {
       double* S = nullptr;  // the columns in storage order, for the final permutation
       LRA_CHECK(lra_wsT(c, WS_QGT, (size_t)m * n, &S));
-      u64x2* part = reinterpret_cast<u64x2*>((char*)area + 256);
+      unsigned* ctr = reinterpret_cast<unsigned*>(area);
+      u64x2* part = reinterpret_cast<u64x2*>((char*)area + kPartOff);
       FState* fs = reinterpret_cast<FState*>(st);
       LRA_CHECK(lra_launch_zero_u32(s0, (unsigned*)area, (i64)(bytes / 4)));
       hipLaunchKernelGGL(qg_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s0, m, n, q.A,
@@ -1410,7 +1440,7 @@ int lra_qrcp_global(lra_ctx* c, const QrcpCall& q) {
                          q.rtol, fs, at);
       LRA_HIP(hipGetLastError());
       for (int s = 0; s < q.kmax; ++s) {
-        launch_fused(s0, s, q, lastrk, at, vn1, vn2, diag, v, fs, part, c->d_err);
+        launch_fused(s0, s, q, lastrk, at, vn1, vn2, diag, v, fs, part, ctr, c->d_err);
         LRA_HIP(hipGetLastError());
         if ((s & 63) == 63) {  // stop enqueueing once the device has stopped
           int done = 0;

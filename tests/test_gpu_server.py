@@ -134,3 +134,31 @@ def test_server_cycles_reuse_exchange_areas(gpu):
         print(f"cycle {cycle}: free {free[-1] >> 20} MiB", flush=True)
     # without reuse, five more cycles of four workers would strand 5 x 4 x 64 MiB
     assert free[0] - min(free[1:]) <= 128 << 20, free
+
+
+def test_server_hbm_qr_beside_cooperative_grids(gpu):
+    """Requests whose rank takes the adaptive loop to an order-1032 sketch on 6000 columns
+    (beyond the register-resident pivoted QR, so it runs the HBM steps, one launch per step)
+    served together with requests on the register-resident cooperative grids: every launch of
+    the HBM steps drains beside a grid that needs the whole device (no block of it waits for
+    a block that has not finished its columns), and each result equals the request alone."""
+    slow = [gpu.to_device(decaying(1200, 6000, 0.03, 200 + i)) for i in range(2)]
+    fast = [gpu.to_device(decaying(2000, 3000, 0.1, 300 + i)) for i in range(6)]
+    mats = [slow[0]] + fast[:3] + [slow[1]] + fast[3:]
+    srv = gpu.Server(hbm_budget=32 << 30, workers=4)
+    try:
+        hs = [srv.submit(A, "idfact", rtol=1e-10, seed=3) for A in mats]
+        print(f"submitted {len(hs)} requests (2 on the HBM pivoted QR)", flush=True)
+        for h in hs:
+            assert srv.wait(h, 120.0) == "done"
+        for i, (h, A) in enumerate(zip(hs, mats)):
+            got = srv.result(h)
+            alone = gpu.idfact(A, rtol=1e-10, seed=3)
+            assert got.k == alone.k
+            if i in (0, 4):  # past order 520: the order-1032 round ran on the HBM steps
+                assert got.k > 520
+            assert np.array_equal(host(got.sk), host(alone.sk))
+            assert np.array_equal(host(got.T), host(alone.T))
+            srv.release(h)
+    finally:
+        srv.close()
