@@ -112,17 +112,20 @@ __global__ void zsprn_s_kernel(uint64_t seed, uint64_t stream, i64 P, zc* s) {
   }
 }
 
-// W (2l x 2P, real) from Phi (l x P, complex), the real form of left multiplication by Phi
-__global__ void zexpand_kernel(i64 l, i64 P, const zc* Phi, double* W) {
-  const i64 lw = 2 * l;
+// Wt (2P x 2l, real, ld 2P) = W' for W the real form of left multiplication by Phi (l x P,
+// complex: each entry x + iy is the 2 x 2 block [x -y; y x] of W), stored transposed so the
+// sketch product reads its k index contiguously (the k-contiguous GEMM kernels); consecutive
+// threads write consecutive rows of Wt.
+__global__ void zexpand_t_kernel(i64 l, i64 P, const zc* Phi, double* Wt) {
+  const i64 kw = 2 * P;
   for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < l * P;
        e += (i64)gridDim.x * blockDim.x) {
-    const i64 p = e % l, i = e / l;
-    const zc z = Phi[e];
-    double* c0 = W + (2 * i) * lw + 2 * p;
-    double* c1 = W + (2 * i + 1) * lw + 2 * p;
-    c0[0] = z.x;  c0[1] = z.y;
-    c1[0] = -z.y; c1[1] = z.x;
+    const i64 i = e % P, p = e / P;
+    const zc z = Phi[p + i * l];
+    double* c0 = Wt + (2 * p) * kw + 2 * i;      // W(2p, 2i .. 2i+1)
+    double* c1 = Wt + (2 * p + 1) * kw + 2 * i;  // W(2p+1, 2i .. 2i+1)
+    c0[0] = z.x;  c0[1] = -z.y;
+    c1[0] = z.y;  c1[1] = z.x;
   }
 }
 
@@ -305,11 +308,12 @@ int zsketch_apply(lra_ctx* c, i64 l, i64 P, i64 N, const zc* Phi, const zc* X, i
   }
   double* W = nullptr;
   LRA_CHECK(lra_wsT(c, WS_TMP2, (size_t)4 * l * P, &W));
-  hipLaunchKernelGGL(zexpand_kernel, dim3(grid_for(l * P)), dim3(256), 0, c->stream, l, P, Phi, W);
+  hipLaunchKernelGGL(zexpand_t_kernel, dim3(grid_for(l * P)), dim3(256), 0, c->stream, l, P, Phi,
+                     W);
   LRA_LAUNCH_CHECK();
   GemmArgs g;
   g.M = 2 * l; g.N = N; g.K = 2 * P; g.alpha = 1.0; g.beta = 0.0;
-  g.A = W; g.lda = 2 * l; g.transA = 0;
+  g.A = W; g.lda = 2 * P; g.transA = 1;
   g.B = reinterpret_cast<const double*>(X); g.ldb = 2 * ldx; g.transB = 0;
   g.C = reinterpret_cast<double*>(B); g.ldc = 2 * l; g.bgather = nullptr; g.prec = LRA_PREC_F64;
   return lra_gemm(c, g);
