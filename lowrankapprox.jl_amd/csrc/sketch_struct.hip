@@ -450,6 +450,32 @@ __global__ void subset_kernel(i64 n, const double* A, i64 lda, int ell, const i6
   }
 }
 
+// One block per column, the column first staged in LDS with coalesced loads (the output rows
+// gather it in permutation order, which read from HBM was one scattered 8-byte load per
+// term); then as sprn_kernel: thread i sums output row i over its group of the permutation,
+// the same products in the same order.  m <= kSprnLdsRows.
+constexpr i64 kSprnLdsRows = 16384;  // 128 KB of LDS
+__global__ __launch_bounds__(256) void sprn_lds_kernel(i64 m, const double* __restrict__ A,
+                                                       i64 lda, int ell,
+                                                       const i64* __restrict__ r,
+                                                       const double* __restrict__ s,
+                                                       double* __restrict__ B, i64 ldb) {
+  extern __shared__ double xs[];
+  const i64 col = blockIdx.x;
+  const double* x = A + col * lda;
+  for (i64 t = threadIdx.x; t < m; t += 256) xs[t] = x[t];
+  __syncthreads();
+  const i64 qq = m / ell, rr = m % ell;
+  for (int i = threadIdx.x; i < ell; i += 256) {
+    const i64 t = i;
+    const i64 start = t <= rr ? t * (qq + 1) : rr * (qq + 1) + (t - rr) * qq;
+    const i64 p = (i + 1) <= rr ? qq + 1 : qq;
+    double acc = 0.0;
+    for (i64 l = 0; l < p; ++l) acc += s[start + l] * xs[r[start + l] - 1];
+    B[i + col * ldb] = acc;
+  }
+}
+
 // one block per column; thread i sums output row i over its group of the permutation
 __global__ void sprn_kernel(i64 m, const double* __restrict__ A, i64 lda, int ell,
                             const i64* __restrict__ r, const double* __restrict__ s,
@@ -596,8 +622,17 @@ int lra_sketch_sprn_impl(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i
     s = ss;
   }
   if (Q > 2147483647ll) { lra_set_error("sprn: too many columns"); return LRA_E_UNSUPPORTED; }
-  hipLaunchKernelGGL(sprn_kernel, dim3((unsigned)Q), dim3(256), 0, c->stream, P, X, ldx,
-                     (int)ell, r, s, B, ldb);
+  if (P <= kSprnLdsRows) {
+    static std::atomic<uint64_t> lds_set{0};
+    lra_lds_limit_once(lds_set, (const void*)sprn_lds_kernel,
+                       (int)(kSprnLdsRows * sizeof(double)));
+    hipLaunchKernelGGL(sprn_lds_kernel, dim3((unsigned)Q), dim3(256),
+                       (size_t)std::max<i64>(P, 1) * sizeof(double), c->stream, P, X, ldx,
+                       (int)ell, r, s, B, ldb);
+  } else {
+    hipLaunchKernelGGL(sprn_kernel, dim3((unsigned)Q), dim3(256), 0, c->stream, P, X, ldx,
+                       (int)ell, r, s, B, ldb);
+  }
   LRA_LAUNCH_CHECK("sprn");
   return LRA_OK;
 }

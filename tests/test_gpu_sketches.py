@@ -77,12 +77,15 @@ def test_srft_config3_shape_closed_form(gpu):
             assert np.allclose(B[i + 1], F[f].imag, atol=1e-10 * np.sqrt(m))
 
 
-def test_subset_and_sprn_injected(gpu, orc):
+@pytest.mark.parametrize("m,n,ell", [(700, 90, 37), (16384, 40, 300), (20000, 24, 61)])
+def test_subset_and_sprn_injected(gpu, orc, m, n, ell):
+    """Subset and sparse-Gaussian sketches with injected draws against the oracle.  The sparse
+    sketch stages each column in LDS up to 16384 rows (the first two shapes) and gathers from
+    HBM beyond (the third)."""
     import torch
     L = gpu._lib.load()
     ctx = gpu.context()
     rng = np.random.default_rng(5)
-    m, n, ell = 700, 90, 37
     A = np.asfortranarray(rng.standard_normal((m, n)))
     Ad = gpu.to_device(A)
     r = rng.integers(1, m + 1, size=ell)
