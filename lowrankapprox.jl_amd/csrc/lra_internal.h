@@ -120,7 +120,7 @@ enum WsSlot {
   WS_F0, WS_F1, WS_F2, WS_F3, WS_F4, WS_F5, WS_F6, WS_F7, WS_F8, WS_F9, WS_J0, WS_J1,
   WS_PI0, WS_PI1, WS_PI2, WS_QG, WS_PR0, WS_PR1, WS_PR2, WS_PR3, WS_PR4, WS_PR5, WS_PR6,
   WS_PR7, WS_PR8, WS_PR9, WS_PR10, WS_PR11, WS_PR12, WS_CU0, WS_CUR1, WS_OP2, WS_OP3,
-  WS_SKETCH3, WS_JPVT2, WS_TAU2, WS_SKETCHP, WS_QGT, WS_ZT, WS_COUNT
+  WS_SKETCH3, WS_JPVT2, WS_TAU2, WS_SKETCHP, WS_QGT, WS_ZT, WS_PRT, WS_COUNT
 };
 static_assert(WS_COUNT <= lra_ctx::kSlots, "workspace slots");
 

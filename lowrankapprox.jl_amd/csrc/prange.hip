@@ -45,6 +45,19 @@ int sketchfact_right(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 l
   const i64 P = trans == LRA_TRANS_N ? m : n;
   const bool adap = o->sketchfact_adap || o->rank < 0;
   i64 target = adap ? o->nb : o->rank;
+  // trans n: A Omega reads A m-contiguous, which the k-contiguous GEMM kernels cannot take as
+  // the product's left operand; A' is formed once (even ld) and every round sketches it with
+  // trans c -- the same product A Omega with the same Omega (it depends on its shape only)
+  const double* Au = A;
+  i64 ldu = lda, mu = m, nu = n;
+  int tu = trans;
+  if (trans == LRA_TRANS_N && m > 0 && n > 0) {
+    double* At = nullptr;
+    const i64 ldt = (n + 1) / 2 * 2;
+    LRA_CHECK(lra_wsT(c, WS_PRT, (size_t)ldt * m, &At));
+    LRA_CHECK(lra_launch_transpose(c->stream, m, n, A, lda, At, ldt));
+    Au = At; ldu = ldt; mu = n; nu = m; tu = LRA_TRANS_C;
+  }
   for (i64 rnd = 0;; ++rnd) {
     const i64 ell = o->randn_samp_mul * target + o->randn_samp_add;
     if (ell <= 0) {
@@ -56,7 +69,7 @@ int sketchfact_right(lra_ctx* c, int trans, i64 m, i64 n, const double* A, i64 l
     LRA_CHECK(lra_wsT(c, sb, (size_t)std::max<i64>(P * ell, 1), &B));
     LRA_CHECK(lra_wsT(c, sp, (size_t)ell, &jp));
     LRA_CHECK(lra_wsT(c, st, (size_t)std::max<i64>(std::min(P, ell), 1), &tau));
-    LRA_CHECK(lra_sketch_gaussian(c, LRA_SIDE_RIGHT, trans, m, n, A, lda, ell, nullptr, o->seed,
+    LRA_CHECK(lra_sketch_gaussian(c, LRA_SIDE_RIGHT, tu, mu, nu, Au, ldu, ell, nullptr, o->seed,
                                   stream_base + (uint64_t)rnd, LRA_PREC_F64, B,
                                   std::max<i64>(P, 1)));
     i64 k = 0;
